@@ -1,0 +1,95 @@
+"""CUBE checkpoint files, byte-compatible with main/checkpoint.f90:12-55 and
+main/particle_initialization.f90:121-157.
+
+Files are stream binaries named '{z:f7.3}_{content}_{image}.bin'
+(basic_functions.fh:252-303) inside '{opath}/image{image}/':
+  info : 224-byte sim_header
+  xp   : 3*nplocal components of izipx bytes
+  vp   : 3*nplocal components of izipv bytes
+  np   : rhoc over physical cells, int32, tile-major (nt^3 * nnt^3)
+  vc   : vfield over physical cells, 3 float32 per cell
+  id   : pid, int32 per particle
+"""
+import dataclasses
+import os
+
+import numpy as np
+
+from ._lib import SimHeader
+
+CONTENTS = ("info", "xp", "vp", "np", "vc", "id")
+
+
+def z2str(z):
+    """basic_functions.fh:252-258: write(str,'(f7.3)') z, then trim."""
+    return ("%7.3f" % np.float32(z)).strip()
+
+
+def output_name(opath, z, content, image=1):
+    return os.path.join(opath, "image%d" % image, "%s_%s_%d.bin" % (z2str(z), content, image))
+
+
+@dataclasses.dataclass
+class BoxState:
+    """Host copy of the resident state in checkpoint layout."""
+    sim: SimHeader
+    xp: np.ndarray      # (nplocal, 3) int8/int16
+    vp: np.ndarray      # (nplocal, 3) int8/int16
+    pid: np.ndarray     # (nplocal,) int32
+    rhoc: np.ndarray    # (nc^3,) int32, checkpoint (tile-major) order
+    vfield: np.ndarray  # (nc^3, 3) float32
+
+    def copy(self):
+        s = SimHeader.from_bytes(self.sim.to_bytes())
+        return BoxState(s, self.xp.copy(), self.vp.copy(), self.pid.copy(), self.rhoc.copy(), self.vfield.copy())
+
+
+def xdtype(izip):
+    return np.int8 if izip == 1 else np.int16
+
+
+def write_checkpoint(state, opath, z, image=1):
+    """checkpoint.f90:36-47."""
+    os.makedirs(os.path.join(opath, "image%d" % image), exist_ok=True)
+    n = state.sim.nplocal
+    with open(output_name(opath, z, "info", image), "wb") as f:
+        f.write(state.sim.to_bytes())
+    state.xp[:n].astype(xdtype(state.sim.izipx)).tofile(output_name(opath, z, "xp", image))
+    state.vp[:n].astype(xdtype(state.sim.izipv)).tofile(output_name(opath, z, "vp", image))
+    state.rhoc.astype(np.int32).tofile(output_name(opath, z, "np", image))
+    state.vfield.astype(np.float32).tofile(output_name(opath, z, "vc", image))
+    state.pid[:n].astype(np.int32).tofile(output_name(opath, z, "id", image))
+
+
+def read_checkpoint(opath, z, nc, image=1, izipx=None, izipv=None):
+    """particle_initialization.f90:133-157; a header whose izipx/izipv differ
+    from the requested ones is rejected as in :140-144."""
+    with open(output_name(opath, z, "info", image), "rb") as f:
+        sim = SimHeader.from_bytes(f.read())
+    if (izipx is not None and sim.izipx != izipx) or (izipv is not None and sim.izipv != izipv):
+        raise ValueError("zip format incompatible: file %d/%d, expected %s/%s" % (sim.izipx, sim.izipv, izipx, izipv))
+    n = sim.nplocal
+    xp = np.fromfile(output_name(opath, z, "xp", image), dtype=xdtype(sim.izipx), count=3 * n).reshape(n, 3)
+    vp = np.fromfile(output_name(opath, z, "vp", image), dtype=xdtype(sim.izipv), count=3 * n).reshape(n, 3)
+    rhoc = np.fromfile(output_name(opath, z, "np", image), dtype=np.int32, count=nc ** 3)
+    vfield = np.fromfile(output_name(opath, z, "vc", image), dtype=np.float32, count=3 * nc ** 3).reshape(-1, 3)
+    pid = np.fromfile(output_name(opath, z, "id", image), dtype=np.int32, count=n)
+    if xp.shape[0] != n or vp.shape[0] != n or pid.shape[0] != n or rhoc.shape[0] != nc ** 3:
+        raise ValueError("truncated checkpoint")
+    if int(rhoc.sum(dtype=np.int64)) != n:
+        raise ValueError("sum(np) = %d != nplocal = %d" % (int(rhoc.sum()), n))
+    return BoxState(sim, xp, vp, pid, rhoc, vfield)
+
+
+def read_z_list(path):
+    """initialize.f90:55-61: one redshift per line, format (f8.4); an empty
+    list is an error ('z_checkpoint.txt empty')."""
+    zs = []
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line:
+                zs.append(float(line))
+    if not zs:
+        raise ValueError("%s empty" % os.path.basename(path))
+    return zs

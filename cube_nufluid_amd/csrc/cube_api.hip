@@ -1,0 +1,372 @@
+// C ABI of libcubehip.so: handle lifecycle, state transfer and the step
+// entry points that replace CUBE's update_x / ext_pp_force / particle_mesh.
+#include "cube_internal.hpp"
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+int cube_fail(cube_handle* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  return code;
+}
+
+// ------------------------------------------------------------------ codec
+// Decoding (update_particle.f90:69) evaluates tan in single precision on a
+// finite set of codes, so it is tabulated here with the host libm tanf.
+// Encoding (update_particle.f90:119) is nint(nv1*atan(x)/pi) with a double
+// atan of x = c*v; it is a non-decreasing step function of x, so the host
+// computes, for every code k, the smallest double x with enc(x) >= k by
+// bisection over the ordered bit patterns.  The device evaluates its own
+// atan and corrects the result against these thresholds, which makes its
+// codes equal to the host libm's for every input.
+static inline int32_t enc_host(double x, double nv1) {
+  double y = nv1 * atan(x) / (double)PI_F;
+  return (int32_t)(y >= 0 ? floor(y + 0.5) : -floor(-y + 0.5));
+}
+static inline uint64_t d2key(double x) {
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+static inline double key2d(uint64_t k) {
+  uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double x;
+  memcpy(&x, &u, 8);
+  return x;
+}
+struct CodecTables {
+  std::vector<float> dec;
+  std::vector<double> thr;
+  int32_t kmax = 0;
+};
+static const CodecTables& codec_tables(int izipv) {
+  static CodecTables tabs[3];
+  static std::once_flag once[3];
+  std::call_once(once[izipv], [izipv]() {
+    CodecTables& t = tabs[izipv];
+    const int64_t nvbin = (int64_t)1 << (8 * izipv);
+    const float nv1f = (float)(nvbin - 1);
+    const double nv1 = (double)nv1f;
+    t.dec.resize(nvbin);
+    for (int64_t v = -nvbin / 2; v < nvbin / 2; v++) t.dec[v + nvbin / 2] = tanf((PI_F * (float)v) / nv1f);
+    t.kmax = enc_host(INFINITY, nv1);
+    t.thr.resize(2 * t.kmax);
+    for (int32_t k = -t.kmax + 1; k <= t.kmax; k++) {
+      double x0 = tan(((double)k - 0.5) * (double)PI_F / nv1);
+      double span = fabs(x0) * 1e-9 + 1e-300;
+      double lo = x0 - span, hi = x0 + span;
+      while (enc_host(lo, nv1) >= k) { span *= 16; lo = x0 - span; }
+      while (enc_host(hi, nv1) < k) { span *= 16; hi = x0 + span; }
+      uint64_t a = d2key(lo), b = d2key(hi);  // enc(a) < k <= enc(b)
+      while (b - a > 1) {
+        uint64_t m = a + (b - a) / 2;
+        if (enc_host(key2d(m), nv1) >= k) b = m; else a = m;
+      }
+      t.thr[k + t.kmax - 1] = key2d(b);
+    }
+  });
+  return tabs[izipv];
+}
+int codec_build(cube_handle* h) {
+  const CodecTables& t = codec_tables((int)h->p.izipv);
+  HIP_TRY(h, hipMalloc(&h->vdec, sizeof(float) * t.dec.size()));
+  HIP_TRY(h, hipMalloc(&h->venc, sizeof(double) * t.thr.size()));
+  HIP_TRY(h, hipMemcpy(h->vdec, t.dec.data(), sizeof(float) * t.dec.size(), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->venc, t.thr.data(), sizeof(double) * t.thr.size(), hipMemcpyHostToDevice));
+  h->vkmax = t.kmax;
+  return 0;
+}
+
+// ------------------------------------------------------------ lifecycle
+static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const float* kern_c_table, cube_handle** out) {
+  if (!p || !kern_f_table || !kern_c_table || !out) return CUBE_ERR_ARG;
+  *out = nullptr;
+  cube_handle* h = new cube_handle();
+  h->p = *p;
+  h->nc = p->nc;
+  h->nnt = p->nnt;
+  h->ncb = p->ncb;
+  int rc = 0;
+  auto fail = [&](int code) { *out = h; return code; };
+  if (!is_pow2(p->nc) || p->nc < 16 || p->nc > 512)
+    return fail(cube_fail(h, CUBE_ERR_ARG, "nc=%lld must be a power of two in [16,512]", (long long)p->nc));
+  if (p->nnt < 1 || p->nc % p->nnt)
+    return fail(cube_fail(h, CUBE_ERR_ARG, "nnt=%lld must divide nc=%lld", (long long)p->nnt, (long long)p->nc));
+  h->nt = p->nc / p->nnt;
+  if (h->nt % 8) return fail(cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 8", (long long)h->nt));
+  if (p->ncb < 1 || p->ncb > h->nt)
+    return fail(cube_fail(h, CUBE_ERR_ARG, "ncb=%lld outside [1, nt]", (long long)p->ncb));
+  if ((p->izipx != 1 && p->izipx != 2) || (p->izipv != 1 && p->izipv != 2))
+    return fail(cube_fail(h, CUBE_ERR_ARG, "izipx/izipv must be 1 or 2"));
+  if (p->precision != 4)
+    return fail(cube_fail(h, CUBE_ERR_ARG, "precision=%d: this build runs the FP32 path (4)", p->precision));
+  if (p->np_max < 1) return fail(cube_fail(h, CUBE_ERR_ARG, "np_max must be positive"));
+  h->prec = p->precision;
+  h->nf = h->nc * CUBE_NCELL;
+  h->n3c = h->nc * h->nc * h->nc;
+  if (hipSetDevice(p->device) != hipSuccess) return fail(cube_fail(h, CUBE_ERR_HIP, "hipSetDevice(%d) failed", p->device));
+#define TRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(cube_fail(h, CUBE_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_))); } while (0)
+  TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  const int64_t npm = p->np_max;
+  TRY(hipMalloc(&h->xp, (size_t)3 * npm * p->izipx));
+  TRY(hipMalloc(&h->vp, (size_t)3 * npm * p->izipv));
+  TRY(hipMalloc(&h->pid, sizeof(int32_t) * npm));
+  TRY(hipMalloc(&h->xp2, (size_t)3 * npm * p->izipx));
+  TRY(hipMalloc(&h->vp2, (size_t)3 * npm * p->izipv));
+  TRY(hipMalloc(&h->pid2, sizeof(int32_t) * npm));
+  TRY(hipMalloc(&h->disp, sizeof(uint16_t) * npm));
+  TRY(hipMalloc(&h->rhoc, sizeof(int32_t) * h->n3c));
+  TRY(hipMalloc(&h->rhoc2, sizeof(int32_t) * h->n3c));
+  TRY(hipMalloc(&h->vfield, sizeof(float) * 3 * h->n3c));
+  TRY(hipMalloc(&h->vfield2, sizeof(float) * 3 * h->n3c));
+  TRY(hipMalloc(&h->cstart, sizeof(int64_t) * (h->n3c + 1)));
+  TRY(hipMalloc(&h->cstart2, sizeof(int64_t) * (h->n3c + 1)));
+  TRY(hipMalloc(&h->fc, (size_t)3 * h->n3c * h->prec));
+  h->dred_n = std::max<int64_t>(h->n3c, 1 << 16) + 16;
+  TRY(hipMalloc(&h->dred, sizeof(double) * h->dred_n));
+  TRY(hipMalloc(&h->uflags, sizeof(uint32_t) * 64));
+  TRY(hipMemset(h->uflags, 0, sizeof(uint32_t) * 64));
+#undef TRY
+  if ((rc = codec_build(h))) return fail(rc);
+  if ((rc = mesh_alloc(h, h->fine, h->nf, h->prec, true))) return fail(rc);
+  if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, true))) return fail(rc);
+  if ((rc = fine_kernel_build(h, kern_f_table))) return fail(rc);
+  if ((rc = coarse_kernel_build(h, kern_c_table))) return fail(rc);
+  *out = h;
+  return CUBE_OK;
+}
+
+extern "C" int cube_finalize(cube_handle* h) {
+  if (!h) return CUBE_OK;
+  void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
+                  h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  mesh_free(h->fine);
+  mesh_free(h->coarse);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return CUBE_OK;
+}
+
+extern "C" int cube_last_error(cube_handle* h, char* buf, size_t n) {
+  if (!buf || n == 0) return CUBE_ERR_ARG;
+  const char* s = h ? h->err.c_str() : "";
+  snprintf(buf, n, "%s", s);
+  return CUBE_OK;
+}
+
+// ------------------------------------------------------------------ state
+extern "C" int cube_upload_state(cube_handle* h, const cube_sim_header* sim, const void* xp, const void* vp,
+                                 const int32_t* pid, const int32_t* rhoc, const float* vfield) {
+  if (!h || !sim || !rhoc || !vfield || (sim->nplocal > 0 && (!xp || !vp || !pid))) return CUBE_ERR_ARG;
+  if (sim->izipx != h->p.izipx || sim->izipv != h->p.izipv)
+    return cube_fail(h, CUBE_ERR_FORMAT, "zip format incompatible: header %lld/%lld, handle %lld/%lld",
+                     (long long)sim->izipx, (long long)sim->izipv, (long long)h->p.izipx, (long long)h->p.izipv);
+  if (sim->nplocal < 0 || sim->nplocal > h->p.np_max)
+    return cube_fail(h, CUBE_ERR_CAPACITY, "nplocal=%lld exceeds np_max=%lld", (long long)sim->nplocal, (long long)h->p.np_max);
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  const int64_t np = sim->nplocal;
+  HIP_TRY(h, hipMemcpy(h->xp, xp, (size_t)3 * np * h->p.izipx, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->vp, vp, (size_t)3 * np * h->p.izipv, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->pid, pid, sizeof(int32_t) * np, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->rhoc, rhoc, sizeof(int32_t) * h->n3c, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->vfield, vfield, sizeof(float) * 3 * h->n3c, hipMemcpyHostToDevice));
+  int r = scan_counts(h, h->rhoc, h->n3c, h->cstart);
+  if (r) return r;
+  int64_t tot;
+  HIP_TRY(h, hipMemcpyAsync(&tot, h->cstart + h->n3c, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (tot != np) return cube_fail(h, CUBE_ERR_FORMAT, "sum(rhoc)=%lld != nplocal=%lld", (long long)tot, (long long)np);
+  h->np = np;
+  h->npglobal = sim->npglobal > 0 ? sim->npglobal : np;
+  h->sim = *sim;
+  h->sigma_vi = sim->sigma_vi;
+  h->sigma_vi_new = sim->sigma_vi;
+  h->vz_max = sim->vz_max;
+  h->mass_p = sim->mass_p_cdm;
+  return CUBE_OK;
+}
+
+extern "C" int cube_download_state(cube_handle* h, cube_sim_header* sim, void* xp, void* vp, int32_t* pid, int32_t* rhoc,
+                                   float* vfield) {
+  if (!h) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (xp) HIP_TRY(h, hipMemcpy(xp, h->xp, (size_t)3 * h->np * h->p.izipx, hipMemcpyDeviceToHost));
+  if (vp) HIP_TRY(h, hipMemcpy(vp, h->vp, (size_t)3 * h->np * h->p.izipv, hipMemcpyDeviceToHost));
+  if (pid) HIP_TRY(h, hipMemcpy(pid, h->pid, sizeof(int32_t) * h->np, hipMemcpyDeviceToHost));
+  if (rhoc) HIP_TRY(h, hipMemcpy(rhoc, h->rhoc, sizeof(int32_t) * h->n3c, hipMemcpyDeviceToHost));
+  if (vfield) HIP_TRY(h, hipMemcpy(vfield, h->vfield, sizeof(float) * 3 * h->n3c, hipMemcpyDeviceToHost));
+  if (sim) {
+    *sim = h->sim;
+    sim->nplocal = h->np;
+    sim->npglobal = h->npglobal;
+    sim->sigma_vi = h->sigma_vi;
+    sim->vz_max = h->vz_max;
+    sim->mass_p_cdm = h->mass_p;
+    sim->izipx = h->p.izipx;
+    sim->izipv = h->p.izipv;
+  }
+  return CUBE_OK;
+}
+
+// ------------------------------------------------------------------ steps
+extern "C" int cube_drift(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
+  if (!h) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  return drift_run(h, dt_old, dt, out);
+}
+
+extern "C" int cube_pp(cube_handle* h, float a_mid, float dt, cube_pp_out* out) {
+  if (!h) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  cube_pp_out o{};
+  int r = pp_run(h, a_mid, dt, vcoef(h->sigma_vi), &o, nullptr);
+  if (r) return r;
+  // 'ftemp' in ext_pp_force.f90:18,121 is SAVEd and never reset: dt_pp uses
+  // the running maximum over every tile of every call.
+  h->ftemp_pp = std::max(h->ftemp_pp, o.f2_max_pp);
+  const float GG = 1.0f / 6.0f / PI_F;
+  o.f2_max_pp = h->ftemp_pp;
+  o.dt_pp = 1.0f * sqrtf(1.0f) / (sqrtf(h->ftemp_pp) * a_mid * GG);
+  h->sim.dt_pp = o.dt_pp;
+  if (out) *out = o;
+  if (o.itest1 != h->np)
+    return cube_fail(h, CUBE_ERR_CHECK, "itest1=%lld /= nplocal=%lld", (long long)o.itest1, (long long)h->np);
+  return CUBE_OK;
+}
+
+static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, double* probe_f, double* probe_c) {
+  int r;
+  double mass_c = 0, mass_f = 0;
+  uint32_t* f2c = h->uflags + 32;
+  uint32_t* f2f = h->uflags + 33;
+  uint32_t* vmx = h->uflags + 34;  // 3 entries
+  HIP_TRY(h, hipMemsetAsync(h->uflags + 32, 0, sizeof(uint32_t) * 8, h->stream));
+  // coarse mesh: force_c for all three components (pm.f90:188-326)
+  if ((r = cic_coarse(h, &mass_c))) return r;
+  if ((r = mesh_fwd_xy(h, h->coarse))) return r;
+  for (int d = 0; d < 3; d++) {
+    if ((r = mesh_zconv(h, h->coarse, d))) return r;
+    void* outc = (char*)h->fc + (size_t)d * h->n3c * h->prec;
+    if ((r = mesh_inv_yx(h, h->coarse, outc, d + 1, f2c))) return r;
+  }
+  // fine mesh, one component at a time, each followed by its kick
+  if ((r = cic_fine(h, &mass_f))) return r;
+  if ((r = mesh_fwd_xy(h, h->fine))) return r;
+  const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
+  for (int d = 0; d < 3; d++) {
+    if ((r = mesh_zconv(h, h->fine, d))) return r;
+    if ((r = mesh_inv_yx(h, h->fine, h->fine.W, d + 1, f2f))) return r;
+    if ((r = kick_component(h, d, a_mid, dt, c_old, c_new, vmx + d, probe_f, probe_c))) return r;
+  }
+  uint32_t hv[5];
+  HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 32, sizeof(hv), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  float f[5];
+  memcpy(f, hv, sizeof(f));
+  if (out) {
+    out->f2_max_coarse = f[0];
+    out->f2_max_fine = f[1];
+    out->vmax[0] = f[2]; out->vmax[1] = f[3]; out->vmax[2] = f[4];
+    out->mass_f = mass_f;
+    out->mass_c = mass_c;
+  }
+  return 0;
+}
+
+extern "C" int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out) {
+  if (!h) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  cube_pm_out o{};
+  int r = pm_forces(h, a_mid, dt, &o, nullptr, nullptr);
+  if (r) return r;
+  h->sigma_vi = h->sigma_vi_new;  // pm.f90:180
+  const float GG = 1.0f / 6.0f / PI_F;
+  o.dt_fine = sqrtf(1.0f / (sqrtf(o.f2_max_fine) * a_mid * GG));
+  o.dt_coarse = sqrtf((float)CUBE_NCELL / (sqrtf(o.f2_max_coarse) * a_mid * GG));
+  float vm = std::max(o.vmax[0], std::max(o.vmax[1], o.vmax[2]));
+  o.dt_vmax = 0.9f * 20 / vm;
+  o.vz_max = o.vmax[2];
+  h->vz_max = o.vz_max;
+  h->sim.dt_fine = o.dt_fine;
+  h->sim.dt_coarse = o.dt_coarse;
+  h->sim.dt_vmax = o.dt_vmax;
+  h->sim.vz_max = o.vz_max;
+  if (out) *out = o;
+  return CUBE_OK;
+}
+
+extern "C" int cube_expansion(float a0, float dt0, float omega_m, float omega_l, float wde, float* da1, float* da2) {
+  if (!da1 || !da2) return CUBE_ERR_ARG;
+  float dt_x = dt0 / 2;
+  double a_x = a0, omHsq = 4.0 / 9.0;
+  for (int half = 0; half < 2; half++) {
+    double a3rlm = pow(a_x, -3 * (double)wde) * (double)omega_l / (double)omega_m;
+    double arkm = a_x * (1.0 - (double)omega_m - (double)omega_l) / (double)omega_m;
+    double adot = sqrt(omHsq * a_x * a_x * a_x * (1.0 + arkm + a3rlm));
+    double addot = a_x * a_x * omHsq * (1.5 + 2.0 * arkm + 1.5 * (1.0 - (double)wde) * a3rlm);
+    double atdot = a_x * adot * omHsq * (3.0 + 6.0 * arkm + 1.5 * (2.0 - 3.0 * (double)wde) * (1.0 - (double)wde) * a3rlm);
+    float da = (float)(adot * dt_x + (addot * dt_x * dt_x) / 2.0 + (atdot * dt_x * dt_x * dt_x) / 6.0);
+    if (half == 0) {
+      *da1 = da;
+      a_x = (double)a0 + (double)da;
+    } else {
+      *da2 = da;
+    }
+  }
+  return CUBE_OK;
+}
+
+// ------------------------------------------------------------- probes
+extern "C" int cube_probe_pm_forces(cube_handle* h, double* f_fine, double* f_coarse) {
+  if (!h || !f_fine || !f_coarse) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  double *pf, *pc;
+  HIP_TRY(h, hipMalloc(&pf, sizeof(double) * 3 * std::max<int64_t>(h->np, 1)));
+  HIP_TRY(h, hipMalloc(&pc, sizeof(double) * 3 * std::max<int64_t>(h->np, 1)));
+  int r = pm_forces(h, 1.0f, 1.0f, nullptr, pf, pc);
+  if (!r) {
+    HIP_TRY(h, hipMemcpy(f_fine, pf, sizeof(double) * 3 * h->np, hipMemcpyDeviceToHost));
+    HIP_TRY(h, hipMemcpy(f_coarse, pc, sizeof(double) * 3 * h->np, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(pf);
+  (void)hipFree(pc);
+  return r;
+}
+
+extern "C" int cube_probe_pp_forces(cube_handle* h, double* f_pp) {
+  if (!h || !f_pp) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  double* pf;
+  HIP_TRY(h, hipMalloc(&pf, sizeof(double) * 3 * std::max<int64_t>(h->np, 1)));
+  cube_pp_out o{};
+  int r = pp_run(h, 1.0f, 1.0f, vcoef(h->sigma_vi), &o, pf);
+  if (!r) HIP_TRY(h, hipMemcpy(f_pp, pf, sizeof(double) * 3 * h->np, hipMemcpyDeviceToHost));
+  (void)hipFree(pf);
+  return r;
+}
+
+extern "C" int cube_probe_density(cube_handle* h, float* rho) {
+  if (!h || !rho) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  int r = cic_fine(h, nullptr);
+  if (r) return r;
+  const int64_t N = h->fine.N;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy2D(rho, sizeof(float) * N, h->fine.R, sizeof(float) * (N + 2), sizeof(float) * N, N * N,
+                         hipMemcpyDeviceToHost));
+  return CUBE_OK;
+}

@@ -1,0 +1,182 @@
+// Internal declarations shared by the HIP translation units of libcubehip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "cube_hip.h"
+
+#define CUBE_NCELL 4
+#define CUBE_NF_CUTOFF 16
+
+// pi = 4*atan(1.) in single precision (parameters.f90:67)
+#define PI_F 3.14159274101257324f
+
+// ---------------------------------------------------------------- complex
+template <typename T>
+struct cpx { T x, y; };
+
+template <typename T>
+__host__ __device__ inline cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return cpx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// ---------------------------------------------------------------- mesh
+// One periodic N^3 mesh with its R2C/C2R work buffers and the imaginary
+// part of its force kernel stored over one octant of k-space.
+struct Mesh {
+  int64_t N = 0;
+  int prec = 4;          // sizeof(real)
+  void* R = nullptr;     // real N*N*(N+2) padded rows; complex view N*N*(N/2+1)
+  void* W = nullptr;     // complex work, same size as R
+  float* S = nullptr;    // f2 accumulator, N^3 float
+  void* K = nullptr;     // kernel [3][H][H][H], layout [d][ky'][kx][kz'], scaled 1/N^3
+  void* tw = nullptr;    // N twiddles exp(-2 pi i m/N), type cpx<real>
+  size_t bytes_R() const { return (size_t)N * N * (N + 2) * prec; }
+};
+
+struct cube_handle {
+  cube_params p;
+  int64_t nc, nt, nnt, ncb, nf, n3c;
+  int64_t np = 0, npglobal = 0;
+  int prec;
+  hipStream_t stream = nullptr;
+  // resident particle state (checkpoint order) and its double buffer
+  void* xp = nullptr; void* vp = nullptr; int32_t* pid = nullptr;
+  void* xp2 = nullptr; void* vp2 = nullptr; int32_t* pid2 = nullptr;
+  uint16_t* disp = nullptr;   // packed cell displacement per particle (drift)
+  int32_t* rhoc = nullptr; float* vfield = nullptr; int64_t* cstart = nullptr;
+  int32_t* rhoc2 = nullptr; float* vfield2 = nullptr; int64_t* cstart2 = nullptr;
+  // codec tables
+  float* vdec = nullptr;      // tanf((pi*vp)/(nvbin-1)) for every code
+  double* venc = nullptr;     // encode thresholds (see codec.cpp)
+  int32_t vkmax = 0;          // largest |code| produced by the encoder
+  // meshes
+  Mesh fine, coarse;
+  void* fc = nullptr;         // coarse force [3][nc^3] (real type of prec)
+  // reductions / flags
+  double* dred = nullptr;     // double partials
+  int64_t dred_n = 0;
+  uint32_t* uflags = nullptr; // small device scratch for maxima and flags
+  int64_t* scan_tmp = nullptr;
+  int64_t scan_tmp_n = 0;
+  // scalars mirrored from CUBE's module state
+  float sigma_vi = 0, sigma_vi_new = 0, vz_max = 0, mass_p = 0;
+  float ftemp_pp = 0;         // ext_pp_force.f90:18 running max (SAVE)
+  cube_sim_header sim{};
+  std::string err;
+};
+
+// ------------------------------------------------------------- errors
+int cube_fail(cube_handle* h, int code, const char* fmt, ...);
+#define HIP_TRY(h, call)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return cube_fail((h), CUBE_ERR_HIP, "%s failed: %s (%s:%d)", #call,      \
+                       hipGetErrorString(e_), __FILE__, __LINE__);              \
+  } while (0)
+
+// ------------------------------------------------------------- codec (host)
+int codec_build(cube_handle* h);  // vdec/venc tables from the host libm
+
+// ------------------------------------------------------------- device helpers
+struct Geo {  // box geometry passed by value to kernels
+  int32_t nc, nt, nnt, ncb, izipx, izipv;
+};
+
+__device__ __forceinline__ int32_t wrapc(int32_t a, int32_t n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+// checkpoint-order rank of global coarse cell (0-based coords)
+__device__ __forceinline__ int64_t rank_of(const Geo& g, int32_t x, int32_t y, int32_t z) {
+  int64_t t = ((int64_t)(z / g.nt) * g.nnt + y / g.nt) * g.nnt + x / g.nt;
+  int64_t nt = g.nt;
+  return t * nt * nt * nt + ((int64_t)(z % g.nt) * nt + y % g.nt) * nt + x % g.nt;
+}
+__device__ __forceinline__ void cell_of_rank(const Geo& g, int64_t r, int32_t* c) {
+  int64_t nt3 = (int64_t)g.nt * g.nt * g.nt;
+  int64_t t = r / nt3, l = r - t * nt3;
+  int32_t tx = (int32_t)(t % g.nnt), ty = (int32_t)((t / g.nnt) % g.nnt), tz = (int32_t)(t / ((int64_t)g.nnt * g.nnt));
+  int32_t lx = (int32_t)(l % g.nt), ly = (int32_t)((l / g.nt) % g.nt), lz = (int32_t)(l / ((int64_t)g.nt * g.nt));
+  c[0] = tx * g.nt + lx;
+  c[1] = ty * g.nt + ly;
+  c[2] = tz * g.nt + lz;
+}
+__device__ __forceinline__ uint32_t load_xu(const void* xp, int64_t q, int izipx) {
+  return izipx == 1 ? (uint32_t)(uint8_t)((const int8_t*)xp)[q] : (uint32_t)(uint16_t)((const int16_t*)xp)[q];
+}
+__device__ __forceinline__ int32_t load_v(const void* vp, int64_t q, int izipv) {
+  return izipv == 1 ? (int32_t)((const int8_t*)vp)[q] : (int32_t)((const int16_t*)vp)[q];
+}
+__device__ __forceinline__ void store_v(void* vp, int64_t q, int izipv, int32_t v) {
+  if (izipv == 1) ((int8_t*)vp)[q] = (int8_t)v; else ((int16_t*)vp)[q] = (int16_t)v;
+}
+__device__ __forceinline__ void store_x(void* xp, int64_t q, int izipx, int64_t v) {
+  if (izipx == 1) ((int8_t*)xp)[q] = (int8_t)(uint8_t)(uint64_t)v;
+  else ((int16_t*)xp)[q] = (int16_t)(uint16_t)(uint64_t)v;
+}
+// (int(xp+ishift,izipx)+rshift)*x_resolution == (u+0.5)/2^(8*izipx)
+__device__ __forceinline__ double xoff(uint32_t u, int izipx) {
+  return ((double)u + 0.5) * (izipx == 1 ? (1.0 / 256.0) : (1.0 / 65536.0));
+}
+__device__ __forceinline__ int32_t xsub(uint32_t u, int izipx) {  // fine sub-cell 0..3
+  return izipx == 1 ? (int32_t)(u >> 6) : (int32_t)(u >> 14);
+}
+__device__ __forceinline__ double nint_d(double y) {
+  return y >= 0 ? floor(y + 0.5) : -floor(-y + 0.5);
+}
+// velocity decode: tan table (exact host tanf) then the double divide
+__device__ __forceinline__ double vdecode(const float* tab, int32_t v, int izipv, double c) {
+  int32_t half = izipv == 1 ? 128 : 32768;
+  return (double)tab[v + half] / c;
+}
+// velocity encode with host-libm-exact thresholds.  thr[k + kmax - 1] is the
+// smallest x with enc(x) >= k, for k in [-kmax+1, kmax].
+__device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int izipv, double c, double v) {
+  const double nv1 = izipv == 1 ? 255.0 : 65535.0;
+  double x = c * v;
+  double y = nv1 * atan(x) / (double)PI_F;
+  int32_t n = (int32_t)nint_d(y);
+  if (n > kmax) n = kmax;
+  if (n < -kmax) n = -kmax;
+  for (int it = 0; it < 4; it++) {
+    if (n > -kmax && x < thr[n + kmax - 1]) { n--; continue; }
+    if (n < kmax && x >= thr[n + kmax]) { n++; continue; }
+    break;
+  }
+  return n;
+}
+
+// ------------------------------------------------------------- launchers
+// scan.hip
+int scan_counts(cube_handle* h, const int32_t* cnt, int64_t n, int64_t* cstart);
+int reduce_doubles(cube_handle* h, const double* part, int64_t nparts, int nfields, double* host_out);
+// fft.hip
+int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S);
+void mesh_free(Mesh& m);
+int mesh_fwd_xy(cube_handle* h, Mesh& m);                  // R: X r2c then Y, in place
+int mesh_zconv(cube_handle* h, Mesh& m, int d);              // W = Zinv(i*K_d * Zfwd(R))
+int mesh_inv_yx(cube_handle* h, Mesh& m, void* out, int smode, uint32_t* f2max); // W: Y inv, X c2r -> out
+int mesh_fft3_r2c(cube_handle* h, Mesh& m);                  // full forward on R (for kernel build)
+int fine_kernel_build(cube_handle* h, const float* table);
+int coarse_kernel_build(cube_handle* h, const float* table);
+// pm.hip
+int cic_fine(cube_handle* h, double* mass);
+int cic_coarse(cube_handle* h, double* mass);
+int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, double c_new,
+                   uint32_t* vmax_bits, double* probe_f, double* probe_c);
+// pp.hip
+int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe);
+// drift.hip
+int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
+
+static inline Geo geo_of(const cube_handle* h) {
+  Geo g;
+  g.nc = (int32_t)h->nc; g.nt = (int32_t)h->nt; g.nnt = (int32_t)h->nnt; g.ncb = (int32_t)h->ncb;
+  g.izipx = (int32_t)h->p.izipx; g.izipv = (int32_t)h->p.izipv;
+  return g;
+}
+static inline double vcoef(float sigma) {  // sqrt(pi/2)/(sigma*vrel_boost)
+  return (double)sqrtf(PI_F / 2) / ((double)sigma * 2.5);
+}

@@ -1,0 +1,579 @@
+// Periodic N^3 mesh transforms for the PM force convolution.
+//
+// The reference convolves the CIC density with tabulated vector force
+// kernels through FFTW single-precision R2C/C2R transforms
+// (pm.f90:112-125 per nfe^3 tile, pm.f90:288-300 / pencil_fft.f90 for the
+// nc^3 coarse mesh).  Here every mesh is one periodic N^3 box in HBM and
+// the transforms are three batched 1-D passes staged through LDS:
+//   X : real rows  <-> N/2+1 complex (R2C/C2R via an N/2-point complex FFT)
+//   Y : columns of stride N/2+1
+//   Z : columns of stride N(N/2+1); the convolution multiply i*K_d and the
+//       inverse Z transform are fused into the forward Z pass.
+// Each workgroup transforms W adjacent columns; the 1-D FFT is a Stockham
+// autosort with radix-8/4/2 butterflies held in registers and LDS used only
+// for the exchange between passes.
+#include "cube_internal.hpp"
+#include <math.h>
+#include <vector>
+
+#define NT_FFT 256
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return cpx<T>{a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return cpx<T>{a.x - b.x, a.y - b.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> mul_mi(cpx<T> a) { return INV ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x}; }
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_small(cpx<T>* v) {
+  if constexpr (R == 2) {
+    cpx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    cpx<T> c0 = cadd(v[0], v[2]), c1 = cadd(v[1], v[3]);
+    cpx<T> d0 = csub(v[0], v[2]), d1 = mul_mi<T, INV>(csub(v[1], v[3]));
+    v[0] = cadd(c0, c1);
+    v[2] = csub(c0, c1);
+    v[1] = cadd(d0, d1);
+    v[3] = csub(d0, d1);
+  } else {  // R == 8, decimation in frequency into two DFT-4
+    const T h = (T)0.70710678118654752440;
+    cpx<T> a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a[k] = cadd(v[k], v[k + 4]);
+      b[k] = csub(v[k], v[k + 4]);
+    }
+    // b_k *= w8^k, w8 = exp(-+2 pi i/8)
+    b[1] = INV ? cpx<T>{h * (b[1].x - b[1].y), h * (b[1].x + b[1].y)}
+               : cpx<T>{h * (b[1].x + b[1].y), h * (b[1].y - b[1].x)};
+    b[2] = mul_mi<T, INV>(b[2]);
+    b[3] = INV ? cpx<T>{-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y)}
+               : cpx<T>{h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y)};
+    dft_small<T, 4, INV>(a);
+    dft_small<T, 4, INV>(b);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      v[2 * m] = a[m];
+      v[2 * m + 1] = b[m];
+    }
+  }
+}
+
+// One Stockham pass over all W columns held in s[n*W + w].
+template <typename T, int N, int W, bool INV, int R, int NS>
+__device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ tw, int tws) {
+  constexpr int B = N / R;
+  constexpr int TOT = B * W;
+  constexpr int PER = (TOT + NT_FFT - 1) / NT_FFT;
+  cpx<T> v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int b = tid + q * NT_FFT;
+    if (TOT % NT_FFT == 0 || b < TOT) {
+      int w = b % W, j = b / W;
+      int k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        cpx<T> a = s[(j + r * B) * W + w];
+        if (NS > 1 && r > 0) {
+          cpx<T> t = tw[(k * r * (N / (NS * R))) * tws];
+          if (INV) t.y = -t.y;
+          a = cmul(a, t);
+        }
+        v[q][r] = a;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int b = tid + q * NT_FFT;
+    if (TOT % NT_FFT == 0 || b < TOT) {
+      dft_small<T, R, INV>(v[q]);
+      int w = b % W, j = b / W;
+      int idxD = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int r = 0; r < R; r++) s[(idxD + r * NS) * W + w] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int W, bool INV, int NS>
+struct FftPasses {
+  __device__ __forceinline__ static void run(cpx<T>* s, const cpx<T>* tw, int tws) {
+    if constexpr (NS < N) {
+      constexpr int REM = N / NS;
+      constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
+      fft_pass<T, N, W, INV, R, NS>(s, tw, tws);
+      FftPasses<T, N, W, INV, NS * R>::run(s, tw, tws);
+    }
+  }
+};
+// In-LDS FFT of W columns of length N (twiddles tw[m*tws] = exp(-2 pi i m/N)).
+template <typename T, int N, int W, bool INV>
+__device__ __forceinline__ void lds_fft(cpx<T>* s, const cpx<T>* tw, int tws) {
+  FftPasses<T, N, W, INV, 1>::run(s, tw, tws);
+}
+
+template <typename T>
+constexpr int pick_w(int logn) {
+  int e = sizeof(T) == 4 ? 8192 : 4096;
+  int w = e >> logn;
+  return w < 1 ? 1 : (w > 64 ? 64 : w);
+}
+
+// ------------------------------------------------------------------ X R2C
+// rows of N reals (pitch N+2) -> N/2+1 complex, in place.  nrows = N*N.
+template <typename T, int LOGN>
+__global__ __launch_bounds__(NT_FFT) void k_fft_x_r2c(cpx<T>* data, const cpx<T>* __restrict__ tw, int64_t nrows) {
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1);
+  __shared__ cpx<T> s[H * W];
+  const int64_t r0 = (int64_t)blockIdx.x * W;
+  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
+    int w = idx / H, m = idx % H;
+    int64_t row = r0 + w;
+    s[m * W + w] = row < nrows ? data[row * (H + 1) + m] : cpx<T>{0, 0};
+  }
+  __syncthreads();
+  lds_fft<T, H, W, false>(s, tw, 2);
+  for (int idx = threadIdx.x; idx < (H + 1) * W; idx += NT_FFT) {
+    int w = idx / (H + 1), k = idx % (H + 1);
+    int64_t row = r0 + w;
+    cpx<T> zk = s[(k % H) * W + w];
+    cpx<T> zc = s[((H - k) % H) * W + w];
+    zc.y = -zc.y;
+    cpx<T> e{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
+    cpx<T> dd = csub(zk, zc);
+    cpx<T> o{dd.y * (T)0.5, -dd.x * (T)0.5};
+    cpx<T> X = cadd(e, cmul(tw[k], o));
+    if (row < nrows) data[row * (H + 1) + k] = X;
+  }
+}
+
+// ------------------------------------------------------------------ X C2R
+// rows of N/2+1 complex (src) -> N reals at dst[row*opitch + x].  smode:
+// 0 none, 1 S=f*f, 2 S=S+f*f, 3 max(S+f*f) into *f2max (float bits).
+template <typename T, int LOGN>
+__global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst, int64_t opitch,
+                                                       const cpx<T>* __restrict__ tw, int64_t nrows,
+                                                       int smode, float* S, uint32_t* f2max) {
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1);
+  __shared__ cpx<T> xs[(H + 1) * W];
+  __shared__ cpx<T> s[H * W];
+  __shared__ float red[NT_FFT / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * W;
+  for (int idx = threadIdx.x; idx < (H + 1) * W; idx += NT_FFT) {
+    int w = idx / (H + 1), k = idx % (H + 1);
+    int64_t row = r0 + w;
+    xs[k * W + w] = row < nrows ? src[row * (H + 1) + k] : cpx<T>{0, 0};
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
+    int w = idx / H, k = idx % H;
+    cpx<T> a = xs[k * W + w];
+    cpx<T> b = xs[(H - k) * W + w];
+    b.y = -b.y;
+    cpx<T> sum = cadd(a, b), dif = csub(a, b);
+    cpx<T> t = tw[k];
+    t.y = -t.y;                                // exp(+2 pi i k/N)
+    cpx<T> td = cmul(t, dif);
+    s[k * W + w] = cpx<T>{sum.x - td.y, sum.y + td.x};  // sum + i*t*dif
+  }
+  __syncthreads();
+  lds_fft<T, H, W, true>(s, tw, 2);
+  float mymax = 0.f;
+  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
+    int w = idx / H, m = idx % H;
+    int64_t row = r0 + w;
+    if (row >= nrows) continue;
+    cpx<T> z = s[m * W + w];
+    *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
+    if (smode) {
+      float f0 = (float)z.x, f1 = (float)z.y;
+      float* sp = S + row * N + 2 * m;
+      if (smode == 1) {
+        sp[0] = f0 * f0; sp[1] = f1 * f1;
+      } else if (smode == 2) {
+        sp[0] = sp[0] + f0 * f0; sp[1] = sp[1] + f1 * f1;
+      } else {
+        float a = sp[0] + f0 * f0, b = sp[1] + f1 * f1;
+        mymax = fmaxf(mymax, fmaxf(a, b));
+      }
+    }
+  }
+  if (smode == 3) {
+    for (int off = 32; off > 0; off >>= 1) mymax = fmaxf(mymax, __shfl_xor(mymax, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mymax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int i = 1; i < NT_FFT / 64; i++) m = fmaxf(m, red[i]);
+      atomicMax(f2max, __float_as_uint(m));
+    }
+  }
+}
+
+// ----------------------------------------------------------- strided Y / Z
+// AXIS 1: columns along y (stride H+1) in plane blockIdx.y = z.
+// AXIS 2: columns along z (stride N(H+1)) at blockIdx.y = y.
+template <typename T, int LOGN, int AXIS, bool INV>
+__global__ __launch_bounds__(NT_FFT) void k_fft_strided(cpx<T>* data, const cpx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, KX = N / 2 + 1, W = pick_w<T>(LOGN);
+  __shared__ cpx<T> s[N * W];
+  const int kx0 = blockIdx.x * W;
+  const int64_t stride = AXIS == 1 ? (int64_t)KX : (int64_t)N * KX;
+  const int64_t base = AXIS == 1 ? (int64_t)blockIdx.y * N * KX + kx0 : (int64_t)blockIdx.y * KX + kx0;
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    int n = idx / W, w = idx % W;
+    s[idx] = (kx0 + w < KX) ? data[base + n * stride + w] : cpx<T>{0, 0};
+  }
+  __syncthreads();
+  lds_fft<T, N, W, INV>(s, tw, 1);
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    int n = idx / W, w = idx % W;
+    if (kx0 + w < KX) data[base + n * stride + w] = s[idx];
+  }
+}
+
+// ------------------------------------------------------ fused Z convolution
+// dst = Zinv( i*K_d . Zfwd(src) ) along z for columns (kx0.., ky=blockIdx.y).
+// K is the imaginary part of the kernel transform over one octant,
+// [d][ky'][kx][kz'] with ky'=min(ky,N-ky), kz'=min(kz,N-kz); component d is
+// odd in its own axis and even in the other two.
+template <typename T, int LOGN>
+__global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>* dst, const T* __restrict__ K,
+                                                       int d, const cpx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, KX = N / 2 + 1, H = N / 2 + 1, W = pick_w<T>(LOGN);
+  __shared__ cpx<T> s[N * W];
+  const int kx0 = blockIdx.x * W;
+  const int ky = blockIdx.y;
+  const int64_t stride = (int64_t)N * KX;
+  const int64_t base = (int64_t)ky * KX + kx0;
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    int n = idx / W, w = idx % W;
+    s[idx] = (kx0 + w < KX) ? src[base + n * stride + w] : cpx<T>{0, 0};
+  }
+  __syncthreads();
+  lds_fft<T, N, W, false>(s, tw, 1);
+  const int kyp = ky <= N / 2 ? ky : N - ky;
+  const T sy = (d == 1 && ky > N / 2) ? (T)-1 : (T)1;
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    int w = idx % W, n = idx / W;
+    int kx = kx0 + w;
+    if (kx >= KX) continue;
+    int kzp = n <= N / 2 ? n : N - n;
+    T sz = (d == 2 && n > N / 2) ? (T)-1 : (T)1;
+    T k = K[(((int64_t)d * H + kyp) * KX + kx) * H + kzp] * sy * sz;
+    cpx<T> a = s[idx];
+    s[idx] = cpx<T>{-a.y * k, a.x * k};
+  }
+  __syncthreads();
+  lds_fft<T, N, W, true>(s, tw, 1);
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    int n = idx / W, w = idx % W;
+    if (kx0 + w < KX) dst[base + n * stride + w] = s[idx];
+  }
+}
+
+// ------------------------------------------------------------ dispatchers
+#define LOGN_SWITCH(logn, ...)             \
+  switch (logn) {                           \
+    case 4: { constexpr int L = 4; __VA_ARGS__; } break;   \
+    case 5: { constexpr int L = 5; __VA_ARGS__; } break;   \
+    case 6: { constexpr int L = 6; __VA_ARGS__; } break;   \
+    case 7: { constexpr int L = 7; __VA_ARGS__; } break;   \
+    case 8: { constexpr int L = 8; __VA_ARGS__; } break;   \
+    case 9: { constexpr int L = 9; __VA_ARGS__; } break;   \
+    case 10: { constexpr int L = 10; __VA_ARGS__; } break; \
+    case 11: { constexpr int L = 11; __VA_ARGS__; } break; \
+    default: return cube_fail(h, CUBE_ERR_ARG, "mesh size 2^%d unsupported", logn); \
+  }
+
+static int ilog2(int64_t n) { int l = 0; while ((1LL << l) < n) l++; return l; }
+
+template <typename T>
+static int fwd_xy_t(cube_handle* h, Mesh& m) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N, nrows = N * N;
+  cpx<T>* c = (cpx<T>*)m.R;
+  const cpx<T>* tw = (const cpx<T>*)m.tw;
+  LOGN_SWITCH(logn, {
+    constexpr int WX = pick_w<T>(L - 1);
+    k_fft_x_r2c<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(c, tw, nrows);
+    constexpr int WS = pick_w<T>(L);
+    k_fft_strided<T, L, 1, false><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(c, tw);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_fwd_xy(cube_handle* h, Mesh& m) {
+  return m.prec == 4 ? fwd_xy_t<float>(h, m) : fwd_xy_t<double>(h, m);
+}
+
+template <typename T>
+static int fft3_r2c_t(cube_handle* h, Mesh& m) {
+  int r = fwd_xy_t<T>(h, m);
+  if (r) return r;
+  int logn = ilog2(m.N);
+  int64_t N = m.N;
+  cpx<T>* c = (cpx<T>*)m.R;
+  const cpx<T>* tw = (const cpx<T>*)m.tw;
+  LOGN_SWITCH(logn, {
+    constexpr int WS = pick_w<T>(L);
+    k_fft_strided<T, L, 2, false><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(c, tw);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_fft3_r2c(cube_handle* h, Mesh& m) {
+  return m.prec == 4 ? fft3_r2c_t<float>(h, m) : fft3_r2c_t<double>(h, m);
+}
+
+template <typename T>
+static int zconv_t(cube_handle* h, Mesh& m, int d) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N;
+  LOGN_SWITCH(logn, {
+    constexpr int WS = pick_w<T>(L);
+    k_fft_zconv<T, L><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(
+        (const cpx<T>*)m.R, (cpx<T>*)m.W, (const T*)m.K, d, (const cpx<T>*)m.tw);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_zconv(cube_handle* h, Mesh& m, int d) {
+  return m.prec == 4 ? zconv_t<float>(h, m, d) : zconv_t<double>(h, m, d);
+}
+
+template <typename T>
+static int inv_yx_t(cube_handle* h, Mesh& m, void* out, int64_t opitch, int smode, uint32_t* f2max) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N, nrows = N * N;
+  cpx<T>* c = (cpx<T>*)m.W;
+  const cpx<T>* tw = (const cpx<T>*)m.tw;
+  LOGN_SWITCH(logn, {
+    constexpr int WS = pick_w<T>(L);
+    k_fft_strided<T, L, 1, true><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(c, tw);
+    constexpr int WX = pick_w<T>(L - 1);
+    k_fft_x_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
+        c, (T*)out, opitch, tw, nrows, smode, m.S, f2max);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+// Y inverse on W then C2R into out (pitch N+2 when out==W, N otherwise).
+int mesh_inv_yx(cube_handle* h, Mesh& m, void* out, int smode, uint32_t* f2max) {
+  int64_t opitch = (out == m.W) ? m.N + 2 : m.N;
+  return m.prec == 4 ? inv_yx_t<float>(h, m, out, opitch, smode, f2max)
+                     : inv_yx_t<double>(h, m, out, opitch, smode, f2max);
+}
+
+// -------------------------------------------------------------- allocation
+int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S) {
+  m.N = N;
+  m.prec = prec;
+  int64_t H = N / 2 + 1;
+  HIP_TRY(h, hipMalloc(&m.R, m.bytes_R()));
+  HIP_TRY(h, hipMalloc(&m.W, m.bytes_R()));
+  if (with_S) HIP_TRY(h, hipMalloc(&m.S, sizeof(float) * N * N * N));
+  HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * H * H * prec));
+  HIP_TRY(h, hipMalloc(&m.tw, (size_t)N * 2 * prec));
+  std::vector<double> tw(2 * N);
+  for (int64_t j = 0; j < N; j++) {
+    double a = -2.0 * M_PI * (double)j / (double)N;
+    tw[2 * j] = cos(a);
+    tw[2 * j + 1] = sin(a);
+  }
+  if (prec == 8) {
+    HIP_TRY(h, hipMemcpy(m.tw, tw.data(), sizeof(double) * 2 * N, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> twf(2 * N);
+    for (int64_t j = 0; j < 2 * N; j++) twf[j] = (float)tw[j];
+    HIP_TRY(h, hipMemcpy(m.tw, twf.data(), sizeof(float) * 2 * N, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+void mesh_free(Mesh& m) {
+  if (m.R) (void)hipFree(m.R);
+  if (m.W) (void)hipFree(m.W);
+  if (m.S) (void)hipFree(m.S);
+  if (m.K) (void)hipFree(m.K);
+  if (m.tw) (void)hipFree(m.tw);
+  m = Mesh();
+}
+
+// ---------------------------------------------------------- fine kernel
+// Im of the DFT of the tabulated fine kernel (kernel_f.f90:14-37) evaluated
+// directly on the N^3 mesh as three separable 16-term contractions in FP64:
+//   Im K_d(k) = -sum_a prod_e g_de(k_e, a_e) K_d(a),  a in [0,16)^3,
+//   g_de = 2 sin(2 pi k a/N) for e == d, w(a) cos(2 pi k a/N) otherwise
+//   (w(0)=1, w(a>0)=2), which is the odd/even mirror of the table.
+__device__ __forceinline__ double g_fac(const double* cs, const double* sn, int64_t N, int e, int d, int k, int a) {
+  int64_t m = ((int64_t)k * a) % N;
+  if (e == d) return 2.0 * sn[m];
+  return (a == 0 ? 1.0 : 2.0) * cs[m];
+}
+__global__ void k_kf_stage1(const float* __restrict__ tab, int pp_on, int pp_range, const double* cs,
+                            const double* sn, int64_t N, int64_t H, double* A) {
+  // A[d][kx][y][z] = sum_x K_d(x,y,z) g_d0(kx,x)
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t tot = 3 * H * 256;
+  if (i >= tot) return;
+  int z = (int)(i % 16), y = (int)((i / 16) % 16);
+  int64_t kx = (i / 256) % H;
+  int d = (int)(i / (256 * H));
+  double s = 0;
+  for (int x = 0; x < 16; x++) {
+    double kv = tab[((z * 16 + y) * 16 + x) * 3 + d];
+    if (pp_on && x <= pp_range && y <= pp_range && z <= pp_range) kv = 0;
+    s += kv * g_fac(cs, sn, N, 0, d, (int)kx, x);
+  }
+  A[i] = s;
+}
+__global__ void k_kf_stage2(const double* A, const double* cs, const double* sn, int64_t N, int64_t H, double* B) {
+  // B[d][kx][ky][z] = sum_y A[d][kx][y][z] g_d1(ky,y)
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t tot = 3 * H * H * 16;
+  if (i >= tot) return;
+  int z = (int)(i % 16);
+  int64_t ky = (i / 16) % H, kx = (i / (16 * H)) % H;
+  int d = (int)(i / (16 * H * H));
+  double s = 0;
+  for (int y = 0; y < 16; y++) s += A[((d * H + kx) * 16 + y) * 16 + z] * g_fac(cs, sn, N, 1, d, (int)ky, y);
+  B[i] = s;
+}
+template <typename T>
+__global__ void k_kf_stage3(const double* B, const double* cs, const double* sn, int64_t N, int64_t H, double scale, T* K) {
+  // K[d][ky][kx][kz] = -scale * sum_z B[d][kx][ky][z] g_d2(kz,z)
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t tot = 3 * H * H * H;
+  if (i >= tot) return;
+  int64_t kz = i % H, kx = (i / H) % H, ky = (i / (H * H)) % H;
+  int d = (int)(i / (H * H * H));
+  const double* b = B + ((d * H + kx) * H + ky) * 16;
+  double s = 0;
+  for (int z = 0; z < 16; z++) s += b[z] * g_fac(cs, sn, N, 2, d, (int)kz, z);
+  K[i] = (T)(-scale * s);
+}
+
+int fine_kernel_build(cube_handle* h, const float* table) {
+  Mesh& m = h->fine;
+  int64_t N = m.N, H = N / 2 + 1;
+  std::vector<double> cs(N), sn(N);
+  for (int64_t j = 0; j < N; j++) {
+    cs[j] = cos(2.0 * M_PI * (double)j / (double)N);
+    sn[j] = sin(2.0 * M_PI * (double)j / (double)N);
+  }
+  float* dtab; double *dcs, *dsn, *A, *B;
+  HIP_TRY(h, hipMalloc(&dtab, sizeof(float) * 4096 * 3));
+  HIP_TRY(h, hipMalloc(&dcs, sizeof(double) * N));
+  HIP_TRY(h, hipMalloc(&dsn, sizeof(double) * N));
+  HIP_TRY(h, hipMalloc(&A, sizeof(double) * 3 * H * 256));
+  HIP_TRY(h, hipMalloc(&B, sizeof(double) * 3 * H * H * 16));
+  HIP_TRY(h, hipMemcpy(dtab, table, sizeof(float) * 4096 * 3, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(dcs, cs.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(dsn, sn.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  int64_t t1 = 3 * H * 256, t2 = 3 * H * H * 16, t3 = 3 * H * H * H;
+  k_kf_stage1<<<(unsigned)((t1 + 255) / 256), 256, 0, h->stream>>>(dtab, h->p.pp_on, h->p.pp_range, dcs, dsn, N, H, A);
+  k_kf_stage2<<<(unsigned)((t2 + 255) / 256), 256, 0, h->stream>>>(A, dcs, dsn, N, H, B);
+  double scale = 1.0 / ((double)N * N * N);
+  if (m.prec == 4)
+    k_kf_stage3<float><<<(unsigned)((t3 + 255) / 256), 256, 0, h->stream>>>(B, dcs, dsn, N, H, scale, (float*)m.K);
+  else
+    k_kf_stage3<double><<<(unsigned)((t3 + 255) / 256), 256, 0, h->stream>>>(B, dcs, dsn, N, H, scale, (double*)m.K);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(dtab); (void)hipFree(dcs); (void)hipFree(dsn); (void)hipFree(A); (void)hipFree(B);
+  return 0;
+}
+
+// ---------------------------------------------------------- coarse kernel
+// kernel_c.f90:16-117 with one image: ck = -r/r^3 (r in fine-cell units),
+// the 4^3 table of wfxyzc.2.ascii copied into the eight corner octants with
+// odd/even signs, R2C in FP64, then LRCKCORR for |k|<=8, k_d != 0.
+__global__ void k_ck_fill(double* R, int64_t N, int d, int corrected, const float* __restrict__ tab) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= N * N * N) return;
+  int64_t x = i % N, y = (i / N) % N, z = i / (N * N);
+  int64_t g[3] = {x, y, z};
+  double rx[3];
+  for (int e = 0; e < 3; e++) rx[e] = (double)(CUBE_NCELL * (((g[e] + N / 2) % N) - N / 2));
+  double r = sqrt(rx[0] * rx[0] + rx[1] * rx[1] + rx[2] * rx[2]);
+  double v = r == 0 ? 0.0 : -rx[d] / (r * r * r);
+  if (corrected) {
+    int t[3];
+    bool mir[3], in = true;
+    for (int e = 0; e < 3; e++) {
+      if (g[e] <= 3) { t[e] = (int)g[e]; mir[e] = false; }
+      else if (g[e] >= N - 3) { t[e] = (int)(N - g[e]); mir[e] = true; }
+      else in = false;
+    }
+    if (in) v = (mir[d] ? -1.0 : 1.0) * (double)tab[((t[2] * 16 + t[1] * 4 + t[0]) * 3) + d];
+  }
+  R[(z * N + y) * (N + 2) + x] = v;
+}
+// octant extraction of Im of the R2C output
+__global__ void k_ck_extract(const double* C, int64_t N, double* out) {
+  int64_t H = N / 2 + 1;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= H * H * H) return;
+  int64_t kz = i % H, kx = (i / H) % H, ky = i / (H * H);
+  out[i] = C[((kz * N + ky) * H + kx) * 2 + 1];
+}
+template <typename T>
+__global__ void k_ck_correct(const double* kc, const double* ku, int64_t N, int d, T* K) {
+  int64_t H = N / 2 + 1;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= H * H * H) return;
+  int64_t kz = i % H, kx = (i / H) % H, ky = i / (H * H);
+  int64_t kk[3] = {kx, ky, kz};
+  double kv[3], ks[3];
+  for (int e = 0; e < 3; e++) {
+    kv[e] = (double)(((kk[e] + N / 2) % N) - N / 2);  // mod(i+nc/2-1,nc)-nc/2 with 1-based i
+    ks[e] = 2.0 * sin(M_PI * kv[e] / (double)N);
+  }
+  double kr = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+  double v = kc[i];
+  if (!(kr > 8.0 || kv[d] == 0)) {
+    if (kk[d] == N / 2) v = 0.0;  // odd component at its own Nyquist plane: exactly zero
+    else v = v * 0.25 * M_PI * ks[d] / (ks[0] * ks[0] + ks[1] * ks[1] + ks[2] * ks[2]) / ku[i];
+  }
+  K[(int64_t)d * H * H * H + i] = (T)(v / ((double)N * N * N));
+}
+
+int coarse_kernel_build(cube_handle* h, const float* table) {
+  Mesh tmp;
+  int64_t N = h->coarse.N, H = N / 2 + 1, H3 = H * H * H;
+  int r = mesh_alloc(h, tmp, N, 8, false);
+  if (r) return r;
+  float* dtab;
+  double *kc, *ku;
+  HIP_TRY(h, hipMalloc(&dtab, sizeof(float) * 64 * 3));
+  HIP_TRY(h, hipMalloc(&kc, sizeof(double) * H3));
+  HIP_TRY(h, hipMalloc(&ku, sizeof(double) * H3));
+  HIP_TRY(h, hipMemcpy(dtab, table, sizeof(float) * 64 * 3, hipMemcpyHostToDevice));
+  unsigned gfill = (unsigned)((N * N * N + 255) / 256), gext = (unsigned)((H3 + 255) / 256);
+  for (int d = 0; d < 3; d++) {
+    k_ck_fill<<<gfill, 256, 0, h->stream>>>((double*)tmp.R, N, d, 1, dtab);
+    if ((r = mesh_fft3_r2c(h, tmp))) return r;
+    k_ck_extract<<<gext, 256, 0, h->stream>>>((const double*)tmp.R, N, kc);
+    k_ck_fill<<<gfill, 256, 0, h->stream>>>((double*)tmp.R, N, d, 0, dtab);
+    if ((r = mesh_fft3_r2c(h, tmp))) return r;
+    k_ck_extract<<<gext, 256, 0, h->stream>>>((const double*)tmp.R, N, ku);
+    if (h->coarse.prec == 4)
+      k_ck_correct<float><<<gext, 256, 0, h->stream>>>(kc, ku, N, d, (float*)h->coarse.K);
+    else
+      k_ck_correct<double><<<gext, 256, 0, h->stream>>>(kc, ku, N, d, (double*)h->coarse.K);
+    HIP_TRY(h, hipGetLastError());
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(dtab); (void)hipFree(kc); (void)hipFree(ku);
+  mesh_free(tmp);
+  return 0;
+}

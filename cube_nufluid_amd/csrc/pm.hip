@@ -1,0 +1,326 @@
+// Particle <-> mesh kernels of particle_mesh (pm.f90): CIC mass assignment on
+// the fine and coarse meshes and the per-component kick that applies the
+// fine kick (pm.f90:131-156) and the coarse kick (pm.f90:335-372) with their
+// intermediate re-quantisation.
+//
+// Coordinates follow the reference's tile frames: positions are formed in
+// double relative to the tile that owns the destination node and rounded to
+// float there (pm.f90:87, 138, 255, 348), so every CIC weight equals the
+// reference's weight for the same particle.
+#include "cube_internal.hpp"
+
+#define CIC_NT 512
+#define KICK_NT 256
+#define KICK_CH 256
+
+// --------------------------------------------------------------- helpers
+// Load cstart for the cells of a (BX+2*lo)^3 region around block origin c0
+// into LDS as per-cell begin/offset arrays (one segment per cell).
+template <int SPAN>
+__device__ void load_region(const Geo& g, const int64_t* cstart, int c0x, int c0y, int c0z, int lo,
+                            int64_t* beg, int32_t* off) {
+  constexpr int NCELL = SPAN * SPAN * SPAN;
+  for (int i = threadIdx.x; i < NCELL; i += blockDim.x) {
+    int rx = i % SPAN - lo, ry = (i / SPAN) % SPAN - lo, rz = i / (SPAN * SPAN) - lo;
+    int64_t r = rank_of(g, wrapc(c0x + rx, g.nc), wrapc(c0y + ry, g.nc), wrapc(c0z + rz, g.nc));
+    int64_t b = cstart[r];
+    beg[i] = b;
+    off[i + 1] = (int32_t)(cstart[r + 1] - b);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    off[0] = 0;
+    for (int i = 0; i < NCELL; i++) off[i + 1] += off[i];
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ int seg_find(const int32_t* off, int n, int32_t t) {
+  int lo = 0, hi = n;  // off[lo] <= t < off[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------- fine CIC
+// One workgroup per BC^3 coarse cells = (4BC)^3 fine nodes, binned in LDS.
+// Nodes sit on fine-cell left edges (pm.f90:87 has no -0.5); a particle in
+// coarse cell c touches fine nodes 4c..4c+4, so the block gathers cells
+// [c0-1, c0+BC) and keeps taps on its own nodes: every node is written once.
+template <int BC>
+__global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
+                                                     float mass, float* R, int64_t N, double* part) {
+  constexpr int F = 4 * BC, SPAN = BC + 1, NCELL = SPAN * SPAN * SPAN;
+  __shared__ float s[F * F * F];
+  __shared__ int64_t beg[NCELL];
+  __shared__ int32_t off[NCELL + 1];
+  __shared__ double red[CIC_NT / 64];
+  const int nbx = g.nc / BC;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
+  const int c0x = bx * BC, c0y = by * BC, c0z = bz * BC;
+  // block-local tile frame: 0-based offset of c0 inside its tile
+  const int l0x = c0x % g.nt, l0y = c0y % g.nt, l0z = c0z % g.nt;
+  for (int i = threadIdx.x; i < F * F * F; i += CIC_NT) s[i] = 0.f;
+  load_region<SPAN>(g, cstart, c0x, c0y, c0z, 1, beg, off);
+  const int32_t total = off[NCELL];
+  for (int32_t t = threadIdx.x; t < total; t += CIC_NT) {
+    int ci = seg_find(off, NCELL, t);
+    int64_t p = beg[ci] + (t - off[ci]);
+    int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
+    int l0[3] = {l0x, l0y, l0z};
+    float dx1[3], dx2[3];
+    int n1[3];
+    for (int e = 0; e < 3; e++) {
+      int il = l0[e] + rel[e] + 1;  // tile-local 1-based cell index (0 = left ghost)
+      uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
+      float tx = (float)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
+      int a1 = (int)floorf(tx) + 1;
+      dx1[e] = (float)a1 - tx;
+      dx2[e] = 1 - dx1[e];
+      n1[e] = (a1 - 1) - CUBE_NCELL * l0[e];  // block-local node of tap 1
+    }
+    const float w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
+                        dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
+                        dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
+                        dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
+    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
+      if ((unsigned)x < (unsigned)F && (unsigned)y < (unsigned)F && (unsigned)z < (unsigned)F)
+        atomicAdd(&s[(z * F + y) * F + x], w[q]);
+    }
+  }
+  __syncthreads();
+  double ms = 0;
+  for (int i = threadIdx.x; i < F * F * F; i += CIC_NT) {
+    int x = i % F, y = (i / F) % F, z = i / (F * F);
+    int64_t gx = 4 * c0x + x, gy = 4 * c0y + y, gz = 4 * c0z + z;
+    float v = s[i];
+    R[(gz * N + gy) * (N + 2) + gx] = v;
+    ms += (double)v;
+  }
+  for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ms;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int i = 0; i < CIC_NT / 64; i++) m += red[i];
+    part[blockIdx.x] = m;
+  }
+}
+
+// ------------------------------------------------------------ coarse CIC
+// Nodes at coarse-cell centres (pm.f90:255 '-0.5'); a particle in cell c
+// touches nodes c-1..c+1, so the block gathers cells [c0-1, c0+BC].
+template <int BC>
+__global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
+                                                       float mass, float* R, double* part) {
+  constexpr int SPAN = BC + 2, NCELL = SPAN * SPAN * SPAN;
+  __shared__ float s[BC * BC * BC];
+  __shared__ int64_t beg[NCELL];
+  __shared__ int32_t off[NCELL + 1];
+  __shared__ double red[CIC_NT / 64];
+  const int nbx = g.nc / BC;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
+  const int c0x = bx * BC, c0y = by * BC, c0z = bz * BC;
+  const int l0x = c0x % g.nt, l0y = c0y % g.nt, l0z = c0z % g.nt;
+  for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) s[i] = 0.f;
+  load_region<SPAN>(g, cstart, c0x, c0y, c0z, 1, beg, off);
+  const int32_t total = off[NCELL];
+  for (int32_t t = threadIdx.x; t < total; t += CIC_NT) {
+    int ci = seg_find(off, NCELL, t);
+    int64_t p = beg[ci] + (t - off[ci]);
+    int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
+    int l0[3] = {l0x, l0y, l0z};
+    float dx1[3], dx2[3];
+    int n1[3];
+    for (int e = 0; e < 3; e++) {
+      int il = l0[e] + rel[e] + 1;
+      uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
+      float tx = (float)(((double)(il - 1) + xoff(u, g.izipx)) - 0.5);
+      int a1 = (int)floorf(tx) + 1;  // 1-based tile-local node
+      dx1[e] = (float)a1 - tx;
+      dx2[e] = 1 - dx1[e];
+      n1[e] = (a1 - 1) - l0[e];
+    }
+    const float w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
+                        dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
+                        dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
+                        dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
+    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
+      if ((unsigned)x < (unsigned)BC && (unsigned)y < (unsigned)BC && (unsigned)z < (unsigned)BC)
+        atomicAdd(&s[(z * BC + y) * BC + x], w[q]);
+    }
+  }
+  __syncthreads();
+  const int64_t N = g.nc;
+  double ms = 0;
+  for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) {
+    int x = i % BC, y = (i / BC) % BC, z = i / (BC * BC);
+    float v = s[i];
+    R[((int64_t)(c0z + z) * N + (c0y + y)) * (N + 2) + (c0x + x)] = v;
+    ms += (double)v;
+  }
+  for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ms;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int i = 0; i < CIC_NT / 64; i++) m += red[i];
+    part[blockIdx.x] = m;
+  }
+}
+
+int cic_fine(cube_handle* h, double* mass) {
+  Geo g = geo_of(h);
+  if (h->nt % 8) return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 8", (long long)h->nt);
+  int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
+  if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
+  k_cic_fine<8><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
+  HIP_TRY(h, hipGetLastError());
+  if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
+  return 0;
+}
+int cic_coarse(cube_handle* h, double* mass) {
+  Geo g = geo_of(h);
+  int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
+  k_cic_coarse<8><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->coarse.R, h->dred);
+  HIP_TRY(h, hipGetLastError());
+  if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
+  return 0;
+}
+
+// ------------------------------------------------------------------ kick
+// Component d of the fine kick (decode with sigma_old, encode with
+// sigma_new; pm.f90:145-154) followed by component d of the coarse kick
+// (decode/encode with sigma_new; pm.f90:353-363).  Components are
+// independent in both kicks, so running them per component reproduces the
+// reference's sequence exactly.  With probe arrays set, the kick is
+// replaced by writing the interpolated forces.
+__global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
+                                                  int64_t n3c, const float* __restrict__ vfield,
+                                                  const float* __restrict__ Ff, int64_t Nf,
+                                                  const float* __restrict__ Fc,
+                                                  float a_mid, float dt, double c_old, double c_new,
+                                                  const float* __restrict__ vdec, const double* __restrict__ venc, int32_t kmax,
+                                                  uint32_t* vmax_bits, double* probe_f, double* probe_c) {
+  __shared__ int64_t cs[KICK_CH + 1];
+  __shared__ float red[KICK_NT / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * KICK_CH;
+  for (int i = threadIdx.x; i <= KICK_CH; i += KICK_NT) {
+    int64_t r = r0 + i;
+    cs[i] = cstart[r < n3c ? r : n3c];
+  }
+  __syncthreads();
+  const int64_t p0 = cs[0], p1 = cs[KICK_CH];
+  const int64_t Nc = g.nc;
+  float vmax = 0.f;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += KICK_NT) {
+    int lo = 0, hi = KICK_CH;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (cs[mid] <= p) lo = mid; else hi = mid;
+    }
+    int64_t r = r0 + lo;
+    int32_t c[3];
+    cell_of_rank(g, r, c);
+    float dx1[3], dx2[3], cx1[3], cx2[3];
+    int64_t f1[3], f2[3], k1[3], k2[3];
+    for (int e = 0; e < 3; e++) {
+      int torg = (c[e] / g.nt) * g.nt;
+      int il = c[e] - torg + 1;
+      double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
+      float tx = (float)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xo);
+      int a1 = (int)floorf(tx) + 1;
+      dx1[e] = (float)a1 - tx;
+      dx2[e] = 1 - dx1[e];
+      // a tile-local float coordinate may round up to 4*nt; wrap both taps
+      f1[e] = ((int64_t)CUBE_NCELL * torg + a1 - 1) % Nf;
+      f2[e] = (f1[e] + 1) % Nf;
+      float tc = (float)(((double)(torg + il - 1) + xo) - 0.5);
+      int b1 = (int)floorf(tc) + 1;  // 1-based global node, 0..nc (0 and nc+1 are halo)
+      cx1[e] = (float)b1 - tc;
+      cx2[e] = 1 - cx1[e];
+      k1[e] = b1 - 1 < 0 ? Nc - 1 : b1 - 1;
+      k2[e] = b1 >= Nc ? b1 - Nc : b1;
+    }
+    const int64_t P = Nf + 2;
+#define FFI(x, y, z) Ff[((z) * Nf + (y)) * P + (x)]
+#define FCI(x, y, z) Fc[((z) * Nc + (y)) * Nc + (x)]
+    if (probe_f) {
+      double s = 0;
+      s += (double)(FFI(f1[0], f1[1], f1[2]) * dx1[0] * dx1[1] * dx1[2]);
+      s += (double)(FFI(f2[0], f1[1], f1[2]) * dx2[0] * dx1[1] * dx1[2]);
+      s += (double)(FFI(f1[0], f2[1], f1[2]) * dx1[0] * dx2[1] * dx1[2]);
+      s += (double)(FFI(f1[0], f1[1], f2[2]) * dx1[0] * dx1[1] * dx2[2]);
+      s += (double)(FFI(f1[0], f2[1], f2[2]) * dx1[0] * dx2[1] * dx2[2]);
+      s += (double)(FFI(f2[0], f1[1], f2[2]) * dx2[0] * dx1[1] * dx2[2]);
+      s += (double)(FFI(f2[0], f2[1], f1[2]) * dx2[0] * dx2[1] * dx1[2]);
+      s += (double)(FFI(f2[0], f2[1], f2[2]) * dx2[0] * dx2[1] * dx2[2]);
+      probe_f[3 * p + d] = s;
+      s = 0;
+      s += (double)(FCI(k1[0], k1[1], k1[2]) * cx1[0] * cx1[1] * cx1[2]);
+      s += (double)(FCI(k2[0], k1[1], k1[2]) * cx2[0] * cx1[1] * cx1[2]);
+      s += (double)(FCI(k1[0], k2[1], k1[2]) * cx1[0] * cx2[1] * cx1[2]);
+      s += (double)(FCI(k1[0], k1[1], k2[2]) * cx1[0] * cx1[1] * cx2[2]);
+      s += (double)(FCI(k1[0], k2[1], k2[2]) * cx1[0] * cx2[1] * cx2[2]);
+      s += (double)(FCI(k2[0], k1[1], k2[2]) * cx2[0] * cx1[1] * cx2[2]);
+      s += (double)(FCI(k2[0], k2[1], k1[2]) * cx2[0] * cx2[1] * cx1[2]);
+      s += (double)(FCI(k2[0], k2[1], k2[2]) * cx2[0] * cx2[1] * cx2[2]);
+      probe_c[3 * p + d] = s;
+      continue;
+    }
+    const float A = a_mid, T = dt;
+    double v = vdecode(vdec, load_v(vp, 3 * p + d, g.izipv), g.izipv, c_old);
+    v = v + (double)(FFI(f1[0], f1[1], f1[2]) * A * T / 6.0f / PI_F * dx1[0] * dx1[1] * dx1[2]);
+    v = v + (double)(FFI(f2[0], f1[1], f1[2]) * A * T / 6.0f / PI_F * dx2[0] * dx1[1] * dx1[2]);
+    v = v + (double)(FFI(f1[0], f2[1], f1[2]) * A * T / 6.0f / PI_F * dx1[0] * dx2[1] * dx1[2]);
+    v = v + (double)(FFI(f1[0], f1[1], f2[2]) * A * T / 6.0f / PI_F * dx1[0] * dx1[1] * dx2[2]);
+    v = v + (double)(FFI(f1[0], f2[1], f2[2]) * A * T / 6.0f / PI_F * dx1[0] * dx2[1] * dx2[2]);
+    v = v + (double)(FFI(f2[0], f1[1], f2[2]) * A * T / 6.0f / PI_F * dx2[0] * dx1[1] * dx2[2]);
+    v = v + (double)(FFI(f2[0], f2[1], f1[2]) * A * T / 6.0f / PI_F * dx2[0] * dx2[1] * dx1[2]);
+    v = v + (double)(FFI(f2[0], f2[1], f2[2]) * A * T / 6.0f / PI_F * dx2[0] * dx2[1] * dx2[2]);
+    int32_t q = vencode(venc, kmax, g.izipv, c_new, v);
+    v = vdecode(vdec, q, g.izipv, c_new);
+    v = v + (double)(FCI(k1[0], k1[1], k1[2]) * A * T / 6.0f / PI_F * cx1[0] * cx1[1] * cx1[2]);
+    v = v + (double)(FCI(k2[0], k1[1], k1[2]) * A * T / 6.0f / PI_F * cx2[0] * cx1[1] * cx1[2]);
+    v = v + (double)(FCI(k1[0], k2[1], k1[2]) * A * T / 6.0f / PI_F * cx1[0] * cx2[1] * cx1[2]);
+    v = v + (double)(FCI(k1[0], k1[1], k2[2]) * A * T / 6.0f / PI_F * cx1[0] * cx1[1] * cx2[2]);
+    v = v + (double)(FCI(k1[0], k2[1], k2[2]) * A * T / 6.0f / PI_F * cx1[0] * cx2[1] * cx2[2]);
+    v = v + (double)(FCI(k2[0], k1[1], k2[2]) * A * T / 6.0f / PI_F * cx2[0] * cx1[1] * cx2[2]);
+    v = v + (double)(FCI(k2[0], k2[1], k1[2]) * A * T / 6.0f / PI_F * cx2[0] * cx2[1] * cx1[2]);
+    v = v + (double)(FCI(k2[0], k2[1], k2[2]) * A * T / 6.0f / PI_F * cx2[0] * cx2[1] * cx2[2]);
+#undef FFI
+#undef FCI
+    float av = (float)fabs(v + (double)vfield[r * 3 + d]);
+    vmax = fmaxf(vmax, av);
+    store_v(vp, 3 * p + d, g.izipv, vencode(venc, kmax, g.izipv, c_new, v));
+  }
+  if (!probe_f) {
+    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[i]);
+      atomicMax(vmax_bits, __float_as_uint(m));
+    }
+  }
+}
+
+int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, double c_new,
+                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
+  Geo g = geo_of(h);
+  int64_t nb = (h->n3c + KICK_CH - 1) / KICK_CH;
+  const float* Fc = (const float*)h->fc + (int64_t)d * h->n3c;
+  k_kick<<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
+                                                   (const float*)h->fine.W, h->fine.N, Fc, a_mid, dt, c_old, c_new,
+                                                   h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}

@@ -1,0 +1,211 @@
+"""Host driver mirroring CUBE's main loop (main/main.f90:47-98) and
+timestep (main/timestep.f90:8-95) over the C ABI of libcubehip.so.
+
+All time-stepping scalars are real(4) in the reference; they are kept as
+numpy float32 here so the dt/a sequence is computed with the same rounding.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from .checkpoint import BoxState, write_checkpoint
+from .params import NCELL, read_kernel_tables
+
+f32 = np.float32
+DT_MAX = f32(1.0)     # variables.f90:20
+RA_MAX = f32(0.2)     # parameters.f90:110
+
+
+class Cube:
+    """One box resident on the GPU, driven like CUBE's main program."""
+
+    def __init__(self, cfg, state, device=0, np_max=None, precision=4, z_checkpoints=None):
+        self.lib = _lib.load()
+        self.cfg = cfg
+        p = _lib.CubeParams()
+        p.nc, p.nnt, p.ncb = cfg.nc, cfg.nnt, cfg.ncb
+        p.izipx, p.izipv = cfg.izipx, cfg.izipv
+        p.np_max = int(np_max if np_max is not None else max(cfg.np_max, state.sim.nplocal))
+        p.pp_on, p.pp_range, p.rsoft = int(cfg.pp_on), cfg.pp_range, cfg.rsoft
+        p.omega_m, p.omega_l, p.wde = cfg.omega_m, cfg.omega_l, cfg.wde
+        p.precision, p.device = precision, device
+        self.params = p
+        kf, kc = read_kernel_tables()
+        h = ctypes.c_void_p()
+        rc = self.lib.cube_init(ctypes.byref(p), _lib.ptr(kf), _lib.ptr(kc), ctypes.byref(h))
+        self.h = h
+        _lib.check(self.lib, h, rc)
+        self.upload(state)
+        # initialize.f90:40-53 and particle_initialization.f90:136-138
+        self.z_checkpoints = list(z_checkpoints) if z_checkpoints else [float(1.0 / state.sim.a - 1.0), 0.0]
+        self.cur_checkpoint = 1
+        self.a = f32(1.0 / (1.0 + f32(self.z_checkpoints[0])))
+        self.t = f32(0)
+        self.tau = f32(-3 / np.sqrt(1.0 / (1 + self.z_checkpoints[0])))
+        self.dt = f32(0)
+        self.dt_old = f32(0)
+        self.da = f32(0)
+        self.a_mid = self.a
+        self.dt_neu = DT_MAX
+        self.checkpoint_step = False
+        self.final_step = False
+        self.cur_checkpoint += 1  # main.f90:42
+        self.timings = {}
+
+    # ------------------------------------------------------------- state
+    def upload(self, state):
+        self.sim = _lib.SimHeader.from_bytes(state.sim.to_bytes())
+        _lib.check(self.lib, self.h, self.lib.cube_upload_state(
+            self.h, ctypes.byref(self.sim), _lib.ptr(np.ascontiguousarray(state.xp)),
+            _lib.ptr(np.ascontiguousarray(state.vp)), _lib.ptr(np.ascontiguousarray(state.pid)),
+            _lib.ptr(np.ascontiguousarray(state.rhoc)), _lib.ptr(np.ascontiguousarray(state.vfield))))
+
+    def download(self):
+        s = _lib.SimHeader()
+        _lib.check(self.lib, self.h, self.lib.cube_download_state(self.h, ctypes.byref(s), None, None, None, None, None))
+        n = s.nplocal
+        xdt = np.int8 if self.cfg.izipx == 1 else np.int16
+        vdt = np.int8 if self.cfg.izipv == 1 else np.int16
+        xp = np.empty((n, 3), dtype=xdt)
+        vp = np.empty((n, 3), dtype=vdt)
+        pid = np.empty(n, dtype=np.int32)
+        rhoc = np.empty(self.cfg.nc ** 3, dtype=np.int32)
+        vf = np.empty((self.cfg.nc ** 3, 3), dtype=np.float32)
+        _lib.check(self.lib, self.h, self.lib.cube_download_state(
+            self.h, ctypes.byref(s), _lib.ptr(xp), _lib.ptr(vp), _lib.ptr(pid), _lib.ptr(rhoc), _lib.ptr(vf)))
+        for f in ("a", "t", "tau"):
+            setattr(s, f, float(getattr(self, f)))
+        return BoxState(s, xp, vp, pid, rhoc, vf)
+
+    def close(self):
+        if self.h:
+            self.lib.cube_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------- subroutines
+    def expansion(self, a, dt):
+        return _lib.expansion(a, dt, self.cfg.omega_m, self.cfg.omega_l, self.cfg.wde)
+
+    def timestep(self):
+        """timestep.f90:8-95 (head-image branch; one image)."""
+        self.dt_old = self.dt
+        self.sim.timestep += 1
+        dt_e = DT_MAX
+        ntry = 0
+        while True:
+            ntry += 1
+            d1, d2 = self.expansion(self.a, dt_e)
+            da = f32(d1 + d2)
+            ra = f32(da / f32(self.a + da))
+            if ra > RA_MAX:
+                dt_e = f32(dt_e * f32(RA_MAX / ra))
+            else:
+                break
+            if ntry > 10:
+                break
+        s = self.sim
+        dt = min(dt_e, f32(s.dt_fine), f32(s.dt_coarse), f32(s.dt_pp), f32(s.dt_vmax), f32(1000.0), self.dt_neu)
+        d1, d2 = self.expansion(self.a, dt)
+        da = f32(d1 + d2)
+        self.checkpoint_step = False
+        z_next = f32(self.z_checkpoints[self.cur_checkpoint - 1])
+        a_next = f32(f32(1.0) / f32(1 + z_next))
+        if da >= f32(a_next - self.a):
+            self.checkpoint_step = True
+            if self.cur_checkpoint == len(self.z_checkpoints):
+                self.final_step = True
+            it = 0
+            while abs(f32(f32(self.a + da) / a_next) - 1) >= 1e-6 and it < 100:
+                dt = f32(dt * f32(f32(a_next - self.a) / da))
+                d1, d2 = self.expansion(self.a, dt)
+                da = f32(d1 + d2)
+                it += 1
+        self.a_mid = f32(self.a + f32(da / 2))
+        self.tau = f32(self.tau + dt)
+        self.t = f32(self.t + dt)
+        self.a = f32(self.a + da)
+        self.da = da
+        self.dt = dt
+        self.dt_neu = f32(1000.0)
+
+    def update_x(self):
+        out = _lib.DriftOut()
+        _lib.check(self.lib, self.h, self.lib.cube_drift(self.h, float(self.dt_old), float(self.dt), ctypes.byref(out)))
+        self.last_drift = out
+        return out
+
+    def ext_pp_force(self):
+        out = _lib.PPOut()
+        _lib.check(self.lib, self.h, self.lib.cube_pp(self.h, float(self.a_mid), float(self.dt), ctypes.byref(out)))
+        self.sim.dt_pp = out.dt_pp
+        self.last_pp = out
+        return out
+
+    def particle_mesh(self):
+        out = _lib.PMOut()
+        _lib.check(self.lib, self.h, self.lib.cube_pm(self.h, float(self.a_mid), float(self.dt), ctypes.byref(out)))
+        s = self.sim
+        s.dt_fine, s.dt_coarse, s.dt_vmax, s.vz_max = out.dt_fine, out.dt_coarse, out.dt_vmax, out.vz_max
+        s.vsim2phys = (1.5 / self.a) * self.cfg.box * self.cfg.h0 * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+        self.last_pm = out
+        return out
+
+    # --------------------------------------------------------- main loop
+    def step(self, opath=None):
+        """One pass of main.f90:48-97; returns True after the final checkpoint."""
+        t0 = time.perf_counter()
+        self.timestep()
+        self.update_x()
+        if self.cfg.pp_on:
+            self.ext_pp_force()
+        self.particle_mesh()
+        if self.checkpoint_step:
+            self.dt_old = f32(0)
+            self.update_x()
+            if opath is not None:
+                st = self.download()
+                st.sim.cur_checkpoint = self.cur_checkpoint
+                st.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+                write_checkpoint(st, opath, self.z_checkpoints[self.cur_checkpoint - 1])
+            self.cur_checkpoint += 1
+            if self.final_step:
+                return True
+            self.dt = f32(0)
+        self.timings.setdefault("step", []).append(time.perf_counter() - t0)
+        return False
+
+    def run(self, max_steps=2500, opath=None):
+        n = 0
+        while n < max_steps:
+            n += 1
+            if self.step(opath):
+                break
+        return n
+
+    # ------------------------------------------------------ diagnostics
+    def probe_pm_forces(self):
+        n = self.sim.nplocal
+        ff = np.empty((n, 3), dtype=np.float64)
+        fc = np.empty((n, 3), dtype=np.float64)
+        _lib.check(self.lib, self.h, self.lib.cube_probe_pm_forces(self.h, _lib.ptr(ff), _lib.ptr(fc)))
+        return ff, fc
+
+    def probe_pp_forces(self):
+        n = self.sim.nplocal
+        fp = np.empty((n, 3), dtype=np.float64)
+        _lib.check(self.lib, self.h, self.lib.cube_probe_pp_forces(self.h, _lib.ptr(fp)))
+        return fp
+
+    def probe_density(self):
+        nf = self.cfg.nf
+        rho = np.empty((nf, nf, nf), dtype=np.float32)
+        _lib.check(self.lib, self.h, self.lib.cube_probe_density(self.h, _lib.ptr(rho)))
+        return rho

@@ -1,0 +1,162 @@
+/*
+ * cube_hip.h -- C ABI of libcubehip.so, the MI355X (gfx950) implementation of
+ * CUBE's per-step gravity and update path.
+ *
+ * CUBE's driver (CUBE_v2.0/work/main/main.f90:47-98) calls argument-less
+ * Fortran subroutines that share module-global state.  This library keeps
+ * that state resident in HBM behind an opaque handle and exposes one entry
+ * point per subroutine of the path.  Every entry point returns 0 on success
+ * or a nonzero cube_status; the message is available from cube_last_error().
+ * The Fortran binding (cube_nufluid_amd/fortran/cube_hip_mod.f90) turns a
+ * nonzero status into 'error stop', matching the reference's fail-stop
+ * 'stop' statements.
+ *
+ * Conventions: one host thread per handle; calls are ordered on the handle's
+ * HIP stream; entry points that return scalars synchronise that stream.
+ * Host arrays use CUBE's checkpoint layout (main/checkpoint.f90:36-47):
+ * particles in tile order (tile x fastest), then coarse cell (x fastest),
+ * then in-cell order; rhoc/vfield over physical cells in the same order.
+ */
+#ifndef CUBE_HIP_H
+#define CUBE_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cube_handle cube_handle;
+
+enum cube_status {
+  CUBE_OK = 0,
+  CUBE_ERR_ARG = 1,       /* invalid parameter or array size */
+  CUBE_ERR_CAPACITY = 2,  /* particle capacity / displacement bound exceeded
+                             (update_particle.f90:90-96, buffer_grid.f90:131-137) */
+  CUBE_ERR_FORMAT = 3,    /* header/zip mismatch (particle_initialization.f90:140-144) */
+  CUBE_ERR_CHECK = 4,     /* conservation check failed (ext_pp_force.f90:154-158) */
+  CUBE_ERR_HIP = 5        /* HIP runtime error */
+};
+
+/* Compile-time constants of main/parameters.f90 (lines 8-116) that shape the
+   path, as run-time values.  nn=1 (one image). */
+typedef struct cube_params {
+  int64_t nc;        /* coarse cells per dim (parameters.f90:29)             */
+  int64_t nnt;       /* tiles per dim (:28); nt = nc/nnt                     */
+  int64_t ncb;       /* coarse buffer cells (:47)                            */
+  int64_t izipx;     /* bytes per position component, 1 or 2 (:8)           */
+  int64_t izipv;     /* bytes per velocity component, 1 or 2 (:9)           */
+  int64_t np_max;    /* particle capacity of the resident arrays            */
+  int32_t pp_on;     /* Extended_pp_force (:60)                             */
+  int32_t pp_range;  /* (:62)                                               */
+  float rsoft;       /* PP softening (:61)                                   */
+  float omega_m;     /* (:91)                                               */
+  float omega_l;     /* (:99)                                               */
+  float wde;         /* (:100)                                              */
+  int32_t precision; /* 4: FP32 meshes/forces (production); 8: FP64 oracle   */
+  int32_t device;    /* HIP device ordinal                                   */
+} cube_params;
+
+/* 224-byte checkpoint header, main/parameters.f90:129-153 (17 x int64 then
+   22 x float32, stream-written by checkpoint.f90:36). */
+typedef struct cube_sim_header {
+  int64_t nplocal, npglobal, nplocal_nu, npglobal_nu;
+  int64_t izipx, izipv, izipx_nu, izipv_nu;
+  int64_t image;
+  int64_t nn, nnt, nt, ncell, ncb;
+  int64_t timestep;
+  int64_t cur_checkpoint;
+  int64_t cur_halofind;
+  float a, t, tau;
+  float dt_pp, dt_fine, dt_coarse, dt_vmax, dt_vmax_nu;
+  float mass_p_cdm, mass_p_nu;
+  float box;
+  float h0, omega_m, omega_l, s8, vsim2phys, sigma_vres, sigma_vi, sigma_vi_nu;
+  float z_i, z_i_nu;
+  float vz_max;
+} cube_sim_header;
+
+/* Results of one drift (update_particle.f90:221-252). */
+typedef struct cube_drift_out {
+  float sigma_vi_new;    /* std_vsim_res/sqrt(3.) (:226)                      */
+  int32_t nlayer;        /* 2*ceiling(dt_mid*vz_max/ncell)+1 (:39)           */
+  double std_vsim, std_vsim_c, std_vsim_res; /* (:221-223), simulation units */
+  int64_t npcheck;       /* particles after re-binning (:247-252)            */
+  int32_t max_disp[3];   /* largest cell displacement per dimension          */
+  int32_t frame_conflicts; /* tile-frame rounding disagreements (expected 0) */
+} cube_drift_out;
+
+/* Results of one PP pass (ext_pp_force.f90:139-151). */
+typedef struct cube_pp_out {
+  float dt_pp;           /* 1/(sqrt(f2_max_pp)*a_mid*GG), running max (:139) */
+  float f2_max_pp;
+  int64_t npairs;        /* pair evaluations incl. self (:98)                */
+  int64_t itest1;        /* particles updated (:87)                          */
+} cube_pp_out;
+
+/* Results of one particle_mesh call (pm.f90:427-446). */
+typedef struct cube_pm_out {
+  float dt_fine, dt_coarse, dt_vmax, vz_max;
+  float vmax[3];
+  float f2_max_fine, f2_max_coarse;
+  double mass_f, mass_c;  /* testrhof, testrhoc (pm.f90:446)                */
+} cube_pm_out;
+
+/* Replaces initialize.f90:21-38 (create_cubefft_plan, create_penfft_plan,
+   kernel_f, kernel_c).  kern_f_table: kernels/wfxyzf.3.ascii as 4096 rows x
+   3 floats in file order (kernel_f.f90:14-23); kern_c_table:
+   kernels/wfxyzc.2.ascii as 64 rows x 3 floats (kernel_c.f90:31-39). */
+int cube_init(const cube_params* p, const float* kern_f_table, const float* kern_c_table,
+              cube_handle** out);
+
+/* Replaces particle_initialization.f90:133-157 (the reads) plus the halo
+   set-up of buffer_grid/buffer_x/buffer_v (main.f90:20-23).  xp/vp hold
+   3*nplocal components of izipx/izipv bytes; rhoc nc^3 int32; vfield
+   3*nc^3 float32. */
+int cube_upload_state(cube_handle* h, const cube_sim_header* sim, const void* xp, const void* vp,
+                      const int32_t* pid, const int32_t* rhoc, const float* vfield);
+
+/* Replaces checkpoint.f90:24-47 (the array contents it writes); sim receives
+   nplocal/npglobal, sigma_vi, vz_max, dt_* as held by the handle. */
+int cube_download_state(cube_handle* h, cube_sim_header* sim, void* xp, void* vp, int32_t* pid,
+                        int32_t* rhoc, float* vfield);
+
+/* Replaces update_x -> update_xp (update_particle.f90:5-259) followed by
+   buffer_grid and buffer_x (main.f90:57-59). */
+int cube_drift(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
+
+/* Replaces ext_pp_force (ext_pp_force.f90:9-160): pair force over fine-cell
+   neighbourhoods within +-pp_range, kick and re-quantise in place. */
+int cube_pp(cube_handle* h, float a_mid, float dt, cube_pp_out* out);
+
+/* Replaces particle_mesh (pm.f90:2-451) and buffer_v (main.f90:63-64):
+   fine and coarse PM forces, kicks, re-quantisation, dt constraints. */
+int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out);
+
+/* Replaces expansion (timestep.f90:270-314): third-order Taylor step of the
+   Friedmann equation, real(4) interface, real(8) internals. */
+int cube_expansion(float a, float dt, float omega_m, float omega_l, float wde, float* da1, float* da2);
+
+/* Copies the message of the last failing call into buf (NUL-terminated). */
+int cube_last_error(cube_handle* h, char* buf, size_t n);
+
+/* Replaces finalize.f90 (plan destruction); frees all device memory. */
+int cube_finalize(cube_handle* h);
+
+/* ---- diagnostics with no counterpart in the reference: per-particle force
+   probes used by the parity tests and the FP32-vs-FP64 error report.  Each
+   writes 3*nplocal doubles in particle order and leaves the state unchanged.
+   fine/coarse: sum over the 8 CIC taps of F*w (pm.f90:146-153, 354-361
+   without the a_mid*dt/6/pi factor).  pp: the pair sum f_tot of
+   ext_pp_force.f90:92-113. */
+int cube_probe_pm_forces(cube_handle* h, double* f_fine, double* f_coarse);
+int cube_probe_pp_forces(cube_handle* h, double* f_pp);
+
+/* Copies the fine-mesh CIC density of the current positions (nf^3 float,
+   x fastest) -- the periodic field pm.f90:76-107 builds tile by tile. */
+int cube_probe_density(cube_handle* h, float* rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
