@@ -84,6 +84,9 @@ SIGNATURES = {
     "cube_probe_pm_forces": (c_i32, [_P, _P, _P]),
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
+    "cube_profile_enable": (c_i32, [_P, c_i32]),
+    "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),
+    "cube_profile_slot_name": (c_i32, [c_i32, ctypes.c_char_p, ctypes.c_size_t]),
 }
 
 _lib = None

@@ -147,8 +147,62 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   return CUBE_OK;
 }
 
+hipEvent_t prof_event(cube_handle* h) {
+  hipEvent_t e = nullptr;
+  if (!h->prof_pool.empty()) {
+    e = h->prof_pool.back();
+    h->prof_pool.pop_back();
+  } else {
+    (void)hipEventCreate(&e);
+  }
+  return e;
+}
+
+extern "C" int cube_profile_enable(cube_handle* h, int on) {
+  if (!h) return CUBE_ERR_ARG;
+  h->prof_on = on != 0;
+  return CUBE_OK;
+}
+
+static const char* kSlotNames[PS_NSLOT] = {"drift_pass1", "drift_scan", "drift_pass3", "pp", "cic_coarse",
+                                           "fft_coarse", "cic_fine", "fft_x_r2c", "fft_y_fwd", "fft_zconv",
+                                           "fft_y_inv", "fft_x_c2r", "kick"};
+
+extern "C" int cube_profile_read(cube_handle* h, double* ms, int64_t* count, int nslot) {
+  if (!h || nslot < 0) return CUBE_ERR_ARG;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (ProfRec& r : h->prof_recs) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      h->prof_ms[r.slot] += t;
+      h->prof_n[r.slot] += 1;
+    }
+    h->prof_pool.push_back(r.a);
+    h->prof_pool.push_back(r.b);
+  }
+  h->prof_recs.clear();
+  for (int i = 0; i < nslot && i < PS_NSLOT; i++) {
+    if (ms) ms[i] = h->prof_ms[i];
+    if (count) count[i] = h->prof_n[i];
+    h->prof_ms[i] = 0;
+    h->prof_n[i] = 0;
+  }
+  return PS_NSLOT;
+}
+
+extern "C" int cube_profile_slot_name(int slot, char* buf, size_t n) {
+  if (slot < 0 || slot >= PS_NSLOT || !buf || n == 0) return CUBE_ERR_ARG;
+  snprintf(buf, n, "%s", kSlotNames[slot]);
+  return CUBE_OK;
+}
+
 extern "C" int cube_finalize(cube_handle* h) {
   if (!h) return CUBE_OK;
+  for (ProfRec& r : h->prof_recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
   void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
                   h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc};
   for (void* b : bufs)

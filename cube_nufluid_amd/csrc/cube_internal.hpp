@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 #include "cube_hip.h"
 
 #define CUBE_NCELL 4
@@ -34,6 +35,13 @@ struct Mesh {
   size_t bytes_R() const { return (size_t)N * N * (N + 2) * prec; }
 };
 
+// Per-kernel timing with HIP events on the handle's stream (diagnostic).
+enum ProfSlot {
+  PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT3, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
+  PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_NSLOT
+};
+struct ProfRec { int slot; hipEvent_t a, b; };
+
 struct cube_handle {
   cube_params p;
   int64_t nc, nt, nnt, ncb, nf, n3c;
@@ -64,7 +72,25 @@ struct cube_handle {
   float ftemp_pp = 0;         // ext_pp_force.f90:18 running max (SAVE)
   cube_sim_header sim{};
   std::string err;
+  // profiling
+  bool prof_on = false;
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
+  double prof_ms[PS_NSLOT] = {0};
+  int64_t prof_n[PS_NSLOT] = {0};
 };
+
+hipEvent_t prof_event(cube_handle* h);
+static inline int prof_begin(cube_handle* h, int slot) {
+  if (!h->prof_on) return -1;
+  ProfRec r{slot, prof_event(h), prof_event(h)};
+  (void)hipEventRecord(r.a, h->stream);
+  h->prof_recs.push_back(r);
+  return (int)h->prof_recs.size() - 1;
+}
+static inline void prof_end(cube_handle* h, int id) {
+  if (id >= 0) (void)hipEventRecord(h->prof_recs[id].b, h->stream);
+}
 
 // ------------------------------------------------------------- errors
 int cube_fail(cube_handle* h, int code, const char* fmt, ...);

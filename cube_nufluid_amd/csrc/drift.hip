@@ -197,8 +197,10 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   HIP_TRY(h, hipMemsetAsync(flags, 0, sizeof(uint32_t) * 8, h->stream));
   HIP_TRY(h, hipMemsetAsync(h->rhoc2, 0, sizeof(int32_t) * h->n3c, h->stream));
   int64_t nb1 = (h->n3c + DR_CH - 1) / DR_CH;
+  int pid1 = prof_begin(h, PS_DRIFT1);
   k_drift_pass1<<<(unsigned)nb1, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, dt_mid, c_old, h->vdec, dmax,
                                                          h->disp, h->rhoc2, flags);
+  prof_end(h, pid1);
   HIP_TRY(h, hipGetLastError());
   uint32_t hf[8];
   HIP_TRY(h, hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, h->stream));
@@ -206,14 +208,18 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   if (hf[3])
     return cube_fail(h, CUBE_ERR_CAPACITY, "a particle moved more than %d coarse cells in one drift (buffer ncb=%lld)", dmax,
                      (long long)h->ncb);
+  int pid2 = prof_begin(h, PS_DRIFT_SCAN);
   int r = scan_counts(h, h->rhoc2, h->n3c, h->cstart2);
+  prof_end(h, pid2);
   if (r) return r;
   int64_t nb3 = (h->n3c + DR_NT - 1) / DR_NT;
   if (nb3 * 3 > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
+  int pid3 = prof_begin(h, PS_DRIFT3);
   k_drift_pass3<<<(unsigned)nb3, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->pid, h->cstart, h->vfield, h->disp, h->cstart2, h->n3c,
                                                          (int)hf[0], (int)hf[1], (int)hf[2], nlayer, dt_mid, c_old, h->vdec,
                                                          h->venc, h->vkmax, h->xp2, h->vp2, h->pid2, h->vfield2, h->rhoc2,
                                                          h->dred, flags);
+  prof_end(h, pid3);
   HIP_TRY(h, hipGetLastError());
   double sums[3];
   if ((r = reduce_doubles(h, h->dred, nb3, 3, sums))) return r;

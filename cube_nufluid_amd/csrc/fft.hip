@@ -303,11 +303,16 @@ static int fwd_xy_t(cube_handle* h, Mesh& m) {
   int64_t N = m.N, nrows = N * N;
   cpx<T>* c = (cpx<T>*)m.R;
   const cpx<T>* tw = (const cpx<T>*)m.tw;
+  const bool fine = &m == &h->fine;
   LOGN_SWITCH(logn, {
     constexpr int WX = pick_w<T>(L - 1);
+    int p1 = prof_begin(h, fine ? PS_FFT_XF : PS_FFT_C);
     k_fft_x_r2c<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(c, tw, nrows);
+    prof_end(h, p1);
     constexpr int WS = pick_w<T>(L);
+    int p2 = prof_begin(h, fine ? PS_FFT_YF : PS_FFT_C);
     k_fft_strided<T, L, 1, false><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(c, tw);
+    prof_end(h, p2);
   });
   HIP_TRY(h, hipGetLastError());
   return 0;
@@ -339,11 +344,13 @@ template <typename T>
 static int zconv_t(cube_handle* h, Mesh& m, int d) {
   int logn = ilog2(m.N);
   int64_t N = m.N;
+  int p1 = prof_begin(h, &m == &h->fine ? PS_FFT_ZCONV : PS_FFT_C);
   LOGN_SWITCH(logn, {
     constexpr int WS = pick_w<T>(L);
     k_fft_zconv<T, L><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(
         (const cpx<T>*)m.R, (cpx<T>*)m.W, (const T*)m.K, d, (const cpx<T>*)m.tw);
   });
+  prof_end(h, p1);
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
@@ -357,12 +364,17 @@ static int inv_yx_t(cube_handle* h, Mesh& m, void* out, int64_t opitch, int smod
   int64_t N = m.N, nrows = N * N;
   cpx<T>* c = (cpx<T>*)m.W;
   const cpx<T>* tw = (const cpx<T>*)m.tw;
+  const bool fine = &m == &h->fine;
   LOGN_SWITCH(logn, {
     constexpr int WS = pick_w<T>(L);
+    int p1 = prof_begin(h, fine ? PS_FFT_YI : PS_FFT_C);
     k_fft_strided<T, L, 1, true><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(c, tw);
+    prof_end(h, p1);
     constexpr int WX = pick_w<T>(L - 1);
+    int p2 = prof_begin(h, fine ? PS_FFT_XI : PS_FFT_C);
     k_fft_x_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
         c, (T*)out, opitch, tw, nrows, smode, m.S, f2max);
+    prof_end(h, p2);
   });
   HIP_TRY(h, hipGetLastError());
   return 0;

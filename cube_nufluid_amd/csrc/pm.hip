@@ -181,7 +181,9 @@ int cic_fine(cube_handle* h, double* mass) {
   if (h->nt % 8) return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 8", (long long)h->nt);
   int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
   if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
+  int pr = prof_begin(h, PS_CIC_F);
   k_cic_fine<8><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
+  prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
   return 0;
@@ -189,7 +191,9 @@ int cic_fine(cube_handle* h, double* mass) {
 int cic_coarse(cube_handle* h, double* mass) {
   Geo g = geo_of(h);
   int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
+  int pr = prof_begin(h, PS_CIC_C);
   k_cic_coarse<8><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->coarse.R, h->dred);
+  prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
   return 0;
@@ -318,9 +322,11 @@ int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, d
   Geo g = geo_of(h);
   int64_t nb = (h->n3c + KICK_CH - 1) / KICK_CH;
   const float* Fc = (const float*)h->fc + (int64_t)d * h->n3c;
+  int pr = prof_begin(h, PS_KICK);
   k_kick<<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
                                                    (const float*)h->fine.W, h->fine.N, Fc, a_mid, dt, c_old, c_new,
                                                    h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+  prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   return 0;
 }

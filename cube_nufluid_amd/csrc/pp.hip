@@ -167,8 +167,10 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
   unsigned long long* cnt = (unsigned long long*)(h->uflags + 16);
   HIP_TRY(h, hipMemsetAsync(h->uflags + 8, 0, sizeof(uint32_t) * 16, h->stream));
   int64_t nb = (h->nc / PP_BP) * (h->nc / PP_BP) * (h->nc / PP_BP);
+  int pr = prof_begin(h, PS_PP);
   k_pp<<<(unsigned)nb, PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, h->p.pp_range, a_mid, dt, c_old,
                                                h->vdec, h->venc, h->vkmax, f2, cnt, cnt + 1, probe);
+  prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   uint32_t hv[12];
   HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 8, sizeof(uint32_t) * 12, hipMemcpyDeviceToHost, h->stream));

@@ -3,8 +3,8 @@
 Follows the encoding conventions of util/initial_conditions.f90:759-954:
 particles start on a lattice of np_nc per coarse cell per dimension at
 q = (i-1)/np_nc + 0.5/ncell (coarse units), are displaced by the
-Zel'dovich field, binned with g = ceiling(x), stored as
-xp = floor(x/x_resolution) narrowed to izipx bytes, velocities stored
+Zel'dovich field, binned into cells, stored as the in-cell offset
+floor(frac(x)/x_resolution) narrowed to izipx bytes, velocities stored
 relative to the coarse-cell mean vfield and encoded with
 nint((nvbin-1)*atan(sqrt(pi/2)/(sigma_vi*vrel_boost)*v)/pi).
 
@@ -121,11 +121,13 @@ def encode_particles(cfg, pos, vel, pid, a, sigma_vi=None, z_i=None):
     (simulation units) into a BoxState in checkpoint order."""
     nc, nt, nnt = cfg.nc, cfg.nt, cfg.nnt
     pos = np.mod(np.asarray(pos, dtype=np.float64), nc)
-    cell1 = np.ceil(pos).astype(np.int64)                      # g=ceiling(x), 1-based
-    cell1[cell1 == 0] = nc                                      # x==0 wraps to cell nc
-    c0 = cell1 - 1
-    xres = 1.0 / 2 ** (8 * cfg.izipx)
-    u = np.floor(pos / xres).astype(np.int64) % (2 ** (8 * cfg.izipx))
+    # cell = floor(x) and u = floor(frac(x)/x_resolution): the decoded position
+    # (cell + (u+0.5)*x_resolution) always lies in the cell.  The reference's
+    # g=ceiling(x) with xp=floor(x/x_resolution) moves a particle sitting
+    # exactly on a cell face into the cell below it.
+    c0 = np.floor(pos).astype(np.int64) % nc
+    nbin = 2 ** (8 * cfg.izipx)
+    u = np.clip(np.floor((pos - np.floor(pos)) * nbin).astype(np.int64), 0, nbin - 1)
     xp = u.astype(np.uint16 if cfg.izipx == 2 else np.uint8).view(xdtype(cfg.izipx))
     r = cell_rank(c0[:, 0], c0[:, 1], c0[:, 2], nt, nnt)
     order = np.lexsort((pid, r))
@@ -184,7 +186,7 @@ def zeldovich_ic(cfg, seed, z=None):
     del wk, pk
     psi = np.empty((ng, ng, ng, 3), dtype=np.float64)
     for e, ke in enumerate((kx, ky, kz)):
-        psi[..., e] = np.fft.irfftn(1j * ke / k2 * delta_k, s=(ng, ng, ng))   # Mpc/h
+        psi[..., e] = np.fft.irfftn(1j * ke / k2 * delta_k, s=(ng, ng, ng), axes=(0, 1, 2))   # Mpc/h
     del delta_k, kx, ky, kz, k2
     psi *= cfg.nf / cfg.box                                                    # fine cells
     # lattice point i (0-based) at q = i/np_nc + 0.5/ncell coarse cells; array axes are (z,y,x)

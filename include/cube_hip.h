@@ -156,6 +156,15 @@ int cube_probe_pp_forces(cube_handle* h, double* f_pp);
    x fastest) -- the periodic field pm.f90:76-107 builds tile by tile. */
 int cube_probe_density(cube_handle* h, float* rho);
 
+/* Per-kernel timing with HIP events recorded on the handle's stream (no
+   counterpart in the reference, whose only timers are the per-phase
+   system_clock prints, e.g. pm.f90:45,182-183).  cube_profile_read returns
+   the number of slots and fills accumulated milliseconds and launch counts
+   per slot since the previous read; cube_profile_slot_name names a slot. */
+int cube_profile_enable(cube_handle* h, int on);
+int cube_profile_read(cube_handle* h, double* ms, int64_t* count, int nslot);
+int cube_profile_slot_name(int slot, char* buf, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -355,8 +355,15 @@ static void build_kern_c(orc_t* o, int threads) {
           for (int e = 0; e < 3; e++) ks[e] = 2 * sinf(PI_F * kx[e] / (float)nc);
           float v = cim[q];
           if (!(kr > 8.0f || kx[d] == 0)) {
-            float s2 = ks[0] * ks[0] + ks[1] * ks[1] + ks[2] * ks[2];
-            v = v * 0.25f * PI_F * ks[d] / s2 / cim0[q];
+            /* On the odd component's own Nyquist plane (kx(d) = -nc/2) both
+               Im FT(ck) and Im FT(-r/r^3) vanish exactly; the reference divides
+               FFTW round-off there (only reachable for nc=16, |k|=8).  The
+               exact-arithmetic value 0 is used. */
+            if (kx[d] == (float)(-nc / 2)) v = 0.0f;
+            else {
+              float s2 = ks[0] * ks[0] + ks[1] * ks[1] + ks[2] * ks[2];
+              v = v * 0.25f * PI_F * ks[d] / s2 / cim0[q];
+            }
           }
           kc[q] = v;
         }
