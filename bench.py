@@ -1,0 +1,183 @@
+"""Benchmark: CUBE PM+PP steps on one MI355X through libcubehip.so.
+
+A step is one pass of CUBE's main loop body (main.f90:48-97): timestep,
+drift/re-bin, PP force + kick, fine and coarse PM force + kick.  The timed
+region holds exactly --steps steps between device synchronisations; inputs
+are resident in HBM before it starts.  Prints one JSON line.
+
+Extra fields:
+  roofline     -- the kernel with the largest share of step time: algorithmic
+                  bytes per launch / mean launch time (HIP events on the
+                  library's stream), against MI355X's 8 TB/s;
+  cpu_baseline -- the CPU oracle (a C restatement of CUBE's step, OpenMP on
+                  the host cores) timed on a bounded sample of the same
+                  workload: the first tiles of the same box.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(slot, cfg, npart):
+    """Minimal HBM bytes per launch of each profiled kernel class
+    (particle arrays, meshes, offsets; re-reads of halo particles, padding
+    and tile overheads are not counted)."""
+    bx, bv = 3 * cfg.izipx, 3 * cfg.izipv
+    mf, mc = float(cfg.nf) ** 3, float(cfg.nc) ** 3
+    n = float(npart)
+    table = {
+        "drift_pass1": n * (bx + bv + 2) + 24 * mc,
+        "drift_scan": 12 * mc,
+        "drift_pass3": n * (2 * bx + 2 * bv + 10) + 44 * mc,
+        "pp": n * (bx + 2 * bv) + 8 * mc,
+        "cic_coarse": n * bx + 12 * mc,
+        "cic_fine": n * bx + 4 * mf + 8 * mc,
+        "fft_x_r2c": 8 * mf,
+        "fft_y_fwd": 8 * mf,
+        "fft_zconv": 8.5 * mf,
+        "fft_y_inv": 8 * mf,
+        "fft_x_c2r": 8 * mf + 16.0 / 3 * mf,
+        "kick": n * (bx + 2 * cfg.izipv) + 4 * mf + 20 * mc,
+        "fft_coarse": None,
+    }
+    return table.get(slot)
+
+
+def make_state(cfg, seed, z):
+    from cube_nufluid_amd.ic import zeldovich_ic
+    return zeldovich_ic(cfg, seed=seed, z=z)
+
+
+def cpu_baseline(cfg, st, steps_dt, sample_tiles, threads):
+    """Time the oracle's full step (drift + PP + PM) on the first
+    `sample_tiles` tiles of the same box; value = particles in the sample /
+    wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    from cube_nufluid_amd.ic import rank_to_cell  # noqa: F401
+    orc = Oracle(cfg, threads=threads)
+    orc.set_state(st)
+    orc.lib.orc_set_tile_limit(orc.o, sample_tiles)
+    nt3 = cfg.nt ** 3
+    n_sample = int(st.rhoc[: sample_tiles * nt3].sum())
+    dt = float(steps_dt)
+    t0 = time.perf_counter()
+    orc.drift(dt, dt)
+    if cfg.pp_on:
+        orc.pp(0.5, dt)
+    orc.pm(0.5, dt)
+    wall = time.perf_counter() - t0
+    orc.close()
+    return {"value": n_sample / wall, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle, one full step (drift+PP+fine/coarse PM) on the first %d of %d tiles "
+                      "(%d particles) of the same box; %.1f s" % (sample_tiles, cfg.nnt ** 3, n_sample, wall)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default=os.environ.get("CUBE_BENCH_CONFIG", "C2"))
+    ap.add_argument("--z", type=float, default=None, help="redshift of the Zel'dovich start state")
+    ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-tiles", type=int, default=0, help="tiles in the CPU sample (0: one per thread)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs one box on one GPU; multi-GPU runs are not part of this build (asked for %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cube_nufluid_amd.params import CONFIGS
+    from cube_nufluid_amd.driver import Cube
+    from cube_nufluid_amd import _lib
+
+    cfg = CONFIGS[args.config]
+    z0 = args.z if args.z is not None else 1.0
+    t0 = time.perf_counter()
+    st = make_state(cfg, args.seed, z0)
+    log("state %s: %d particles at z=%.2f (%.1f s)" % (args.config, st.sim.nplocal, z0, time.perf_counter() - t0))
+    cube = Cube(cfg, st, device=0, z_checkpoints=[z0, 0.0])
+    lib = cube.lib
+    npart = st.sim.nplocal
+    for i in range(args.warmup):
+        cube.step()
+        log("warmup step %d done" % (i + 1))
+    # one profiled step for the per-kernel table (events add no host sync)
+    lib.cube_profile_enable(cube.h, 1)
+    lib.cube_profile_read(cube.h, None, None, 0)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        cube.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    nslot = 16
+    ms = np.zeros(nslot)
+    cnt = np.zeros(nslot, dtype=np.int64)
+    ns = lib.cube_profile_read(cube.h, _lib.ptr(ms), _lib.ptr(cnt), nslot)
+    names = []
+    for s in range(ns):
+        b = ctypes.create_string_buffer(64)
+        lib.cube_profile_slot_name(s, b, 64)
+        names.append(b.value.decode())
+    per_kernel = {names[s]: {"ms_total": round(float(ms[s]), 3), "launches": int(cnt[s])} for s in range(ns) if cnt[s]}
+    log("per-kernel: " + json.dumps(per_kernel))
+    cands = [s for s in range(ns) if cnt[s] and algorithmic_bytes(names[s], cfg, npart) is not None]
+    dom = max(cands, key=lambda s: ms[s])
+    avg_ms = ms[dom] / cnt[dom]
+    abytes = algorithmic_bytes(names[dom], cfg, npart)
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": abytes, "mean_launch_ms": round(avg_ms, 4)}
+    value = npart * args.steps / wall
+    out = {
+        "metric": "particle-steps/s (PM+PP)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Zel'dovich state, EH no-wiggle LCDM)",
+        "config": {"workload": args.config, "particles": npart, "fine_mesh": cfg.nf, "coarse_mesh": cfg.nc,
+                   "box_mpc_h": cfg.box, "z_start": z0, "izipx": cfg.izipx, "izipv": cfg.izipv,
+                   "pp": bool(cfg.pp_on), "pp_range": cfg.pp_range, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "per_kernel_ms": per_kernel,
+    }
+    if not args.no_cpu_baseline:
+        st_cpu = cube.download()
+        cube.close()
+        threads = min(16, os.cpu_count() or 1)
+        tiles = args.cpu_tiles or min(threads, cfg.nnt ** 3)
+        out["cpu_baseline"] = cpu_baseline(cfg, st_cpu, float(cube.dt) or 0.05, tiles, threads)
+    else:
+        cube.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,6 +58,7 @@ typedef struct orc {
   int64_t *cstart;
   float sigma_vi, sigma_vi_new, vz_max, mass_p;
   float ftemp_pp; /* ext_pp_force.f90:18 'real ftemp' is SAVEd, never reset */
+  int64_t tile_limit; /* timing samples only: process tiles with index < tile_limit */
   float *kern_f;  /* (nfe/2+1, nfe, nfe, 3) Fortran order */
   float *kern_c;  /* (nc/2+1, nc, nc, 3) */
   float fk_table[16 * 16 * 16 * 3];
@@ -405,6 +406,7 @@ void orc_kernels(orc_t* o, float* kf, float* kc) {
 void orc_scalars(orc_t* o, float* out) {
   out[0] = o->sigma_vi; out[1] = o->sigma_vi_new; out[2] = o->vz_max; out[3] = o->mass_p; out[4] = o->ftemp_pp;
 }
+void orc_set_tile_limit(orc_t* o, int64_t n) { o->tile_limit = n; }
 void orc_set_scalars(orc_t* o, const float* in) {
   o->sigma_vi = in[0]; o->sigma_vi_new = in[1]; o->vz_max = in[2]; o->mass_p = in[3]; o->ftemp_pp = in[4];
 }
@@ -472,6 +474,7 @@ int orc_drift(orc_t* o, float dt_old, float dt, double* out, int threads) {
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
+        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         int32_t* rhoce = (int32_t*)calloc(E * E * E, sizeof(int32_t));
         float* vfn = (float*)calloc(3 * E * E * E, sizeof(float));
 #define EI(i, j, k) ((((k)-lo) * E + ((j)-lo)) * E + ((i)-lo))
@@ -528,6 +531,7 @@ int orc_drift(orc_t* o, float dt_old, float dt, double* out, int threads) {
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
+        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         int32_t* rholocal = (int32_t*)calloc(nt * nt * nt, sizeof(int32_t));
         for (int il = 0; il < nlayer; il++)
           for (int64_t k = 1 - ncb + il; k <= nt + ncb; k += nlayer)
@@ -566,6 +570,7 @@ int orc_drift(orc_t* o, float dt_old, float dt, double* out, int threads) {
   /* sigma_v statistics (update_particle.f90:173-226) */
   double sum_c = 0, sum_r = 0, sum_s = 0;
   for (int64_t r = 0; r < n3; r++) {
+    if (o->tile_limit > 0 && r / (nt * nt * nt) >= o->tile_limit) continue;
     float f0 = o->vfield[r * 3], f1 = o->vfield[r * 3 + 1], f2 = o->vfield[r * 3 + 2];
     sum_c += (double)(f0 * f0 + f1 * f1 + f2 * f2);
     for (int64_t ip = o->cstart[r]; ip < o->cstart[r + 1]; ip++) {
@@ -600,6 +605,7 @@ int orc_pp(orc_t* o, float a_mid, float dt, double* out, float* fpp_out, int thr
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
+        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         int64_t nitems = 0;
         for (int64_t k = 0; k <= nt + 1; k++)
           for (int64_t j = 0; j <= nt + 1; j++)
@@ -676,7 +682,7 @@ int orc_pp(orc_t* o, float a_mid, float dt, double* out, float* fpp_out, int thr
   o->ftemp_pp = ftemp;
   float dt_pp = 1.0f * sqrtf(1.0f) / (sqrtf(ftemp) * a_mid * GG);
   if (out) { out[0] = dt_pp; out[1] = (double)npairs_tot; out[2] = (double)itest1; out[3] = ftemp; }
-  if (itest1 != o->np) { snprintf(o->err, sizeof o->err, "itest1 %lld != nplocal %lld", (long long)itest1, (long long)o->np); return 3; }
+  if (itest1 != o->np && o->tile_limit == 0) { snprintf(o->err, sizeof o->err, "itest1 %lld != nplocal %lld", (long long)itest1, (long long)o->np); return 3; }
   return 0;
 }
 
@@ -704,6 +710,7 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
+        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         float* rho = (float*)calloc(n3f, sizeof(float));
         float* cre = (float*)malloc(sizeof(float) * hf * nfe * nfe);
         float* cim = (float*)malloc(sizeof(float) * hf * nfe * nfe);
@@ -820,6 +827,7 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
+        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         float* r3t = (float*)calloc(T * T * T, sizeof(float)); /* r3t(-1:nt+2)^3 */
 #define RT(i, j, k) r3t[(((k) + 1) * T + ((j) + 1)) * T + ((i) + 1)]
         for (int il = 0; il < 3; il++)
@@ -881,6 +889,7 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 64) reduction(max : vm0, vm1, vm2)
   for (int64_t r = 0; r < n3c; r++) {
     int64_t t = r / (nt * nt * nt), l = r % (nt * nt * nt);
+    if (o->tile_limit > 0 && t >= o->tile_limit) continue;
     int64_t txyz[3] = {t % nnt, (t / nnt) % nnt, t / (nnt * nnt)};
     int64_t c[3] = {l % nt + 1, (l / nt) % nt + 1, l / (nt * nt) + 1};
     for (int64_t ip = o->cstart[r]; ip < o->cstart[r + 1]; ip++) {
@@ -977,6 +986,7 @@ int orc_density(orc_t* o, float* rho_out, int threads) {
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
+        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         float* rho = (float*)calloc(nfe * nfe * nfe, sizeof(float));
 #define RF(i, j, k) rho[((int64_t)(k)-1) * nfe * nfe + ((int64_t)(j)-1) * nfe + ((i)-1)]
         for (int il = 0; il < 3; il++)

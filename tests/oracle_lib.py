@@ -49,6 +49,7 @@ def load():
         "orc_vencode": (_i32, [_f64, _f32, _i64]),
         "orc_fft_r2c": (None, [_P, _i64, _P, _P, ctypes.c_int]),
         "orc_density": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+        "orc_set_tile_limit": (None, [_P, _i64]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
