@@ -112,11 +112,11 @@ def main():
     cfg = CONFIGS[args.config]
     z0 = args.z if args.z is not None else 1.0
     t0 = time.perf_counter()
-    st = make_state(cfg, args.seed, z0)
-    log("state %s: %d particles at z=%.2f (%.1f s)" % (args.config, st.sim.nplocal, z0, time.perf_counter() - t0))
-    cube = Cube(cfg, st, device=0, z_checkpoints=[z0, 0.0])
+    cube = Cube.from_ic(cfg, args.seed, z0, device=0, z_checkpoints=[z0, 0.0])
+    npart = cube.sim.nplocal
+    log("state %s: %d particles, Zel'dovich at z=%.2f generated in HBM (%.1f s incl. init)"
+        % (args.config, npart, z0, time.perf_counter() - t0))
     lib = cube.lib
-    npart = st.sim.nplocal
     for i in range(args.warmup):
         cube.step()
         log("warmup step %d done" % (i + 1))

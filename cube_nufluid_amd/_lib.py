@@ -66,6 +66,11 @@ class PMOut(ctypes.Structure):
                 ("mass_f", c_f64), ("mass_c", c_f64)]
 
 
+class ICParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("np_nc", c_i32), ("nk", c_i32), ("a", c_f32), ("box", c_f32),
+                ("vfac", c_f64), ("lnk", ctypes.c_void_p), ("lnp", ctypes.c_void_p)]
+
+
 assert ctypes.sizeof(SimHeader) == 224
 assert ctypes.sizeof(CubeParams) == 80
 
@@ -84,6 +89,7 @@ SIGNATURES = {
     "cube_probe_pm_forces": (c_i32, [_P, _P, _P]),
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
+    "cube_ic_zeldovich": (c_i32, [_P, ctypes.POINTER(ICParams), ctypes.POINTER(SimHeader)]),
     "cube_profile_enable": (c_i32, [_P, c_i32]),
     "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),
     "cube_profile_slot_name": (c_i32, [c_i32, ctypes.c_char_p, ctypes.c_size_t]),

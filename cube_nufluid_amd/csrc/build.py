@@ -25,6 +25,7 @@ SOURCES = {
     "pp.hip": ["-ffp-contract=off"],
     "scan.hip": [],
     "fft.hip": [],
+    "ic.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

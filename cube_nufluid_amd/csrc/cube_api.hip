@@ -139,8 +139,8 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   TRY(hipMemset(h->uflags, 0, sizeof(uint32_t) * 64));
 #undef TRY
   if ((rc = codec_build(h))) return fail(rc);
-  if ((rc = mesh_alloc(h, h->fine, h->nf, h->prec, true))) return fail(rc);
-  if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, true))) return fail(rc);
+  if ((rc = mesh_alloc(h, h->fine, h->nf, h->prec, true, true))) return fail(rc);
+  if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, true, true))) return fail(rc);
   if ((rc = fine_kernel_build(h, kern_f_table))) return fail(rc);
   if ((rc = coarse_kernel_build(h, kern_c_table))) return fail(rc);
   *out = h;

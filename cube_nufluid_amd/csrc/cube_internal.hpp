@@ -179,7 +179,9 @@ __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int 
 int scan_counts(cube_handle* h, const int32_t* cnt, int64_t n, int64_t* cstart);
 int reduce_doubles(cube_handle* h, const double* part, int64_t nparts, int nfields, double* host_out);
 // fft.hip
-int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S);
+int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S, bool with_K);
+int mesh_pass(cube_handle* h, Mesh& m, void* buf, int axis, bool inverse);    // strided y (1) / z (2) pass
+int mesh_c2r(cube_handle* h, Mesh& m, const void* src, void* dst, int64_t opitch);
 void mesh_free(Mesh& m);
 int mesh_fwd_xy(cube_handle* h, Mesh& m);                  // R: X r2c then Y, in place
 int mesh_zconv(cube_handle* h, Mesh& m, int d);              // W = Zinv(i*K_d * Zfwd(R))

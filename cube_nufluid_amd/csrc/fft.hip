@@ -387,14 +387,14 @@ int mesh_inv_yx(cube_handle* h, Mesh& m, void* out, int smode, uint32_t* f2max) 
 }
 
 // -------------------------------------------------------------- allocation
-int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S) {
+int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S, bool with_K) {
   m.N = N;
   m.prec = prec;
   int64_t H = N / 2 + 1;
   HIP_TRY(h, hipMalloc(&m.R, m.bytes_R()));
   HIP_TRY(h, hipMalloc(&m.W, m.bytes_R()));
   if (with_S) HIP_TRY(h, hipMalloc(&m.S, sizeof(float) * N * N * N));
-  HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * H * H * prec));
+  if (with_K) HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * H * H * prec));
   HIP_TRY(h, hipMalloc(&m.tw, (size_t)N * 2 * prec));
   std::vector<double> tw(2 * N);
   for (int64_t j = 0; j < N; j++) {
@@ -562,7 +562,7 @@ __global__ void k_ck_correct(const double* kc, const double* ku, int64_t N, int 
 int coarse_kernel_build(cube_handle* h, const float* table) {
   Mesh tmp;
   int64_t N = h->coarse.N, H = N / 2 + 1, H3 = H * H * H;
-  int r = mesh_alloc(h, tmp, N, 8, false);
+  int r = mesh_alloc(h, tmp, N, 8, false, false);
   if (r) return r;
   float* dtab;
   double *kc, *ku;
@@ -588,4 +588,43 @@ int coarse_kernel_build(cube_handle* h, const float* table) {
   (void)hipFree(dtab); (void)hipFree(kc); (void)hipFree(ku);
   mesh_free(tmp);
   return 0;
+}
+
+// ------------------------------------------------------ generic passes
+// One strided pass (axis 1: y, axis 2: z) over a complex buffer of mesh m.
+template <typename T>
+static int pass_t(cube_handle* h, Mesh& m, void* buf, int axis, bool inv) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N;
+  cpx<T>* c = (cpx<T>*)buf;
+  const cpx<T>* tw = (const cpx<T>*)m.tw;
+  LOGN_SWITCH(logn, {
+    constexpr int WS = pick_w<T>(L);
+    dim3 g((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N);
+    if (axis == 1 && !inv) k_fft_strided<T, L, 1, false><<<g, NT_FFT, 0, h->stream>>>(c, tw);
+    if (axis == 1 && inv) k_fft_strided<T, L, 1, true><<<g, NT_FFT, 0, h->stream>>>(c, tw);
+    if (axis == 2 && !inv) k_fft_strided<T, L, 2, false><<<g, NT_FFT, 0, h->stream>>>(c, tw);
+    if (axis == 2 && inv) k_fft_strided<T, L, 2, true><<<g, NT_FFT, 0, h->stream>>>(c, tw);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_pass(cube_handle* h, Mesh& m, void* buf, int axis, bool inverse) {
+  return m.prec == 4 ? pass_t<float>(h, m, buf, axis, inverse) : pass_t<double>(h, m, buf, axis, inverse);
+}
+// C2R along x of a complex buffer into real rows of pitch opitch (no S epilogue).
+template <typename T>
+static int c2r_t(cube_handle* h, Mesh& m, const void* src, void* dst, int64_t opitch) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N, nrows = N * N;
+  LOGN_SWITCH(logn, {
+    constexpr int WX = pick_w<T>(L - 1);
+    k_fft_x_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
+        (const cpx<T>*)src, (T*)dst, opitch, (const cpx<T>*)m.tw, nrows, 0, nullptr, nullptr);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_c2r(cube_handle* h, Mesh& m, const void* src, void* dst, int64_t opitch) {
+  return m.prec == 4 ? c2r_t<float>(h, m, src, dst, opitch) : c2r_t<double>(h, m, src, dst, opitch);
 }

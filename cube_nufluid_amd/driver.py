@@ -37,9 +37,49 @@ class Cube:
         rc = self.lib.cube_init(ctypes.byref(p), _lib.ptr(kf), _lib.ptr(kc), ctypes.byref(h))
         self.h = h
         _lib.check(self.lib, h, rc)
+        if state is None:
+            raise ValueError("a state is required; use Cube.from_ic for a generated start")
         self.upload(state)
+        self._init_time(state.sim.a, z_checkpoints)
+
+    @classmethod
+    def from_ic(cls, cfg, seed, z, device=0, z_checkpoints=None, np_max=None):
+        """Generate seeded Zel'dovich initial conditions directly in HBM
+        (cube_ic_zeldovich) and return the resident box."""
+        from .ic import power_table, velocity_factor
+        self = cls.__new__(cls)
+        self.lib = _lib.load()
+        self.cfg = cfg
+        p = _lib.CubeParams()
+        p.nc, p.nnt, p.ncb = cfg.nc, cfg.nnt, cfg.ncb
+        p.izipx, p.izipv = cfg.izipx, cfg.izipv
+        p.np_max = int(np_max if np_max is not None else cfg.np_max)
+        p.pp_on, p.pp_range, p.rsoft = int(cfg.pp_on), cfg.pp_range, cfg.rsoft
+        p.omega_m, p.omega_l, p.wde = cfg.omega_m, cfg.omega_l, cfg.wde
+        p.precision, p.device = 4, device
+        self.params = p
+        kf, kc = read_kernel_tables()
+        h = ctypes.c_void_p()
+        rc = self.lib.cube_init(ctypes.byref(p), _lib.ptr(kf), _lib.ptr(kc), ctypes.byref(h))
+        self.h = h
+        _lib.check(self.lib, h, rc)
+        a = 1.0 / (1.0 + z)
+        lnk, lnp = power_table(cfg, a)
+        ip = _lib.ICParams()
+        ip.seed, ip.np_nc, ip.nk, ip.a, ip.box = seed, cfg.np_nc, len(lnk), a, cfg.box
+        ip.vfac = velocity_factor(cfg, a)
+        ip.lnk, ip.lnp = _lib.ptr(lnk), _lib.ptr(lnp)
+        sim = _lib.SimHeader()
+        _lib.check(self.lib, h, self.lib.cube_ic_zeldovich(h, ctypes.byref(ip), ctypes.byref(sim)))
+        sim.h0, sim.omega_m, sim.omega_l, sim.s8 = cfg.h0, cfg.omega_m, cfg.omega_l, cfg.sigma8
+        sim.vsim2phys = (150.0 / a) * cfg.box * np.sqrt(cfg.omega_m) / cfg.nf
+        self.sim = sim
+        self._init_time(np.float32(a), z_checkpoints or [z, 0.0])
+        return self
+
+    def _init_time(self, a0, z_checkpoints):
         # initialize.f90:40-53 and particle_initialization.f90:136-138
-        self.z_checkpoints = list(z_checkpoints) if z_checkpoints else [float(1.0 / state.sim.a - 1.0), 0.0]
+        self.z_checkpoints = list(z_checkpoints) if z_checkpoints else [float(1.0 / a0 - 1.0), 0.0]
         self.cur_checkpoint = 1
         self.a = f32(1.0 / (1.0 + f32(self.z_checkpoints[0])))
         self.t = f32(0)

@@ -12,6 +12,8 @@ Differences from the reference generator, which reads CAMB tables and
 sigma_v tables that the target configurations do not use:
   * the linear spectrum is the Eisenstein & Hu (1998) no-wiggle fit,
     eqs. (26)-(31), normalised to sigma_8 with a top-hat of 8 Mpc/h;
+  * the noise is SplitMix64 + Box-Muller per lattice site (reproducible on
+    the GPU by cube_ic_zeldovich);
   * the displacement is computed spectrally on the particle lattice,
     psi_k = i k delta_k / k^2, rather than by central differences of the
     fine-grid potential;
@@ -164,29 +166,65 @@ def encode_particles(cfg, pos, vel, pid, a, sigma_vi=None, z_i=None):
                     rhoc, np.ascontiguousarray(vfield))
 
 
+def splitmix64(x):
+    """SplitMix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    x = np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def lattice_noise(ng, seed):
+    """Unit Gaussian per lattice site (x fastest): Box-Muller of
+    SplitMix64(SplitMix64(seed) + site), the same draw cube_ic_zeldovich makes."""
+    seed0 = splitmix64(np.array([seed], dtype=np.uint64))[0]
+    out = np.empty(ng ** 3, dtype=np.float32)
+    chunk = 1 << 22
+    for s0 in range(0, ng ** 3, chunk):
+        idx = np.arange(s0, min(ng ** 3, s0 + chunk), dtype=np.uint64)
+        h = splitmix64(idx + seed0)
+        u1 = ((h >> np.uint64(32)).astype(np.float64) + 0.5) * (1.0 / 4294967296.0)
+        u2 = ((h & np.uint64(0xFFFFFFFF)).astype(np.float64) + 0.5) * (1.0 / 4294967296.0)
+        out[s0:s0 + len(idx)] = (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).astype(np.float32)
+    return out.reshape(ng, ng, ng)
+
+
+def power_table(cfg, a, nk=4096):
+    """ln k [h/Mpc] and ln P(k) [(Mpc/h)^3] at scale factor a on a log grid."""
+    lnk = np.linspace(np.log(1e-5), np.log(1e3), nk)
+    p = linear_power(cfg)(np.exp(lnk)) * dgrow(a, cfg.omega_m, cfg.omega_l) ** 2
+    return lnk, np.log(p)
+
+
+def velocity_factor(cfg, a):
+    om_a = cfg.omega_m / (cfg.omega_m + cfg.omega_l * a ** 3)
+    return a * a * hubble_sim(a, cfg.omega_m, cfg.omega_l) * om_a ** 0.55
+
+
 def zeldovich_ic(cfg, seed, z=None):
     """Gaussian field with the EH no-wiggle spectrum, Zel'dovich displacement
-    on the particle lattice, encoded as a BoxState."""
+    on the particle lattice, encoded as a BoxState (host version of
+    cube_ic_zeldovich: same noise, spectrum table and Nyquist treatment)."""
     z = cfg.z_i if z is None else z
     a = 1.0 / (1.0 + z)
     ng = cfg.nc * cfg.np_nc
-    rng = np.random.default_rng(seed)
-    noise = rng.standard_normal((ng, ng, ng), dtype=np.float32)
-    wk = np.fft.rfftn(noise)
-    del noise
+    wk = np.fft.rfftn(lattice_noise(ng, seed).astype(np.float64))
     kf = 2 * np.pi / cfg.box
-    k1 = np.fft.fftfreq(ng, d=1.0 / ng) * kf
-    kr = np.fft.rfftfreq(ng, d=1.0 / ng) * kf
+    k1 = np.fft.fftfreq(ng, d=1.0 / ng)
+    kr = np.fft.rfftfreq(ng, d=1.0 / ng)
     kz, ky, kx = np.meshgrid(k1, k1, kr, indexing="ij")
-    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    k2 = (kx ** 2 + ky ** 2 + kz ** 2) * kf * kf
     k2[0, 0, 0] = 1.0
-    pk = linear_power(cfg)(np.sqrt(k2)) * dgrow(a, cfg.omega_m, cfg.omega_l) ** 2
+    lnk, lnp = power_table(cfg, a)
+    pk = np.exp(np.interp(0.5 * np.log(k2), lnk, lnp))
     delta_k = wk * np.sqrt(ng ** 3 * pk / cfg.box ** 3)
     delta_k[0, 0, 0] = 0
     del wk, pk
     psi = np.empty((ng, ng, ng, 3), dtype=np.float64)
     for e, ke in enumerate((kx, ky, kz)):
-        psi[..., e] = np.fft.irfftn(1j * ke / k2 * delta_k, s=(ng, ng, ng), axes=(0, 1, 2))   # Mpc/h
+        fac = 1j * (ke * kf) / k2
+        fac[np.abs(ke) == ng // 2] = 0  # odd derivative on its own Nyquist plane
+        psi[..., e] = np.fft.irfftn(fac * delta_k, s=(ng, ng, ng), axes=(0, 1, 2))   # Mpc/h
     del delta_k, kx, ky, kz, k2
     psi *= cfg.nf / cfg.box                                                    # fine cells
     # lattice point i (0-based) at q = i/np_nc + 0.5/ncell coarse cells; array axes are (z,y,x)
@@ -195,6 +233,5 @@ def zeldovich_ic(cfg, seed, z=None):
     psi = psi.reshape(-1, 3)
     pid = (iz.reshape(-1).astype(np.int64) * ng + iy.reshape(-1)) * ng + ix.reshape(-1) + 1
     pos = q + psi / NCELL
-    om_a = cfg.omega_m / (cfg.omega_m + cfg.omega_l * a ** 3)
-    vel = psi * (a * a * hubble_sim(a, cfg.omega_m, cfg.omega_l) * om_a ** 0.55)
+    vel = psi * velocity_factor(cfg, a)
     return encode_particles(cfg, pos, vel, pid.astype(np.int32), np.float32(a), z_i=np.float32(z))

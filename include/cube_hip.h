@@ -156,6 +156,25 @@ int cube_probe_pp_forces(cube_handle* h, double* f_pp);
    x fastest) -- the periodic field pm.f90:76-107 builds tile by tile. */
 int cube_probe_density(cube_handle* h, float* rho);
 
+/* Seeded Zel'dovich initial conditions written straight into the resident
+   state -- the role of the reference's IC tool (util/initial_conditions.f90:
+   214-954), which writes CUBE's first checkpoint.  Noise: unit Gaussian per
+   lattice site from SplitMix64(SplitMix64(seed)+site) and Box-Muller;
+   displacement psi_k = i k/k^2 sqrt(ng^3 P(k)/box^3) noise_k on the
+   np_nc*nc lattice; x = q + psi, v = psi*vfac; cells in lattice order.
+   Replaces the state; sim_out (optional) receives the header. */
+typedef struct cube_ic_params {
+  uint64_t seed;
+  int32_t np_nc;      /* particles per coarse cell per dimension              */
+  int32_t nk;         /* entries of the power-spectrum table (>= 2)           */
+  float a;            /* scale factor of the state                            */
+  float box;          /* box size, Mpc/h                                      */
+  double vfac;        /* velocity per fine-cell displacement, a^2 H(a) f(a)   */
+  const double* lnk;  /* ln k, k in h/Mpc, increasing                         */
+  const double* lnp;  /* ln P(k), (Mpc/h)^3, at scale factor a                */
+} cube_ic_params;
+int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_sim_header* sim_out);
+
 /* Per-kernel timing with HIP events recorded on the handle's stream (no
    counterpart in the reference, whose only timers are the per-phase
    system_clock prints, e.g. pm.f90:45,182-183).  cube_profile_read returns
