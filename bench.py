@@ -112,7 +112,8 @@ def main():
     cfg = CONFIGS[args.config]
     z0 = args.z if args.z is not None else 1.0
     t0 = time.perf_counter()
-    cube = Cube.from_ic(cfg, args.seed, z0, device=0, z_checkpoints=[z0, 0.0])
+    # the next checkpoint redshift lies beyond the timed steps
+    cube = Cube.from_ic(cfg, args.seed, z0, device=0, z_checkpoints=[z0, min(0.0, z0) - 0.9])
     npart = cube.sim.nplocal
     log("state %s: %d particles, Zel'dovich at z=%.2f generated in HBM (%.1f s incl. init)"
         % (args.config, npart, z0, time.perf_counter() - t0))

@@ -126,6 +126,7 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   TRY(hipMalloc(&h->vp2, (size_t)3 * npm * p->izipv));
   TRY(hipMalloc(&h->pid2, sizeof(int32_t) * npm));
   TRY(hipMalloc(&h->disp, sizeof(uint16_t) * npm));
+  TRY(hipMalloc(&h->order, sizeof(int32_t) * 2 * npm));
   TRY(hipMalloc(&h->rhoc, sizeof(int32_t) * h->n3c));
   TRY(hipMalloc(&h->rhoc2, sizeof(int32_t) * h->n3c));
   TRY(hipMalloc(&h->vfield, sizeof(float) * 3 * h->n3c));
@@ -203,7 +204,7 @@ extern "C" int cube_finalize(cube_handle* h) {
     (void)hipEventDestroy(r.b);
   }
   for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
-  void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
+  void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->order, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
                   h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc};
   for (void* b : bufs)
     if (b) (void)hipFree(b);

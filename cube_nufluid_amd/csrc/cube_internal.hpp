@@ -52,6 +52,7 @@ struct cube_handle {
   void* xp = nullptr; void* vp = nullptr; int32_t* pid = nullptr;
   void* xp2 = nullptr; void* vp2 = nullptr; int32_t* pid2 = nullptr;
   uint16_t* disp = nullptr;   // packed cell displacement per particle (drift)
+  int32_t* order = nullptr;   // drift: arrival order, [0,np_max) particle index, [np_max,2np_max) source rank
   int32_t* rhoc = nullptr; float* vfield = nullptr; int64_t* cstart = nullptr;
   int32_t* rhoc2 = nullptr; float* vfield2 = nullptr; int64_t* cstart2 = nullptr;
   // codec tables
@@ -70,6 +71,7 @@ struct cube_handle {
   // scalars mirrored from CUBE's module state
   float sigma_vi = 0, sigma_vi_new = 0, vz_max = 0, mass_p = 0;
   float ftemp_pp = 0;         // ext_pp_force.f90:18 running max (SAVE)
+  uint32_t pp_overflow_blocks = 0;  // PP blocks handled by the chunked fallback in the last call
   cube_sim_header sim{};
   std::string err;
   // profiling

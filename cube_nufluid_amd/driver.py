@@ -156,7 +156,9 @@ class Cube:
         d1, d2 = self.expansion(self.a, dt)
         da = f32(d1 + d2)
         self.checkpoint_step = False
-        z_next = f32(self.z_checkpoints[self.cur_checkpoint - 1])
+        # initialize.f90:49 fills z_checkpoint with -0.5 beyond the list
+        zc = self.z_checkpoints
+        z_next = f32(zc[self.cur_checkpoint - 1] if self.cur_checkpoint <= len(zc) else -0.5)
         a_next = f32(f32(1.0) / f32(1 + z_next))
         if da >= f32(a_next - self.a):
             self.checkpoint_step = True
@@ -164,7 +166,7 @@ class Cube:
                 self.final_step = True
             it = 0
             while abs(f32(f32(self.a + da) / a_next) - 1) >= 1e-6 and it < 100:
-                dt = f32(dt * f32(f32(a_next - self.a) / da))
+                dt = f32(f32(dt * f32(a_next - self.a)) / da)  # dt*(a_next-a)/da, left to right
                 d1, d2 = self.expansion(self.a, dt)
                 da = f32(d1 + d2)
                 it += 1

@@ -112,3 +112,58 @@ def test_density_matches_oracle():
     o = orc.density()
     assert np.max(np.abs(g - o)) <= 1e-5 * np.max(np.abs(o))
     cube.close()
+
+
+def clustered_state(cfg, n_clump, n_bg, seed, sigma_cells=0.4, vel_scale=0.5):
+    from cube_nufluid_amd.ic import encode_particles
+    rng = np.random.default_rng(seed)
+    c = rng.normal(cfg.nc / 2 + 0.3, sigma_cells, size=(n_clump, 3))
+    b = rng.uniform(0, cfg.nc, size=(n_bg, 3))
+    pos = np.concatenate([c, b])
+    vel = rng.normal(0, vel_scale, size=pos.shape)
+    return encode_particles(cfg, pos, vel, np.arange(1, len(pos) + 1, dtype=np.int32), np.float32(0.5))
+
+
+def test_pp_clustered_overflow_path():
+    """A clump whose block holds more PP candidates than the binned kernel's
+    LDS table is handled by the chunked kernel with identical pairs."""
+    from cube_nufluid_amd.driver import Cube
+    from oracle_lib import Oracle
+    cfg = BoxConfig(nc=16, nnt=2, ncb=6, box=50.0, pp_on=True, pp_range=1)
+    st = clustered_state(cfg, 6000, 20000, seed=9)
+    cube = Cube(cfg, st)
+    orc = Oracle(cfg)
+    orc.set_state(st)
+    gp = cube.probe_pp_forces()
+    r = orc.pp(1.0, 1.0, forces=True)
+    assert rel_rms(gp, r["f_pp"].astype(np.float64)) < FORCE_RTOL
+    cube.a_mid, cube.dt = np.float32(0.2), np.float32(0.1)
+    o = cube.ext_pp_force()
+    assert o.npairs == r["npairs"] and o.itest1 == st.sim.nplocal
+    cube.close()
+
+
+@pytest.mark.parametrize("flow,dt", [((2.2, -1.7, 0.9), 2.0), ((1.5, 0.2, -1.2), 1.0)])
+def test_drift_large_displacements(flow, dt):
+    """Bulk flows moving particles ~2 and ~5 coarse cells per drift: the
+    LDS-ordered path and the global-scan path (source region beyond the LDS
+    table) give the oracle's result bit for bit."""
+    from cube_nufluid_amd.driver import Cube
+    from cube_nufluid_amd.ic import encode_particles
+    from oracle_lib import Oracle
+    cfg = BoxConfig(nc=32, nnt=2, ncb=6, box=100.0)
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, cfg.nc, size=(60000, 3))
+    vel = np.asarray(flow) * 4 + rng.normal(0, 0.8, size=pos.shape)
+    st = encode_particles(cfg, pos, vel, np.arange(1, 60001, dtype=np.int32), np.float32(0.5))
+    cube = Cube(cfg, st)
+    orc = Oracle(cfg)
+    orc.set_state(st)
+    cube.dt_old, cube.dt = np.float32(dt), np.float32(dt)
+    g = cube.update_x()
+    orc.drift(dt, dt)
+    print("max displacement", list(g.max_disp), flush=True)
+    a, b = cube.download(), orc.get_state()
+    for f in ("rhoc", "pid", "xp", "vp", "vfield"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+    cube.close()
