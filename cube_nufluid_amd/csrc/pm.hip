@@ -9,7 +9,7 @@
 // reference's weight for the same particle.
 #include "cube_internal.hpp"
 
-#define CIC_NT 512
+#define CIC_NT 256
 #define KICK_NT 256
 #define KICK_CH 256
 
@@ -48,11 +48,16 @@ __device__ __forceinline__ int seg_find(const int32_t* off, int n, int32_t t) {
 // Nodes sit on fine-cell left edges (pm.f90:87 has no -0.5); a particle in
 // coarse cell c touches fine nodes 4c..4c+4, so the block gathers cells
 // [c0-1, c0+BC) and keeps taps on its own nodes: every node is written once.
+// Each weight (the reference's float product) is accumulated as a 64-bit
+// fixed-point integer (2^-40 units), so node sums are exact and independent
+// of the order in which threads add them; the node value is the sum rounded
+// once to float.
+#define CIC_FIX 1099511627776.0  // 2^40
 template <int BC>
 __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
                                                      float mass, float* R, int64_t N, double* part) {
   constexpr int F = 4 * BC, SPAN = BC + 1, NCELL = SPAN * SPAN * SPAN;
-  __shared__ float s[F * F * F];
+  __shared__ unsigned long long s[F * F * F];
   __shared__ int64_t beg[NCELL];
   __shared__ int32_t off[NCELL + 1];
   __shared__ double red[CIC_NT / 64];
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
   const int c0x = bx * BC, c0y = by * BC, c0z = bz * BC;
   // block-local tile frame: 0-based offset of c0 inside its tile
   const int l0x = c0x % g.nt, l0y = c0y % g.nt, l0z = c0z % g.nt;
-  for (int i = threadIdx.x; i < F * F * F; i += CIC_NT) s[i] = 0.f;
+  for (int i = threadIdx.x; i < F * F * F; i += CIC_NT) s[i] = 0ull;
   load_region<SPAN>(g, cstart, c0x, c0y, c0z, 1, beg, off);
   const int32_t total = off[NCELL];
   for (int32_t t = threadIdx.x; t < total; t += CIC_NT) {
@@ -89,7 +94,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
     for (int q = 0; q < 8; q++) {
       int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
       if ((unsigned)x < (unsigned)F && (unsigned)y < (unsigned)F && (unsigned)z < (unsigned)F)
-        atomicAdd(&s[(z * F + y) * F + x], w[q]);
+        atomicAdd(&s[(z * F + y) * F + x], (unsigned long long)llrint((double)w[q] * CIC_FIX));
     }
   }
   __syncthreads();
@@ -97,9 +102,9 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
   for (int i = threadIdx.x; i < F * F * F; i += CIC_NT) {
     int x = i % F, y = (i / F) % F, z = i / (F * F);
     int64_t gx = 4 * c0x + x, gy = 4 * c0y + y, gz = 4 * c0z + z;
-    float v = s[i];
-    R[(gz * N + gy) * (N + 2) + gx] = v;
-    ms += (double)v;
+    double vd = (double)(long long)s[i] * (1.0 / CIC_FIX);
+    R[(gz * N + gy) * (N + 2) + gx] = (float)vd;
+    ms += vd;
   }
   for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ms;
@@ -118,7 +123,7 @@ template <int BC>
 __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
                                                        float mass, float* R, double* part) {
   constexpr int SPAN = BC + 2, NCELL = SPAN * SPAN * SPAN;
-  __shared__ float s[BC * BC * BC];
+  __shared__ unsigned long long s[BC * BC * BC];
   __shared__ int64_t beg[NCELL];
   __shared__ int32_t off[NCELL + 1];
   __shared__ double red[CIC_NT / 64];
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
   const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
   const int c0x = bx * BC, c0y = by * BC, c0z = bz * BC;
   const int l0x = c0x % g.nt, l0y = c0y % g.nt, l0z = c0z % g.nt;
-  for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) s[i] = 0.f;
+  for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) s[i] = 0ull;
   load_region<SPAN>(g, cstart, c0x, c0y, c0z, 1, beg, off);
   const int32_t total = off[NCELL];
   for (int32_t t = threadIdx.x; t < total; t += CIC_NT) {
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
     for (int q = 0; q < 8; q++) {
       int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
       if ((unsigned)x < (unsigned)BC && (unsigned)y < (unsigned)BC && (unsigned)z < (unsigned)BC)
-        atomicAdd(&s[(z * BC + y) * BC + x], w[q]);
+        atomicAdd(&s[(z * BC + y) * BC + x], (unsigned long long)llrint((double)w[q] * CIC_FIX));
     }
   }
   __syncthreads();
@@ -162,9 +167,9 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
   double ms = 0;
   for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) {
     int x = i % BC, y = (i / BC) % BC, z = i / (BC * BC);
-    float v = s[i];
-    R[((int64_t)(c0z + z) * N + (c0y + y)) * (N + 2) + (c0x + x)] = v;
-    ms += (double)v;
+    double vd = (double)(long long)s[i] * (1.0 / CIC_FIX);
+    R[((int64_t)(c0z + z) * N + (c0y + y)) * (N + 2) + (c0x + x)] = (float)vd;
+    ms += vd;
   }
   for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ms;
@@ -178,11 +183,11 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
 
 int cic_fine(cube_handle* h, double* mass) {
   Geo g = geo_of(h);
-  if (h->nt % 8) return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 8", (long long)h->nt);
-  int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
+  if (h->nt % 4) return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
+  int64_t nb = (h->nc / 4) * (h->nc / 4) * (h->nc / 4);
   if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   int pr = prof_begin(h, PS_CIC_F);
-  k_cic_fine<8><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
+  k_cic_fine<4><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);

@@ -9,7 +9,9 @@
 // the block and tests fine-cell adjacency exactly as the lists do, so the
 // pair set is identical.  Positions relative to the block are exact in
 // FP32 (they carry at most 19 significant bits), the pair force is FP32 and
-// the per-particle sum is FP64.  The kick and re-quantisation with the
+// the per-particle sum is a 64-bit fixed-point integer in 2^-32 units, so it
+// does not depend on the order in which candidates landed in LDS and the
+// result is the same on every run.  The kick and re-quantisation with the
 // current sigma (ext_pp_force.f90:90, 114-119) are applied in place.
 #include "cube_internal.hpp"
 #include <string.h>
@@ -18,6 +20,7 @@
 #define PP_NT 64
 #define PP_BP 2
 #define PP_CAP 1024
+#define PP_FIX 4294967296.0  // 2^32
 
 struct PPcand { float x, y, z; int32_t fc; };
 
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
         fi[e] = CUBE_NCELL * rel[e] + xsub(u, g.izipx);
       }
     }
-    double f[3] = {0, 0, 0};
+    long long fa[3] = {0, 0, 0};
     for (int32_t j0 = 0; j0 < ntot; j0 += PP_CAP) {
       if (threadIdx.x == 0) ncand = 0;
       __syncthreads();
@@ -139,13 +142,14 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
           float rc3 = rc * rc * rc;
           float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
           float s = mass * pc / (r2 * r);
-          f[0] += (double)(s * dx);
-          f[1] += (double)(s * dy);
-          f[2] += (double)(s * dz);
+          fa[0] += llrint((double)(s * dx) * PP_FIX);
+          fa[1] += llrint((double)(s * dy) * PP_FIX);
+          fa[2] += llrint((double)(s * dz) * PP_FIX);
         }
       }
       __syncthreads();
     }
+    const double f[3] = {(double)fa[0] * (1.0 / PP_FIX), (double)fa[1] * (1.0 / PP_FIX), (double)fa[2] * (1.0 / PP_FIX)};
     if (act) {
       if (probe) {
         for (int e = 0; e < 3; e++) probe[3 * pi + e] = f[e];
@@ -292,7 +296,7 @@ __global__ __launch_bounds__(PB_NT) void k_pp_binned(Geo g, const void* __restri
     const int fi[3] = {(int)floorf(pi.x) - lo, (int)floorf(pi.y) - lo, (int)floorf(pi.z) - lo};
     if (fi[0] < pr || fi[0] >= pr + FB || fi[1] < pr || fi[1] >= pr + FB || fi[2] < pr || fi[2] >= pr + FB) continue;
     ninner++;
-    double f[3] = {0, 0, 0};
+    long long fa[3] = {0, 0, 0};
     for (int kz = fi[2] - pr; kz <= fi[2] + pr; kz++)
       for (int ky = fi[1] - pr; ky <= fi[1] + pr; ky++) {
         const int rowc = (kz * W + ky) * W;
@@ -309,11 +313,12 @@ __global__ __launch_bounds__(PB_NT) void k_pp_binned(Geo g, const void* __restri
           float rc3 = rc * rc * rc;
           float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
           float sc = mass * pc / (r2 * r);
-          f[0] += (double)(sc * dx);
-          f[1] += (double)(sc * dy);
-          f[2] += (double)(sc * dz);
+          fa[0] += llrint((double)(sc * dx) * PP_FIX);
+          fa[1] += llrint((double)(sc * dy) * PP_FIX);
+          fa[2] += llrint((double)(sc * dz) * PP_FIX);
         }
       }
+    const double f[3] = {(double)fa[0] * (1.0 / PP_FIX), (double)fa[1] * (1.0 / PP_FIX), (double)fa[2] * (1.0 / PP_FIX)};
     const int64_t p = pi.p;
     if (probe) {
       for (int e = 0; e < 3; e++) probe[3 * p + e] = f[e];

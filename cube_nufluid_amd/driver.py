@@ -165,7 +165,7 @@ class Cube:
             if self.cur_checkpoint == len(self.z_checkpoints):
                 self.final_step = True
             it = 0
-            while abs(f32(f32(self.a + da) / a_next) - 1) >= 1e-6 and it < 100:
+            while abs(f32(f32(self.a + da) / a_next) - f32(1)) >= f32(1e-6) and it < 100:  # real(4) 1e-6
                 dt = f32(f32(dt * f32(a_next - self.a)) / da)  # dt*(a_next-a)/da, left to right
                 d1, d2 = self.expansion(self.a, dt)
                 da = f32(d1 + d2)
