@@ -111,7 +111,8 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
     return fail(cube_fail(h, CUBE_ERR_ARG, "izipx/izipv must be 1 or 2"));
   if (p->precision != 4)
     return fail(cube_fail(h, CUBE_ERR_ARG, "precision=%d: this build runs the FP32 path (4)", p->precision));
-  if (p->np_max < 1) return fail(cube_fail(h, CUBE_ERR_ARG, "np_max must be positive"));
+  if (p->np_max < 1 || p->np_max >= ((int64_t)1 << 31))
+    return fail(cube_fail(h, CUBE_ERR_ARG, "np_max=%lld outside [1, 2^31)", (long long)p->np_max));
   h->prec = p->precision;
   h->nf = h->nc * CUBE_NCELL;
   h->n3c = h->nc * h->nc * h->nc;
@@ -127,6 +128,7 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   TRY(hipMalloc(&h->pid2, sizeof(int32_t) * npm));
   TRY(hipMalloc(&h->disp, sizeof(uint16_t) * npm));
   TRY(hipMalloc(&h->order, sizeof(int32_t) * 2 * npm));
+  TRY(hipMalloc(&h->dbig, sizeof(int32_t) * (npm / 257 + 2)));
   TRY(hipMalloc(&h->rhoc, sizeof(int32_t) * h->n3c));
   TRY(hipMalloc(&h->rhoc2, sizeof(int32_t) * h->n3c));
   TRY(hipMalloc(&h->vfield, sizeof(float) * 3 * h->n3c));
@@ -142,6 +144,14 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   if ((rc = codec_build(h))) return fail(rc);
   if ((rc = mesh_alloc(h, h->fine, h->nf, h->prec, true, true))) return fail(rc);
   if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, true, true))) return fail(rc);
+  // the fine work mesh is idle during the drift and serves as its merge buffer
+  if (h->fine.bytes_R() >= sizeof(uint64_t) * (size_t)npm) {
+    h->dsort_tmp = h->fine.W;
+  } else {
+    if (hipMalloc(&h->dsort_tmp, sizeof(uint64_t) * (size_t)npm) != hipSuccess)
+      return fail(cube_fail(h, CUBE_ERR_HIP, "hipMalloc of the drift merge buffer failed"));
+    h->dsort_own = true;
+  }
   if ((rc = fine_kernel_build(h, kern_f_table))) return fail(rc);
   if ((rc = coarse_kernel_build(h, kern_c_table))) return fail(rc);
   *out = h;
@@ -165,7 +175,8 @@ extern "C" int cube_profile_enable(cube_handle* h, int on) {
   return CUBE_OK;
 }
 
-static const char* kSlotNames[PS_NSLOT] = {"drift_pass1", "drift_scan", "drift_pass3", "pp", "cic_coarse",
+static const char* kSlotNames[PS_NSLOT] = {"drift_pass1", "drift_scan", "drift_scatter", "drift_sort_big", "drift_write",
+                                           "pp", "cic_coarse",
                                            "fft_coarse", "cic_fine", "fft_x_r2c", "fft_y_fwd", "fft_zconv",
                                            "fft_y_inv", "fft_x_c2r", "kick"};
 
@@ -205,7 +216,8 @@ extern "C" int cube_finalize(cube_handle* h) {
   }
   for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
   void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->order, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
-                  h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc};
+                  h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc, h->dbig,
+                  h->dsort_own ? h->dsort_tmp : nullptr};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   mesh_free(h->fine);

@@ -37,7 +37,7 @@ struct Mesh {
 
 // Per-kernel timing with HIP events on the handle's stream (diagnostic).
 enum ProfSlot {
-  PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT3, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
+  PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT_SCATTER, PS_DRIFT_SORT, PS_DRIFT_WRITE, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
   PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_NSLOT
 };
 struct ProfRec { int slot; hipEvent_t a, b; };
@@ -52,7 +52,10 @@ struct cube_handle {
   void* xp = nullptr; void* vp = nullptr; int32_t* pid = nullptr;
   void* xp2 = nullptr; void* vp2 = nullptr; int32_t* pid2 = nullptr;
   uint16_t* disp = nullptr;   // packed cell displacement per particle (drift)
-  int32_t* order = nullptr;   // drift: arrival order, [0,np_max) particle index, [np_max,2np_max) source rank
+  int32_t* order = nullptr;   // 8*np_max bytes: drift arrival keys (uint64), PP overflow lists
+  int32_t* dbig = nullptr;    // drift: cells with more than 256 arrivals
+  void* dsort_tmp = nullptr;  // drift: merge buffer for crowded cells, 8*np_max bytes (aliases fine.W when it fits)
+  bool dsort_own = false;
   int32_t* rhoc = nullptr; float* vfield = nullptr; int64_t* cstart = nullptr;
   int32_t* rhoc2 = nullptr; float* vfield2 = nullptr; int64_t* cstart2 = nullptr;
   // codec tables

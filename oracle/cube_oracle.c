@@ -59,6 +59,7 @@ typedef struct orc {
   float sigma_vi, sigma_vi_new, vz_max, mass_p;
   float ftemp_pp; /* ext_pp_force.f90:18 'real ftemp' is SAVEd, never reset */
   int64_t tile_limit; /* timing samples only: process tiles with index < tile_limit */
+  double t_global;    /* seconds of the last PM call spent on box-wide coarse mesh work */
   float *kern_f;  /* (nfe/2+1, nfe, nfe, 3) Fortran order */
   float *kern_c;  /* (nc/2+1, nc, nc, 3) */
   float fk_table[16 * 16 * 16 * 3];
@@ -407,6 +408,7 @@ void orc_scalars(orc_t* o, float* out) {
   out[0] = o->sigma_vi; out[1] = o->sigma_vi_new; out[2] = o->vz_max; out[3] = o->mass_p; out[4] = o->ftemp_pp;
 }
 void orc_set_tile_limit(orc_t* o, int64_t n) { o->tile_limit = n; }
+double orc_global_seconds(const orc_t* o) { return o->t_global; }
 void orc_set_scalars(orc_t* o, const float* in) {
   o->sigma_vi = in[0]; o->sigma_vi_new = in[1]; o->vz_max = in[2]; o->mass_p = in[3]; o->ftemp_pp = in[4];
 }
@@ -469,43 +471,63 @@ int orc_drift(orc_t* o, float dt_old, float dt, double* out, int threads) {
   int32_t* rhoc_new = (int32_t*)calloc(n3, sizeof(int32_t));
   float* vf_new = (float*)calloc(3 * n3, sizeof(float));
   int bad = 0;
+  /* Tiles run in parallel when there are at least as many as threads;
+     otherwise (a timing sample of few tiles) tiles run one after another and
+     the planes k of one layer run in parallel, as the reference's own OpenMP
+     loop does (update_particle.f90:57-63): planes of a layer lie nlayer apart
+     and a particle moves less than nlayer/2 planes, so their destinations are
+     disjoint and each destination still sees its arrivals in order.  A tile
+     where some particle moves further is redone serially. */
+  const int64_t ntiles = nnt * nnt * nnt;
+  const int64_t active = o->tile_limit > 0 && o->tile_limit < ntiles ? o->tile_limit : ntiles;
+  const int par_tiles = active >= threads;
+  const int64_t hl = (nlayer - 1) / 2;
   /* pass 1 (update_particle.f90:57-85) per tile */
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3)
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3) if (par_tiles)
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
         if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
-        int32_t* rhoce = (int32_t*)calloc(E * E * E, sizeof(int32_t));
-        float* vfn = (float*)calloc(3 * E * E * E, sizeof(float));
+        int32_t* rhoce = (int32_t*)malloc(sizeof(int32_t) * E * E * E);
+        float* vfn = (float*)malloc(sizeof(float) * 3 * E * E * E);
 #define EI(i, j, k) ((((k)-lo) * E + ((j)-lo)) * E + ((i)-lo))
-        for (int64_t k = 1 - ncb; k <= nt + ncb; k++)
-          for (int64_t j = 1 - ncb; j <= nt + ncb; j++)
-            for (int64_t i = 1 - ncb; i <= nt + ncb; i++) {
-              int64_t r = rank_loc(o, tx, ty, tz, i, j, k);
-              for (int d = 0; d < 3; d++) vfn[EI(i, j, k) * 3 + d] = (float)((double)o->vfield[r * 3 + d] * wv);
-            }
-        for (int il = 0; il < nlayer; il++)
-          for (int64_t k = 1 - ncb + il; k <= nt + ncb; k += nlayer)
+        for (int attempt = par_tiles ? 1 : 0; attempt < 2; attempt++) {
+          const int inner = attempt == 0;
+          int far = 0;
+          memset(rhoce, 0, sizeof(int32_t) * E * E * E);
+          for (int64_t k = 1 - ncb; k <= nt + ncb; k++)
             for (int64_t j = 1 - ncb; j <= nt + ncb; j++)
               for (int64_t i = 1 - ncb; i <= nt + ncb; i++) {
                 int64_t r = rank_loc(o, tx, ty, tz, i, j, k);
-                int64_t c[3] = {i, j, k};
-                for (int64_t ip = o->cstart[r]; ip < o->cstart[r + 1]; ip++) {
-                  int64_t ig[3];
-                  double vreal[3];
-                  for (int d = 0; d < 3; d++) {
-                    double xq = ((double)c[d] - 1.0) + xoff(o->xp[ip * 3 + d], o->izipx);
-                    vreal[d] = orc_vdecode(o->vp[ip * 3 + d], sig, o->izipv) + (double)o->vfield[r * 3 + d];
-                    double deltax = ((double)dt_mid * vreal[d]) / (double)o->ncell;
-                    ig[d] = (int64_t)ceil(xq + deltax);
-                    if (ig[d] < lo || ig[d] >= lo + E) bad = 1;
-                  }
-                  if (ig[0] < lo || ig[0] >= lo + E || ig[1] < lo || ig[1] >= lo + E || ig[2] < lo || ig[2] >= lo + E) continue;
-                  int64_t e = EI(ig[0], ig[1], ig[2]);
-                  rhoce[e]++;
-                  for (int d = 0; d < 3; d++) vfn[e * 3 + d] = (float)((double)vfn[e * 3 + d] + vreal[d]);
-                }
+                for (int d = 0; d < 3; d++) vfn[EI(i, j, k) * 3 + d] = (float)((double)o->vfield[r * 3 + d] * wv);
               }
+          for (int il = 0; il < nlayer; il++) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) if (inner) reduction(| : far, bad)
+            for (int64_t k = 1 - ncb + il; k <= nt + ncb; k += nlayer)
+              for (int64_t j = 1 - ncb; j <= nt + ncb; j++)
+                for (int64_t i = 1 - ncb; i <= nt + ncb; i++) {
+                  int64_t r = rank_loc(o, tx, ty, tz, i, j, k);
+                  int64_t c[3] = {i, j, k};
+                  for (int64_t ip = o->cstart[r]; ip < o->cstart[r + 1]; ip++) {
+                    int64_t ig[3];
+                    double vreal[3];
+                    for (int d = 0; d < 3; d++) {
+                      double xq = ((double)c[d] - 1.0) + xoff(o->xp[ip * 3 + d], o->izipx);
+                      vreal[d] = orc_vdecode(o->vp[ip * 3 + d], sig, o->izipv) + (double)o->vfield[r * 3 + d];
+                      double deltax = ((double)dt_mid * vreal[d]) / (double)o->ncell;
+                      ig[d] = (int64_t)ceil(xq + deltax);
+                      if (ig[d] < lo || ig[d] >= lo + E) bad = 1;
+                    }
+                    if (ig[0] < lo || ig[0] >= lo + E || ig[1] < lo || ig[1] >= lo + E || ig[2] < lo || ig[2] >= lo + E) continue;
+                    if (ig[2] - k > hl || k - ig[2] > hl) far = 1;
+                    int64_t e = EI(ig[0], ig[1], ig[2]);
+                    rhoce[e]++;
+                    for (int d = 0; d < 3; d++) vfn[e * 3 + d] = (float)((double)vfn[e * 3 + d] + vreal[d]);
+                  }
+                }
+          }
+          if (!(inner && far)) break;
+        }
         for (int64_t e = 0; e < E * E * E; e++)
           for (int d = 0; d < 3; d++) vfn[e * 3 + d] = (float)((double)vfn[e * 3 + d] / ((double)rhoce[e] + wv));
         for (int64_t k = 1; k <= nt; k++)
@@ -527,40 +549,49 @@ int orc_drift(orc_t* o, float dt_old, float dt, double* out, int threads) {
   int16_t* vpn = (int16_t*)malloc(sizeof(int16_t) * 3 * (npn + 1));
   int32_t* pidn = (int32_t*)malloc(sizeof(int32_t) * (npn + 1));
   /* pass 2 (update_particle.f90:99-128) + compaction (:134-149) */
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3)
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3) if (par_tiles)
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
         if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
-        int32_t* rholocal = (int32_t*)calloc(nt * nt * nt, sizeof(int32_t));
-        for (int il = 0; il < nlayer; il++)
-          for (int64_t k = 1 - ncb + il; k <= nt + ncb; k += nlayer)
-            for (int64_t j = 1 - ncb; j <= nt + ncb; j++)
-              for (int64_t i = 1 - ncb; i <= nt + ncb; i++) {
-                int64_t r = rank_loc(o, tx, ty, tz, i, j, k);
-                int64_t c[3] = {i, j, k};
-                for (int64_t ip = o->cstart[r]; ip < o->cstart[r + 1]; ip++) {
-                  int64_t ig[3];
-                  double vreal[3];
-                  for (int d = 0; d < 3; d++) {
-                    double xq = ((double)c[d] - 1.0) + xoff(o->xp[ip * 3 + d], o->izipx);
-                    vreal[d] = orc_vdecode(o->vp[ip * 3 + d], sig, o->izipv) + (double)o->vfield[r * 3 + d];
-                    double deltax = ((double)dt_mid * vreal[d]) / (double)o->ncell;
-                    ig[d] = (int64_t)ceil(xq + deltax);
+        int32_t* rholocal = (int32_t*)malloc(sizeof(int32_t) * nt * nt * nt);
+        for (int attempt = par_tiles ? 1 : 0; attempt < 2; attempt++) {
+          const int inner = attempt == 0;
+          int far = 0;
+          memset(rholocal, 0, sizeof(int32_t) * nt * nt * nt);
+          for (int il = 0; il < nlayer; il++) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) if (inner) reduction(| : far)
+            for (int64_t k = 1 - ncb + il; k <= nt + ncb; k += nlayer)
+              for (int64_t j = 1 - ncb; j <= nt + ncb; j++)
+                for (int64_t i = 1 - ncb; i <= nt + ncb; i++) {
+                  int64_t r = rank_loc(o, tx, ty, tz, i, j, k);
+                  int64_t c[3] = {i, j, k};
+                  for (int64_t ip = o->cstart[r]; ip < o->cstart[r + 1]; ip++) {
+                    int64_t ig[3];
+                    double vreal[3];
+                    for (int d = 0; d < 3; d++) {
+                      double xq = ((double)c[d] - 1.0) + xoff(o->xp[ip * 3 + d], o->izipx);
+                      vreal[d] = orc_vdecode(o->vp[ip * 3 + d], sig, o->izipv) + (double)o->vfield[r * 3 + d];
+                      double deltax = ((double)dt_mid * vreal[d]) / (double)o->ncell;
+                      ig[d] = (int64_t)ceil(xq + deltax);
+                    }
+                    if (ig[0] < 1 || ig[0] > nt || ig[1] < 1 || ig[1] > nt || ig[2] < 1 || ig[2] > nt) continue;
+                    if (ig[2] - k > hl || k - ig[2] > hl) far = 1;
+                    int64_t rl = ((ig[2] - 1) * nt + (ig[1] - 1)) * nt + (ig[0] - 1);
+                    int64_t rd = rank_loc(o, tx, ty, tz, ig[0], ig[1], ig[2]);
+                    int64_t idx = cs_new[rd] + rholocal[rl]++;
+                    for (int d = 0; d < 3; d++) {
+                      double dd = (double)dt_mid * vreal[d] / (xres * (double)o->ncell);
+                      xpn[idx * 3 + d] = xwrap((int64_t)o->xp[ip * 3 + d] + (int64_t)nint_d(dd), o->izipx);
+                      double vr = vreal[d] - (double)vf_new[rd * 3 + d];
+                      vpn[idx * 3 + d] = (int16_t)orc_vencode(vr, sig, o->izipv);
+                    }
+                    pidn[idx] = o->pid[ip];
                   }
-                  if (ig[0] < 1 || ig[0] > nt || ig[1] < 1 || ig[1] > nt || ig[2] < 1 || ig[2] > nt) continue;
-                  int64_t rl = ((ig[2] - 1) * nt + (ig[1] - 1)) * nt + (ig[0] - 1);
-                  int64_t rd = rank_loc(o, tx, ty, tz, ig[0], ig[1], ig[2]);
-                  int64_t idx = cs_new[rd] + rholocal[rl]++;
-                  for (int d = 0; d < 3; d++) {
-                    double dd = (double)dt_mid * vreal[d] / (xres * (double)o->ncell);
-                    xpn[idx * 3 + d] = xwrap((int64_t)o->xp[ip * 3 + d] + (int64_t)nint_d(dd), o->izipx);
-                    double vr = vreal[d] - (double)vf_new[rd * 3 + d];
-                    vpn[idx * 3 + d] = (int16_t)orc_vencode(vr, sig, o->izipv);
-                  }
-                  pidn[idx] = o->pid[ip];
                 }
-              }
+          }
+          if (!(inner && far)) break;
+        }
         free(rholocal);
       }
 #undef EI
@@ -601,7 +632,10 @@ int orc_pp(orc_t* o, float a_mid, float dt, double* out, float* fpp_out, int thr
   const float GG = 1.0f / 6.0f / PI_F;
   int64_t npairs_tot = 0, itest1 = 0;
   float ftemp = o->ftemp_pp;
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3) reduction(+ : npairs_tot, itest1) reduction(max : ftemp)
+  const int64_t ntiles = nnt * nnt * nnt;
+  const int64_t active = o->tile_limit > 0 && o->tile_limit < ntiles ? o->tile_limit : ntiles;
+  const int par_tiles = active >= threads; /* else: parallel over the fine planes of one tile */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3) reduction(+ : npairs_tot, itest1) reduction(max : ftemp) if (par_tiles)
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
@@ -640,6 +674,8 @@ int orc_pp(orc_t* o, float a_mid, float dt, double* out, float* fpp_out, int thr
             }
         int64_t np_t = 0, it_t = 0;
         float ft = 0;
+        /* each particle's force reads positions only and its kick writes its own velocity */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : np_t, it_t) reduction(max : ft) if (!par_tiles)
         for (int64_t gz = 1; gz <= nft; gz++)
           for (int64_t gy = 1; gy <= nft; gy++)
             for (int64_t gx = 1; gx <= nft; gx++) {
@@ -701,7 +737,9 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
   (void)nft;
   double testrhof = 0;
   float f2f = 0;
-  int inner = threads / (int)(nnt * nnt * nnt);
+  const int64_t ntiles = nnt * nnt * nnt;
+  const int64_t active = o->tile_limit > 0 && o->tile_limit < ntiles ? o->tile_limit : ntiles;
+  int inner = threads / (int)active;
   if (inner < 1) inner = 1;
   int outer = threads / inner;
   if (outer < 1) outer = 1;
@@ -864,6 +902,7 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
 #undef RT
         free(r3t);
       }
+  const double t_glob0 = omp_get_wtime();
   double testrhoc = 0;
   for (int64_t q = 0; q < n3c; q++) testrhoc += (double)r3[q];
   float* cre = (float*)malloc(sizeof(float) * hc * nc * nc);
@@ -883,6 +922,7 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
     float f = fc[q * 3] * fc[q * 3] + fc[q * 3 + 1] * fc[q * 3 + 1] + fc[q * 3 + 2] * fc[q * 3 + 2];
     if (f > f2c) f2c = f;
   }
+  o->t_global = omp_get_wtime() - t_glob0;
   float vmax[3] = {0, 0, 0};
 #define FC(d, x, y, z) fc[((wrapi((z)-1, nc) * nc + wrapi((y)-1, nc)) * nc + wrapi((x)-1, nc)) * 3 + (d)]
   float vm0 = 0, vm1 = 0, vm2 = 0;

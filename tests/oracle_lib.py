@@ -50,6 +50,7 @@ def load():
         "orc_fft_r2c": (None, [_P, _i64, _P, _P, ctypes.c_int]),
         "orc_density": (ctypes.c_int, [_P, _P, ctypes.c_int]),
         "orc_set_tile_limit": (None, [_P, _i64]),
+        "orc_global_seconds": (_f64, [_P]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)

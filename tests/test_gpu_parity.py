@@ -167,3 +167,25 @@ def test_drift_large_displacements(flow, dt):
     for f in ("rhoc", "pid", "xp", "vp", "vfield"):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
     cube.close()
+
+
+@pytest.mark.parametrize("n_clump", [3000, 40000])
+def test_drift_crowded_cells(n_clump):
+    """Cells receiving more than 256 (workgroup sort) and more than 8192
+    (LDS chunks + merge passes) arrivals keep the oracle's order bit for bit."""
+    from cube_nufluid_amd.driver import Cube
+    from oracle_lib import Oracle
+    cfg = BoxConfig(nc=16, nnt=2, ncb=6, box=50.0)
+    st = clustered_state(cfg, n_clump, 20000, seed=11, sigma_cells=0.15, vel_scale=1.5)
+    print("crowded", n_clump, "max cell", int(st.rhoc.max()), flush=True)
+    cube = Cube(cfg, st)
+    orc = Oracle(cfg)
+    orc.set_state(st)
+    cube.dt_old, cube.dt = np.float32(0.3), np.float32(0.3)
+    cube.update_x()
+    orc.drift(0.3, 0.3)
+    a, b = cube.download(), orc.get_state()
+    assert b.rhoc.max() > (8192 if n_clump > 10000 else 256)
+    for f in ("rhoc", "pid", "xp", "vp", "vfield"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+    cube.close()
