@@ -155,6 +155,9 @@ def main():
         names.append(b.value.decode())
     per_kernel = {names[s]: {"ms_total": round(float(ms[s]), 3), "launches": int(cnt[s])} for s in range(ns) if cnt[s]}
     log("per-kernel: " + json.dumps(per_kernel))
+    if cfg.pp_on:
+        lp = cube.last_pp
+        log("pp: npairs %d (%.1f per particle), dense blocks %d" % (lp.npairs, lp.npairs / npart, lp.dense_blocks))
     cands = [s for s in range(ns) if cnt[s] and algorithmic_bytes(names[s], cfg, npart) is not None]
     dom = max(cands, key=lambda s: ms[s])
     avg_ms = ms[dom] / cnt[dom]

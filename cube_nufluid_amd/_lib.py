@@ -57,7 +57,7 @@ class DriftOut(ctypes.Structure):
 
 
 class PPOut(ctypes.Structure):
-    _fields_ = [("dt_pp", c_f32), ("f2_max_pp", c_f32), ("npairs", c_i64), ("itest1", c_i64)]
+    _fields_ = [("dt_pp", c_f32), ("f2_max_pp", c_f32), ("npairs", c_i64), ("itest1", c_i64), ("dense_blocks", c_i64)]
 
 
 class PMOut(ctypes.Structure):

@@ -128,7 +128,12 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   TRY(hipMalloc(&h->pid2, sizeof(int32_t) * npm));
   TRY(hipMalloc(&h->disp, sizeof(uint16_t) * npm));
   TRY(hipMalloc(&h->order, sizeof(int32_t) * 2 * npm));
-  TRY(hipMalloc(&h->dbig, sizeof(int32_t) * (npm / 257 + 2)));
+  TRY(hipMalloc(&h->dbig, sizeof(int32_t) * (npm / 1025 + 2)));
+  {  // PP: per 2^3-cell block 2 int64 info + 2 int32 lists, plus work items
+    const int64_t nb = (h->nc / 2) * (h->nc / 2) * (h->nc / 2);
+    TRY(hipMalloc(&h->pp_meta, sizeof(int64_t) * (2 * nb + npm / 256 + nb + 1) + sizeof(int32_t) * 2 * nb));
+    TRY(hipMalloc(&h->pp_red, 1024 * (2 * sizeof(uint64_t) + sizeof(uint32_t))));
+  }
   TRY(hipMalloc(&h->rhoc, sizeof(int32_t) * h->n3c));
   TRY(hipMalloc(&h->rhoc2, sizeof(int32_t) * h->n3c));
   TRY(hipMalloc(&h->vfield, sizeof(float) * 3 * h->n3c));
@@ -144,13 +149,14 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   if ((rc = codec_build(h))) return fail(rc);
   if ((rc = mesh_alloc(h, h->fine, h->nf, h->prec, true, true))) return fail(rc);
   if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, true, true))) return fail(rc);
-  // the fine work mesh is idle during the drift and serves as its merge buffer
-  if (h->fine.bytes_R() >= sizeof(uint64_t) * (size_t)npm) {
-    h->dsort_tmp = h->fine.W;
+  // the fine work mesh is idle during the drift and serves as its scratch
+  // (crowded-cell merges, then three doubles per particle)
+  if (h->fine.bytes_R() >= 3 * sizeof(double) * (size_t)npm) {
+    h->dscratch = h->fine.W;
   } else {
-    if (hipMalloc(&h->dsort_tmp, sizeof(uint64_t) * (size_t)npm) != hipSuccess)
-      return fail(cube_fail(h, CUBE_ERR_HIP, "hipMalloc of the drift merge buffer failed"));
-    h->dsort_own = true;
+    if (hipMalloc(&h->dscratch, 3 * sizeof(double) * (size_t)npm) != hipSuccess)
+      return fail(cube_fail(h, CUBE_ERR_HIP, "hipMalloc of the drift scratch failed"));
+    h->dscratch_own = true;
   }
   if ((rc = fine_kernel_build(h, kern_f_table))) return fail(rc);
   if ((rc = coarse_kernel_build(h, kern_c_table))) return fail(rc);
@@ -217,7 +223,7 @@ extern "C" int cube_finalize(cube_handle* h) {
   for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
   void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->order, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
                   h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc, h->dbig,
-                  h->dsort_own ? h->dsort_tmp : nullptr};
+                  h->dscratch_own ? h->dscratch : nullptr, h->pp_meta, h->pp_red};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   mesh_free(h->fine);

@@ -53,9 +53,9 @@ struct cube_handle {
   void* xp2 = nullptr; void* vp2 = nullptr; int32_t* pid2 = nullptr;
   uint16_t* disp = nullptr;   // packed cell displacement per particle (drift)
   int32_t* order = nullptr;   // 8*np_max bytes: drift arrival keys (uint64), PP overflow lists
-  int32_t* dbig = nullptr;    // drift: cells with more than 256 arrivals
-  void* dsort_tmp = nullptr;  // drift: merge buffer for crowded cells, 8*np_max bytes (aliases fine.W when it fits)
-  bool dsort_own = false;
+  int32_t* dbig = nullptr;    // drift: cells with more than 1024 arrivals
+  void* dscratch = nullptr;   // drift scratch, 24*np_max bytes (aliases fine.W when it fits)
+  bool dscratch_own = false;
   int32_t* rhoc = nullptr; float* vfield = nullptr; int64_t* cstart = nullptr;
   int32_t* rhoc2 = nullptr; float* vfield2 = nullptr; int64_t* cstart2 = nullptr;
   // codec tables
@@ -75,6 +75,8 @@ struct cube_handle {
   float sigma_vi = 0, sigma_vi_new = 0, vz_max = 0, mass_p = 0;
   float ftemp_pp = 0;         // ext_pp_force.f90:18 running max (SAVE)
   uint32_t pp_overflow_blocks = 0;  // PP blocks handled by the chunked fallback in the last call
+  void* pp_meta = nullptr;    // PP dense-block lists and work items
+  void* pp_red = nullptr;     // PP per-slot totals (1024 slots)
   cube_sim_header sim{};
   std::string err;
   // profiling
@@ -113,26 +115,43 @@ int codec_build(cube_handle* h);  // vdec/venc tables from the host libm
 // ------------------------------------------------------------- device helpers
 struct Geo {  // box geometry passed by value to kernels
   int32_t nc, nt, nnt, ncb, izipx, izipv;
+  int32_t lnt, lnnt;  // log2 of nt and nnt when they are powers of two, else -1
 };
 
-__device__ __forceinline__ int32_t wrapc(int32_t a, int32_t n) {
-  a %= n;
-  return a < 0 ? a + n : a;
+// Cell indices stay below 2^27 (nc <= 512), so index arithmetic is 32-bit;
+// nc is a power of two and so, in the usual layouts, are nt and nnt.
+__host__ __device__ __forceinline__ int32_t wrapc(int32_t a, int32_t n) { return a & (n - 1); }
+__host__ __device__ __forceinline__ void divmod_nt(const Geo& g, uint32_t a, uint32_t& q, uint32_t& r) {
+  if (g.lnt >= 0) { q = a >> g.lnt; r = a & ((1u << g.lnt) - 1); }
+  else { q = a / (uint32_t)g.nt; r = a - q * (uint32_t)g.nt; }
 }
 // checkpoint-order rank of global coarse cell (0-based coords)
-__device__ __forceinline__ int64_t rank_of(const Geo& g, int32_t x, int32_t y, int32_t z) {
-  int64_t t = ((int64_t)(z / g.nt) * g.nnt + y / g.nt) * g.nnt + x / g.nt;
-  int64_t nt = g.nt;
-  return t * nt * nt * nt + ((int64_t)(z % g.nt) * nt + y % g.nt) * nt + x % g.nt;
+__host__ __device__ __forceinline__ int64_t rank_of(const Geo& g, int32_t x, int32_t y, int32_t z) {
+  uint32_t tx, lx, ty, ly, tz, lz;
+  divmod_nt(g, (uint32_t)x, tx, lx);
+  divmod_nt(g, (uint32_t)y, ty, ly);
+  divmod_nt(g, (uint32_t)z, tz, lz);
+  const uint32_t nt = (uint32_t)g.nt, nnt = (uint32_t)g.nnt;
+  const uint32_t t = (tz * nnt + ty) * nnt + tx;
+  return (int64_t)(t * (nt * nt * nt) + (lz * nt + ly) * nt + lx);
 }
-__device__ __forceinline__ void cell_of_rank(const Geo& g, int64_t r, int32_t* c) {
-  int64_t nt3 = (int64_t)g.nt * g.nt * g.nt;
-  int64_t t = r / nt3, l = r - t * nt3;
-  int32_t tx = (int32_t)(t % g.nnt), ty = (int32_t)((t / g.nnt) % g.nnt), tz = (int32_t)(t / ((int64_t)g.nnt * g.nnt));
-  int32_t lx = (int32_t)(l % g.nt), ly = (int32_t)((l / g.nt) % g.nt), lz = (int32_t)(l / ((int64_t)g.nt * g.nt));
-  c[0] = tx * g.nt + lx;
-  c[1] = ty * g.nt + ly;
-  c[2] = tz * g.nt + lz;
+__host__ __device__ __forceinline__ void cell_of_rank(const Geo& g, int64_t r64, int32_t* c) {
+  const uint32_t r = (uint32_t)r64, nt = (uint32_t)g.nt, nnt = (uint32_t)g.nnt;
+  uint32_t t, l, tx, ty, tz, lx, ly, lz;
+  if (g.lnt >= 0 && g.lnnt >= 0) {
+    const uint32_t a = g.lnt, b = g.lnnt;
+    t = r >> (3 * a); l = r & ((1u << (3 * a)) - 1);
+    tx = t & ((1u << b) - 1); ty = (t >> b) & ((1u << b) - 1); tz = t >> (2 * b);
+    lx = l & (nt - 1); ly = (l >> a) & (nt - 1); lz = l >> (2 * a);
+  } else {
+    const uint32_t nt3 = nt * nt * nt;
+    t = r / nt3; l = r - t * nt3;
+    tx = t % nnt; ty = (t / nnt) % nnt; tz = t / (nnt * nnt);
+    lx = l % nt; ly = (l / nt) % nt; lz = l / (nt * nt);
+  }
+  c[0] = (int32_t)(tx * nt + lx);
+  c[1] = (int32_t)(ty * nt + ly);
+  c[2] = (int32_t)(tz * nt + lz);
 }
 __device__ __forceinline__ uint32_t load_xu(const void* xp, int64_t q, int izipx) {
   return izipx == 1 ? (uint32_t)(uint8_t)((const int8_t*)xp)[q] : (uint32_t)(uint16_t)((const int16_t*)xp)[q];
@@ -208,6 +227,8 @@ static inline Geo geo_of(const cube_handle* h) {
   Geo g;
   g.nc = (int32_t)h->nc; g.nt = (int32_t)h->nt; g.nnt = (int32_t)h->nnt; g.ncb = (int32_t)h->ncb;
   g.izipx = (int32_t)h->p.izipx; g.izipv = (int32_t)h->p.izipv;
+  auto lg = [](int64_t n) { int l = 0; while (((int64_t)1 << l) < n) l++; return ((int64_t)1 << l) == n ? l : -1; };
+  g.lnt = lg(h->nt); g.lnnt = lg(h->nnt);
   return g;
 }
 static inline double vcoef(float sigma) {  // sqrt(pi/2)/(sigma*vrel_boost)

@@ -29,9 +29,28 @@
 #define DR_NT 256
 #define DR_CH 256
 #define DMAX 7
+#define DR_LBITS 20  // particles per source cell < 2^20 (sort key field)
+#define DR_RBITS 27  // coarse cells < 2^27 (nc <= 512)
 
 __device__ __forceinline__ uint16_t pack_disp(int dx, int dy, int dz) {
   return (uint16_t)((dx + 8) | ((dy + 8) << 4) | ((dz + 8) << 8));
+}
+
+// Lanes of a wave hold consecutive particles, so equal destination cells in
+// neighbouring lanes form runs; one atomic per run instead of per particle.
+// key < 0 marks an idle lane (made unique per lane by the caller).
+struct WaveRun { int head, len; };
+__device__ __forceinline__ WaveRun wave_run(int64_t key) {
+  const int lane = threadIdx.x & 63;
+  const int64_t prev = __shfl_up(key, 1);
+  const bool is_head = lane == 0 || key != prev;
+  const unsigned long long H = __ballot(is_head);
+  const unsigned long long upto = ~0ull >> (63 - lane);  // bits 0..lane
+  const unsigned long long above = H & ~upto;
+  WaveRun w;
+  w.head = 63 - __clzll(H & upto);
+  w.len = (above ? __ffsll((long long)above) - 1 : 64) - lane;  // meaningful in the head lane
+  return w;
 }
 
 __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __restrict__ xp, const void* __restrict__ vp,
@@ -50,52 +69,64 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
   if (threadIdx.x < 3) smax[threadIdx.x] = 0;
   if (threadIdx.x == 0) { sbad = 0; sconf = 0; }
   __syncthreads();
+  const int lane = threadIdx.x & 63;
   int lmax[3] = {0, 0, 0}, bad = 0, conf = 0;
-  for (int64_t p = cs[0] + threadIdx.x; p < cs[DR_CH]; p += DR_NT) {
-    int lo = 0, hi = DR_CH;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (cs[mid] <= p) lo = mid; else hi = mid;
-    }
-    int64_t r = r0 + lo;
-    int32_t c[3];
-    cell_of_rank(g, r, c);
-    int del[3];
-    for (int e = 0; e < 3; e++) {
-      int il = c[e] % g.nt + 1;  // tile-local 1-based source index
-      double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
-      double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[r * 3 + e];
-      double deltax = ((double)dt_mid * vreal) / (double)CUBE_NCELL;
-      int ig = (int)ceil(((double)il - 1.0) + xo + deltax);
-      int d = ig - il;
-      if (ig < 1 || ig > g.nt) {
-        // the destination lies in the neighbouring tile: that tile computes
-        // this particle from its own frame (source index il -+ nt)
-        int sl = ig < 1 ? il + g.nt : il - g.nt;
-        int ig2 = (int)ceil(((double)sl - 1.0) + xo + deltax);
-        int d2 = ig2 - sl;
-        if (d2 != d) {
-          conf++;
-          if (ig2 >= 1 && ig2 <= g.nt) d = d2;
-        }
+  for (int64_t pb = cs[0]; pb < cs[DR_CH]; pb += DR_NT) {
+    const int64_t p = pb + threadIdx.x;
+    int64_t rd = -1 - lane;
+    if (p < cs[DR_CH]) {
+      int lo = 0, hi = DR_CH;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (cs[mid] <= p) lo = mid; else hi = mid;
       }
-      del[e] = d;
-      int ad = d < 0 ? -d : d;
-      if (ad > dmax) bad = 1;
-      lmax[e] = max(lmax[e], ad);
+      if (cs[lo + 1] - cs[lo] > ((int64_t)1 << DR_LBITS)) bad |= 2;
+      const int64_t r = r0 + lo;
+      int32_t c[3];
+      cell_of_rank(g, r, c);
+      int del[3];
+      bool far = false;
+      for (int e = 0; e < 3; e++) {
+        int il = (c[e] & (g.nt - 1)) + 1;  // tile-local 1-based source index
+        if (g.lnt < 0) il = c[e] % g.nt + 1;
+        double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
+        double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[r * 3 + e];
+        double deltax = ((double)dt_mid * vreal) / (double)CUBE_NCELL;
+        int ig = (int)ceil(((double)il - 1.0) + xo + deltax);
+        int d = ig - il;
+        if (ig < 1 || ig > g.nt) {
+          // the destination lies in the neighbouring tile: that tile computes
+          // this particle from its own frame (source index il -+ nt)
+          int sl = ig < 1 ? il + g.nt : il - g.nt;
+          int ig2 = (int)ceil(((double)sl - 1.0) + xo + deltax);
+          int d2 = ig2 - sl;
+          if (d2 != d) {
+            conf++;
+            if (ig2 >= 1 && ig2 <= g.nt) d = d2;
+          }
+        }
+        del[e] = d;
+        int ad = d < 0 ? -d : d;
+        if (ad > dmax) far = true;
+        lmax[e] = max(lmax[e], ad);
+      }
+      if (far) {
+        bad |= 1;
+      } else {
+        disp[p] = pack_disp(del[0], del[1], del[2]);
+        rd = rank_of(g, wrapc(c[0] + del[0], g.nc), wrapc(c[1] + del[1], g.nc), wrapc(c[2] + del[2], g.nc));
+      }
     }
-    if (bad) continue;
-    disp[p] = pack_disp(del[0], del[1], del[2]);
-    int64_t rd = rank_of(g, wrapc(c[0] + del[0], g.nc), wrapc(c[1] + del[1], g.nc), wrapc(c[2] + del[2], g.nc));
-    atomicAdd(&cnt_new[rd], 1);
+    const WaveRun w = wave_run(rd);
+    if (rd >= 0 && w.head == lane) atomicAdd(&cnt_new[rd], w.len);
   }
   for (int e = 0; e < 3; e++) atomicMax(&smax[e], lmax[e]);
-  if (bad) atomicOr(&sbad, 1);
+  if (bad) atomicOr(&sbad, bad);
   if (conf) atomicAdd(&sconf, conf);
   __syncthreads();
   if (threadIdx.x < 3) atomicMax(&flags[threadIdx.x], (uint32_t)smax[threadIdx.x]);
   if (threadIdx.x == 0) {
-    if (sbad) atomicOr(&flags[3], 1u);
+    if (sbad) atomicOr(&flags[3], (uint32_t)sbad);
     if (sconf) atomicAdd(&flags[4], (uint32_t)sconf);
   }
 }
@@ -103,24 +134,23 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
 // ---------------------------------------------------------------- scatter
 // Every arrival gets a 64-bit key in its destination cell's segment of
 // `keys` (cs_new order):
-//   bits 44..63  ilayer of the source z-plane in the destination tile's frame
-//   bits 32..43  (7-dz, 7-dy, 7-dx), 4 bits each: source plane, row, column
+//   bits 59..63  ilayer of the source z-plane in the destination tile's frame
+//   bits 47..58  (7-dz, 7-dy, 7-dx), 4 bits each: source plane, row, column
 //                ascending in that frame
-//   bits  0..31  source particle index (its position l inside the source cell)
+//   bits 27..46  l, the particle's position inside its source cell
+//   bits  0..26  source cell rank (fixed by the destination and dz,dy,dx)
 // Sorting a segment by key gives the reference's arrival order
 // ilayer -> k -> j -> i -> l (update_particle.f90:99-105).  Slots come from
 // atomics and so are in no particular order before the sort.
-__device__ __forceinline__ uint64_t arrival_key(int ilayer, int dx, int dy, int dz, int64_t p) {
-  return ((uint64_t)ilayer << 44) | ((uint64_t)(7 - dz) << 40) | ((uint64_t)(7 - dy) << 36) | ((uint64_t)(7 - dx) << 32) |
-         (uint64_t)(uint32_t)p;
+__device__ __forceinline__ uint64_t arrival_key(int ilayer, int dx, int dy, int dz, int64_t l, int64_t rs) {
+  return ((uint64_t)ilayer << 59) | ((uint64_t)(7 - dz) << 55) | ((uint64_t)(7 - dy) << 51) | ((uint64_t)(7 - dx) << 47) |
+         ((uint64_t)l << DR_RBITS) | (uint64_t)rs;
 }
-__device__ __forceinline__ void key_disp(uint64_t k, int d[3]) {
-  d[0] = 7 - (int)((k >> 32) & 15);
-  d[1] = 7 - (int)((k >> 36) & 15);
-  d[2] = 7 - (int)((k >> 40) & 15);
-}
+__device__ __forceinline__ int64_t key_rank(uint64_t k) { return (int64_t)(k & ((1ull << DR_RBITS) - 1)); }
+__device__ __forceinline__ int64_t key_l(uint64_t k) { return (int64_t)((k >> DR_RBITS) & ((1ull << DR_LBITS) - 1)); }
 
-#define DR_BIG 256  // cells with more arrivals are sorted by a whole workgroup
+#define DR_SMALL 32   // cells up to this many arrivals: one thread sorts
+#define DR_MED 1024   // up to this many: one wavefront; above: a 1024-thread workgroup
 
 __global__ __launch_bounds__(DR_NT) void k_drift_scatter(Geo g, const int64_t* __restrict__ cstart, int64_t n3c,
                                                          const uint16_t* __restrict__ disp, const int64_t* __restrict__ cs_new,
@@ -133,51 +163,131 @@ __global__ __launch_bounds__(DR_NT) void k_drift_scatter(Geo g, const int64_t* _
     cs[i] = cstart[r < n3c ? r : n3c];
   }
   __syncthreads();
-  for (int64_t p = cs[0] + threadIdx.x; p < cs[DR_CH]; p += DR_NT) {
-    int lo = 0, hi = DR_CH;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (cs[mid] <= p) lo = mid; else hi = mid;
+  const int lane = threadIdx.x & 63;
+  for (int64_t pb = cs[0]; pb < cs[DR_CH]; pb += DR_NT) {
+    const int64_t p = pb + threadIdx.x;
+    int64_t rd = -1 - lane;
+    uint64_t key = 0;
+    if (p < cs[DR_CH]) {
+      int lo = 0, hi = DR_CH;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (cs[mid] <= p) lo = mid; else hi = mid;
+      }
+      const int64_t rs = r0 + lo;
+      int32_t c[3];
+      cell_of_rank(g, rs, c);
+      const uint32_t dc = disp[p];
+      const int d[3] = {(int)(dc & 15) - 8, (int)((dc >> 4) & 15) - 8, (int)((dc >> 8) & 15) - 8};
+      const int cz = wrapc(c[2] + d[2], g.nc);
+      rd = rank_of(g, wrapc(c[0] + d[0], g.nc), wrapc(c[1] + d[1], g.nc), cz);
+      const int sz = cz % g.nt + 1 - d[2];  // source plane in the destination tile's frame
+      const int ilayer = (sz - 1 + g.ncb) % nlayer;
+      key = arrival_key(ilayer, d[0], d[1], d[2], p - cs[lo], rs);
     }
-    int32_t c[3];
-    cell_of_rank(g, r0 + lo, c);
-    const uint32_t dc = disp[p];
-    const int d[3] = {(int)(dc & 15) - 8, (int)((dc >> 4) & 15) - 8, (int)((dc >> 8) & 15) - 8};
-    const int cz = wrapc(c[2] + d[2], g.nc);
-    const int64_t rd = rank_of(g, wrapc(c[0] + d[0], g.nc), wrapc(c[1] + d[1], g.nc), cz);
-    const int sz = cz % g.nt + 1 - d[2];  // source plane in the destination tile's frame
-    const int ilayer = (sz - 1 + g.ncb) % nlayer;
-    const int32_t slot = atomicSub(&cnt[rd], 1) - 1;
-    keys[cs_new[rd] + slot] = arrival_key(ilayer, d[0], d[1], d[2], p);
-    if (slot == DR_BIG) big[atomicAdd(nbig, 1u)] = (int32_t)rd;
+    const WaveRun w = wave_run(rd);
+    int64_t base = 0;
+    if (rd >= 0 && w.head == lane) {
+      const int32_t old = atomicSub(&cnt[rd], w.len);
+      base = cs_new[rd] + old - w.len;
+      // the run holding slot DR_MED lists a crowded cell once (rare)
+      if (old > DR_MED && old - w.len <= DR_MED) big[atomicAdd(nbig, 1u)] = (int32_t)rd;
+    }
+    base = __shfl(base, w.head);
+    if (rd >= 0) keys[base + (lane - w.head)] = key;
   }
 }
 
-// ---------------------------------------------------------------- big cells
-// One workgroup per listed cell (grid-stride over the list).  Segments up to
-// DR_SORT_LDS keys are bitonic-sorted in LDS; longer ones are sorted in
-// DR_SORT_LDS chunks and then merged pairwise between `keys` and `tmp`
-// (each element's output position is its index in its own run plus its rank
-// in the partner run, so a pass is one parallel step).
-#define DR_SORT_NT 1024
-#define DR_SORT_LDS 8192
-__device__ void block_bitonic(uint64_t* s, int n, int npow2) {
-  for (int i = n + threadIdx.x; i < npow2; i += DR_SORT_NT) s[i] = ~0ull;
-  __syncthreads();
+// ---------------------------------------------------------------- sorting
+// Cells up to DR_SMALL arrivals: one thread each, insertion sort in LDS (the
+// small cells of a workgroup's DR_NT cells are packed into LDS, at most
+// DR_NT*DR_SMALL keys).  Up to DR_MED: the workgroup's wavefronts take them
+// in turn, bitonic in LDS.  Larger (listed by the scatter): one 1024-thread
+// workgroup per cell; segments up to DR_SORT_LDS keys are bitonic-sorted in
+// LDS, longer ones sorted in DR_SORT_LDS chunks and then merged pairwise
+// between `keys` and `tmp` (an element's output position is its index in its
+// own run plus its rank in the partner run).
+__device__ __forceinline__ void wave_sync() {
+  // a wavefront's LDS operations execute in order; this keeps the compiler
+  // from moving them across the point
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// compare-exchange network over npow2 keys by NT cooperating lanes; sync()
+// separates the stages
+template <int NT, typename Sync>
+__device__ __forceinline__ void bitonic(uint64_t* s, int n, int npow2, int tid, Sync sync) {
+  for (int i = n + tid; i < npow2; i += NT) s[i] = ~0ull;
+  sync();
   for (int k = 2; k <= npow2; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < npow2; i += DR_SORT_NT) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint64_t a = s[i], b = s[l];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) { s[i] = b; s[l] = a; }
-        }
+      for (int t = tid; t < (npow2 >> 1); t += NT) {
+        const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1)), l = i + j;
+        const uint64_t a = s[i], b = s[l];
+        const bool up = (i & k) == 0;
+        if ((a > b) == up) { s[i] = b; s[l] = a; }
       }
-      __syncthreads();
+      sync();
     }
 }
 
+#define DR_MED_LDS 1024
+__global__ __launch_bounds__(DR_NT) void k_drift_sort_cells(const int64_t* __restrict__ cs_new, int64_t n3c, uint64_t* keys) {
+  __shared__ uint64_t sk[DR_NT * DR_SMALL];  // small cells; then DR_NT/64 wavefront areas of DR_MED_LDS
+  __shared__ int32_t soff[DR_NT / 64];
+  __shared__ int32_t med[DR_NT];
+  __shared__ int32_t nmed;
+  const int64_t r = (int64_t)blockIdx.x * DR_NT + threadIdx.x;
+  int64_t base = 0;
+  int n = 0;
+  if (r < n3c) {
+    base = cs_new[r];
+    n = (int)(cs_new[r + 1] - base);
+  }
+  if (threadIdx.x == 0) nmed = 0;
+  const int m = n <= DR_SMALL ? n : 0;  // keys this thread stages
+  int incl = m;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o);
+    if ((threadIdx.x & 63) >= o) incl += t;
+  }
+  if ((threadIdx.x & 63) == 63) soff[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  if (n > DR_SMALL && n <= DR_MED) med[atomicAdd(&nmed, 1)] = threadIdx.x;
+  int wbase = 0;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) wbase += soff[w];
+  if (m > 1) {
+    uint64_t* a = sk + wbase + incl - m;
+    for (int i = 0; i < m; i++) a[i] = keys[base + i];
+    for (int i = 1; i < m; i++) {
+      const uint64_t v = a[i];
+      int j = i - 1;
+      while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+      a[j + 1] = v;
+    }
+    for (int i = 0; i < m; i++) keys[base + i] = a[i];
+  }
+  __syncthreads();
+  // medium cells: wavefront w sorts list entries w, w + DR_NT/64, ...
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t* ws = sk + wv * DR_MED_LDS;
+  const int nm = nmed;
+  for (int q = wv; q < nm; q += DR_NT / 64) {
+    const int64_t rr = (int64_t)blockIdx.x * DR_NT + med[q];
+    const int64_t b0 = cs_new[rr];
+    const int nn = (int)(cs_new[rr + 1] - b0);
+    int np2 = 64;
+    while (np2 < nn) np2 <<= 1;
+    for (int i = lane; i < nn; i += 64) ws[i] = keys[b0 + i];
+    bitonic<64>(ws, nn, np2, lane, [] __device__() { wave_sync(); });
+    for (int i = lane; i < nn; i += 64) keys[b0 + i] = ws[i];
+    wave_sync();
+  }
+}
+
+#define DR_SORT_NT 1024
+#define DR_SORT_LDS 8192
 __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __restrict__ cs_new, const int32_t* __restrict__ big,
                                                                const uint32_t* __restrict__ nbig, uint64_t* keys, uint64_t* tmp) {
   __shared__ uint64_t s[DR_SORT_LDS];
@@ -186,18 +296,17 @@ __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __
     const int64_t r = big[bi];
     const int64_t base = cs_new[r];
     const int64_t n = cs_new[r + 1] - base;
-    // sort runs of DR_SORT_LDS in LDS
     for (int64_t c0 = 0; c0 < n; c0 += DR_SORT_LDS) {
       const int m = (int)min((int64_t)DR_SORT_LDS, n - c0);
       int np2 = 1;
       while (np2 < m) np2 <<= 1;
       for (int i = threadIdx.x; i < m; i += DR_SORT_NT) s[i] = keys[base + c0 + i];
       __syncthreads();
-      block_bitonic(s, m, np2);
+      bitonic<DR_SORT_NT>(s, m, np2, (int)threadIdx.x, [] __device__() { __syncthreads(); });
       for (int i = threadIdx.x; i < m; i += DR_SORT_NT) keys[base + c0 + i] = s[i];
       __syncthreads();
     }
-    // merge passes (keys are unique: the low word is the particle index)
+    // merge passes (keys are unique: the low bits hold the source cell and l)
     uint64_t* src = keys + base;
     uint64_t* dst = tmp + base;
     for (int64_t run = DR_SORT_LDS; run < n; run <<= 1) {
@@ -228,86 +337,86 @@ __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __
   }
 }
 
-// ---------------------------------------------------------------- write
-// One thread per destination cell; a workgroup's DR_NT consecutive cells own
-// one contiguous key range, staged in LDS when it fits.  Each thread sorts
-// its own segment (cells above DR_BIG arrive sorted), sums vfield_new in
-// arrival order, then writes xp/vp/pid at their slots and the sigma_v sums.
-#define DR_WCAP 6144
-__device__ __forceinline__ void seg_sort(uint64_t* a, int n) {
-  for (int i = 1; i < n; i++) {
-    const uint64_t v = a[i];
-    int j = i - 1;
-    while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
-    a[j + 1] = v;
+// ---------------------------------------------------------------- gather
+// One thread per output slot q (checkpoint order after the drift): the
+// arrival's full velocity vreal (update_particle.f90:69), its new position
+// code (:117) and PID; vreal is kept for the per-cell pass.
+__global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* __restrict__ keys, int64_t np,
+                                                        const int64_t* __restrict__ cstart, const void* __restrict__ xp,
+                                                        const void* __restrict__ vp, const int32_t* __restrict__ pid,
+                                                        const float* __restrict__ vfield, float dt_mid, double c_old,
+                                                        const float* __restrict__ vdec, void* xp_o, int32_t* pid_o,
+                                                        double* vr) {
+  const double xscale = (g.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0) * (double)CUBE_NCELL;
+  for (int64_t q = (int64_t)blockIdx.x * DR_NT + threadIdx.x; q < np; q += (int64_t)gridDim.x * DR_NT) {
+    const uint64_t key = keys[q];
+    const int64_t rs = key_rank(key);
+    const int64_t p = cstart[rs] + key_l(key);
+    for (int e = 0; e < 3; e++) {
+      const double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[rs * 3 + e];
+      int64_t xn = (int64_t)load_xu(xp, 3 * p + e, g.izipx);
+      xn = g.izipx == 1 ? (int64_t)(int8_t)xn : (int64_t)(int16_t)xn;
+      xn += (int64_t)nint_d((double)dt_mid * vreal / xscale);
+      store_x(xp_o, 3 * q + e, g.izipx, xn);
+      vr[3 * q + e] = vreal;
+    }
+    pid_o[q] = pid[p];
   }
 }
 
-__global__ __launch_bounds__(DR_NT) void k_drift_write(Geo g, const void* __restrict__ xp, const void* __restrict__ vp,
-                                                       const int32_t* __restrict__ pid, const float* __restrict__ vfield,
-                                                       const int64_t* __restrict__ cs_new, uint64_t* keys, int64_t n3c,
-                                                       float dt_mid, double c_old, const float* __restrict__ vdec,
-                                                       const double* __restrict__ venc, int32_t kmax, void* xp_o, void* vp_o,
-                                                       int32_t* pid_o, float* vfield_o, int32_t* rhoc_o, double* part) {
-  __shared__ uint64_t sk[DR_WCAP];
+// ---------------------------------------------------------------- cells
+// One workgroup per DR_NT destination cells.  Phase 1, one thread per cell:
+// vfield_new summed in arrival order (update_particle.f90:70-83).  Phase 2,
+// one thread per slot: the velocity residual re-quantised with the old sigma
+// (:119) and the sigma_v sums (:174-200).  part: 3 doubles per workgroup.
+__global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __restrict__ cs_new, int64_t n3c,
+                                                       const float* __restrict__ vfield, const double* __restrict__ vr,
+                                                       double c_old, const float* __restrict__ vdec,
+                                                       const double* __restrict__ venc, int32_t kmax, void* vp_o,
+                                                       float* vfield_o, int32_t* rhoc_o, double* part) {
+  __shared__ int64_t scs[DR_NT + 1];
+  __shared__ float svf[DR_NT * 3];
   __shared__ double red[3][DR_NT / 64];
   const int64_t r0 = (int64_t)blockIdx.x * DR_NT;
-  const int64_t rdst = r0 + threadIdx.x;
-  const int64_t kb0 = cs_new[r0], kb1 = cs_new[min(r0 + DR_NT, n3c)];
-  const bool staged = kb1 - kb0 <= DR_WCAP;
-  if (staged)
-    for (int64_t i = kb0 + threadIdx.x; i < kb1; i += DR_NT) sk[i - kb0] = keys[i];
+  const int ncell = (int)min((int64_t)DR_NT, n3c - r0);
+  for (int i = threadIdx.x; i <= DR_NT; i += DR_NT) scs[i] = cs_new[r0 + min(i, ncell)];
   __syncthreads();
-  double sum_c = 0, sum_r = 0, sum_s = 0;
   const double wv = (double)0.1f;  // weight_v=0.1 is a default-real literal
-  const double xscale = (g.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0) * (double)CUBE_NCELL;
-  if (rdst < n3c) {
-    const int64_t base = cs_new[rdst];
-    const int32_t nnew = (int32_t)(cs_new[rdst + 1] - base);
-    uint64_t* kk = staged ? sk + (base - kb0) : keys + base;
-    if (nnew <= DR_BIG) seg_sort(kk, nnew);
-    int32_t c[3];
-    cell_of_rank(g, rdst, c);
+  double sum_c = 0, sum_r = 0, sum_s = 0;
+  if (threadIdx.x < ncell) {
+    const int64_t r = r0 + threadIdx.x;
+    const int64_t b = scs[threadIdx.x], n = scs[threadIdx.x + 1] - b;
     float vf[3];
-    for (int e = 0; e < 3; e++) vf[e] = (float)((double)vfield[rdst * 3 + e] * wv);
-    for (int32_t k = 0; k < nnew; k++) {
-      const uint64_t key = kk[k];
-      const int64_t p = (int64_t)(uint32_t)key;
-      int d[3];
-      key_disp(key, d);
-      const int64_t rs = rank_of(g, wrapc(c[0] - d[0], g.nc), wrapc(c[1] - d[1], g.nc), wrapc(c[2] - d[2], g.nc));
-      for (int e = 0; e < 3; e++) {
-        double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[rs * 3 + e];
-        vf[e] = (float)((double)vf[e] + vreal);
-      }
-    }
+    for (int e = 0; e < 3; e++) vf[e] = (float)((double)vfield[r * 3 + e] * wv);
+    for (int64_t k = 0; k < n; k++)
+      for (int e = 0; e < 3; e++) vf[e] = (float)((double)vf[e] + vr[3 * (b + k) + e]);
     float vfn[3];
-    for (int e = 0; e < 3; e++) vfn[e] = (float)((double)vf[e] / ((double)nnew + wv));
-    for (int32_t k = 0; k < nnew; k++) {
-      const uint64_t key = kk[k];
-      const int64_t p = (int64_t)(uint32_t)key, q = base + k;
-      int d[3];
-      key_disp(key, d);
-      const int64_t rs = rank_of(g, wrapc(c[0] - d[0], g.nc), wrapc(c[1] - d[1], g.nc), wrapc(c[2] - d[2], g.nc));
-      double vr[3];
-      for (int e = 0; e < 3; e++) {
-        double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[rs * 3 + e];
-        int64_t xn = (int64_t)load_xu(xp, 3 * p + e, g.izipx);
-        xn = g.izipx == 1 ? (int64_t)(int8_t)xn : (int64_t)(int16_t)xn;
-        xn += (int64_t)nint_d((double)dt_mid * vreal / xscale);
-        store_x(xp_o, 3 * q + e, g.izipx, xn);
-        int32_t code_v = vencode(venc, kmax, g.izipv, c_old, vreal - (double)vfn[e]);
-        store_v(vp_o, 3 * q + e, g.izipv, code_v);
-        vr[e] = vdecode(vdec, code_v, g.izipv, c_old);
-      }
-      pid_o[q] = pid[p];
-      sum_r += vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
-      for (int e = 0; e < 3; e++) vr[e] += (double)vfn[e];
-      sum_s += vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2];
+    for (int e = 0; e < 3; e++) {
+      vfn[e] = (float)((double)vf[e] / ((double)n + wv));
+      svf[threadIdx.x * 3 + e] = vfn[e];
+      vfield_o[r * 3 + e] = vfn[e];
     }
-    for (int e = 0; e < 3; e++) vfield_o[rdst * 3 + e] = vfn[e];
-    rhoc_o[rdst] = nnew;
+    rhoc_o[r] = (int32_t)n;
     sum_c = (double)(vfn[0] * vfn[0] + vfn[1] * vfn[1] + vfn[2] * vfn[2]);
+  }
+  __syncthreads();
+  const int64_t q1 = scs[ncell];
+  for (int64_t q = scs[0] + threadIdx.x; q < q1; q += DR_NT) {
+    int lo = 0, hi = ncell;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (scs[mid] <= q) lo = mid; else hi = mid;
+    }
+    double v[3];
+    for (int e = 0; e < 3; e++) {
+      const float vfn = svf[lo * 3 + e];
+      const int32_t code = vencode(venc, kmax, g.izipv, c_old, vr[3 * q + e] - (double)vfn);
+      store_v(vp_o, 3 * q + e, g.izipv, code);
+      v[e] = vdecode(vdec, code, g.izipv, c_old);
+    }
+    sum_r += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    for (int e = 0; e < 3; e++) v[e] += (double)svf[lo * 3 + e];
+    sum_s += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   }
   double v3[3] = {sum_c, sum_r, sum_s};
   for (int f = 0; f < 3; f++) {
@@ -329,7 +438,7 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   const int nlayer = 2 * (int)ceilf(dt_mid * h->vz_max / (float)CUBE_NCELL) + 1;
   const double c_old = vcoef(h->sigma_vi);
   const int dmax = (int)(h->ncb < DMAX ? h->ncb : DMAX);
-  uint32_t* flags = h->uflags;  // [0..2] max disp, [3] too far, [4] frame conflicts, [6] cells above DR_BIG
+  uint32_t* flags = h->uflags;  // [0..2] max disp, [3] too far / crowded, [4] frame conflicts, [6] crowded cells
   HIP_TRY(h, hipMemsetAsync(flags, 0, sizeof(uint32_t) * 8, h->stream));
   HIP_TRY(h, hipMemsetAsync(h->rhoc2, 0, sizeof(int32_t) * h->n3c, h->stream));
   int64_t nb1 = (h->n3c + DR_CH - 1) / DR_CH;
@@ -341,9 +450,11 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   uint32_t hf[8];
   HIP_TRY(h, hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  if (hf[3])
+  if (hf[3] & 1)
     return cube_fail(h, CUBE_ERR_CAPACITY, "a particle moved more than %d coarse cells in one drift (buffer ncb=%lld)", dmax,
                      (long long)h->ncb);
+  if (hf[3] & 2) return cube_fail(h, CUBE_ERR_CAPACITY, "a coarse cell holds more than 2^%d particles", DR_LBITS);
+  if (nlayer > 31) return cube_fail(h, CUBE_ERR_CAPACITY, "nlayer=%d above 31 (dt*vz_max too large)", nlayer);
   int pid2 = prof_begin(h, PS_DRIFT_SCAN);
   int r = scan_counts(h, h->rhoc2, h->n3c, h->cstart2);
   prof_end(h, pid2);
@@ -358,13 +469,17 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   prof_end(h, pid3);
   HIP_TRY(h, hipGetLastError());
   int pid4 = prof_begin(h, PS_DRIFT_SORT);
-  k_drift_sort_big<<<1024, DR_SORT_NT, 0, h->stream>>>(h->cstart2, h->dbig, nbig, keys, (uint64_t*)h->dsort_tmp);
+  k_drift_sort_cells<<<(unsigned)nb3, DR_NT, 0, h->stream>>>(h->cstart2, h->n3c, keys);
+  k_drift_sort_big<<<512, DR_SORT_NT, 0, h->stream>>>(h->cstart2, h->dbig, nbig, keys, (uint64_t*)h->dscratch);
   prof_end(h, pid4);
   HIP_TRY(h, hipGetLastError());
   int pid5 = prof_begin(h, PS_DRIFT_WRITE);
-  k_drift_write<<<(unsigned)nb3, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->pid, h->vfield, h->cstart2, keys, h->n3c, dt_mid, c_old,
-                                                         h->vdec, h->venc, h->vkmax, h->xp2, h->vp2, h->pid2, h->vfield2,
-                                                         h->rhoc2, h->dred);
+  const int64_t ngather = std::min<int64_t>((h->np + DR_NT - 1) / DR_NT, 1 << 20);
+  k_drift_gather<<<(unsigned)std::max<int64_t>(ngather, 1), DR_NT, 0, h->stream>>>(
+      g, keys, h->np, h->cstart, h->xp, h->vp, h->pid, h->vfield, dt_mid, c_old, h->vdec, h->xp2, h->pid2,
+      (double*)h->dscratch);
+  k_drift_cells<<<(unsigned)nb3, DR_NT, 0, h->stream>>>(g, h->cstart2, h->n3c, h->vfield, (const double*)h->dscratch, c_old,
+                                                         h->vdec, h->venc, h->vkmax, h->vp2, h->vfield2, h->rhoc2, h->dred);
   prof_end(h, pid5);
   HIP_TRY(h, hipGetLastError());
   double sums[3];

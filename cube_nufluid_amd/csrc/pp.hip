@@ -16,6 +16,7 @@
 #include "cube_internal.hpp"
 #include <string.h>
 #include <algorithm>
+#include <vector>
 
 #define PP_NT 64
 #define PP_BP 2
@@ -168,231 +169,530 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
     npl += __shfl_xor(npl, o);
   }
   if (threadIdx.x == 0) {
-    atomicMax(f2max_bits, __float_as_uint(f2loc));
-    atomicAdd(npairs_out, npl);
-    atomicAdd(nupd_out, (unsigned long long)ninner);
+    const int sl = blockIdx.x & 1023;
+    atomicMax(f2max_bits + sl, __float_as_uint(f2loc));
+    atomicAdd(npairs_out + sl, npl);
+    atomicAdd(nupd_out + sl, (unsigned long long)ninner);
   }
 }
 
-// ------------------------------------------------------------ binned PP
-// One workgroup per PB^3 coarse cells.  Candidates (particles of the block
-// and its neighbour cells whose fine cell is within pp_range of the block)
-// are counting-sorted by fine cell in LDS; each block particle then visits
-// only the (2 pp_range+1)^3 neighbouring fine-cell runs, which is exactly the
-// linked-list neighbourhood of ext_pp_force.f90:92-110.  Blocks whose
-// candidates exceed PB_CAP are appended to an overflow list and handled by
-// k_pp above.
-#define PB 4
-#define PB_NT 256
-#define PB_CAP 2048
-#define PB_WMAX 20  // window width 4*PB + 2*pp_range for pp_range <= 2
+// ------------------------------------------------------------ fine-cell PP
+// One workgroup per PB^3 coarse cells (a 4PB-wide fine-cell block).
+// Candidate partners -- particles of the block and of its neighbour coarse
+// cells whose fine cell lies within pp_range of the block, a W^3 window with
+// W = 4PB + 2 pp_range -- are counting-sorted by window fine cell, so each
+// block particle visits exactly the (2 pp_range+1)^3 neighbouring fine cells
+// of ext_pp_force.f90:92-110 as (2 pp_range+1)^2 contiguous x-runs.
+//   fast path : the sorted candidates sit in LDS (up to the cap);
+//   dense path: blocks with more candidates are listed; a prep kernel sorts
+//               their candidates into a global pool (the fine work mesh, idle
+//               between drift and PM) and cuts their targets into work items
+//               of PD_NT, which a pairs kernel walks grid-stride.  A block the
+//               pool cannot hold goes to the chunked kernel above.
+// Pair sums are fixed point, so the result does not depend on the order of
+// candidates inside a fine cell or on the path that handled the block.
+// PB = 4 serves pp_range <= 2, PB = 2 pp_range 3..4 (window <= 20 cells).
+#define PD_NT 256
+#define PP_SLOTS 1024
 struct PPent { float x, y, z; int32_t p; };
-__global__ __launch_bounds__(PB_NT) void k_pp_binned(Geo g, const void* __restrict__ xp, void* vp,
-                                                     const int64_t* __restrict__ cstart, float mass, float rsoft, int pr,
-                                                     float a_mid, float dt, double c_old, const float* __restrict__ vdec,
-                                                     const double* __restrict__ venc, int32_t kmax, uint32_t* f2max_bits,
-                                                     unsigned long long* npairs_out, unsigned long long* nupd_out,
-                                                     double* probe, int32_t* ovf_list, uint32_t* ovf_n) {
-  constexpr int SPAN = PB + 2, NCELL = SPAN * SPAN * SPAN, FB = 4 * PB;
-  __shared__ int64_t beg[NCELL];
-  __shared__ int32_t off[NCELL + 1];
-  __shared__ int32_t wcnt[PB_WMAX * PB_WMAX * PB_WMAX + 1];
-  __shared__ PPent ent[PB_CAP];
-  __shared__ int32_t ncand, ninner_s;
-  __shared__ float redf[PB_NT / 64];
-  __shared__ unsigned long long redn[PB_NT / 64];
+
+struct PPctx {  // per-call constants
+  float mass, rsoft, a_mid, dt;
+  double c_old;
+  const float* vdec;
+  const double* venc;
+  int32_t kmax, pr;
+};
+
+template <int PB>
+struct PBgeo {
+  static constexpr int SPAN = PB + 2, NCELL = SPAN * SPAN * SPAN, FB = 4 * PB;
+};
+
+template <int PB>
+__device__ __forceinline__ void pf_origin(const Geo& g, int64_t b, int& x, int& y, int& z) {
   const int nbx = g.nc / PB;
-  const int c0x = (blockIdx.x % nbx) * PB, c0y = ((blockIdx.x / nbx) % nbx) * PB, c0z = (blockIdx.x / (nbx * nbx)) * PB;
-  const int W = FB + 2 * pr, W3 = W * W * W, lo = -pr;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < NCELL; i += PB_NT) {
-    int rx = i % SPAN - 1, ry = (i / SPAN) % SPAN - 1, rz = i / (SPAN * SPAN) - 1;
-    int64_t r = rank_of(g, wrapc(c0x + rx, g.nc), wrapc(c0y + ry, g.nc), wrapc(c0z + rz, g.nc));
+  x = (int)(b % nbx) * PB;
+  y = (int)((b / nbx) % nbx) * PB;
+  z = (int)(b / ((int64_t)nbx * nbx)) * PB;
+}
+
+// inclusive block scan of one int per thread (NT threads); wsum has NT/64 slots
+template <int NT>
+__device__ __forceinline__ int32_t block_incl_scan(int32_t v, int32_t* wsum) {
+  const int lane = threadIdx.x & 63;
+  int32_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int32_t wb = 0;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) wb += wsum[w];
+  __syncthreads();
+  return wb + incl;
+}
+
+// region = SPAN^3 coarse cells [c0-1, c0+PB]; beg/off staged for the workgroup
+template <int PB, int NT>
+__device__ __forceinline__ void pf_region(const Geo& g, const int64_t* cstart, int c0x, int c0y, int c0z, int64_t* beg,
+                                          int32_t* off, int32_t* wsum) {
+  using G = PBgeo<PB>;
+  static_assert(G::NCELL <= NT, "one thread per region cell");
+  const int i = threadIdx.x;
+  int32_t c = 0;
+  if (i < G::NCELL) {
+    const int rx = i % G::SPAN - 1, ry = (i / G::SPAN) % G::SPAN - 1, rz = i / (G::SPAN * G::SPAN) - 1;
+    const int64_t r = rank_of(g, wrapc(c0x + rx, g.nc), wrapc(c0y + ry, g.nc), wrapc(c0z + rz, g.nc));
     beg[i] = cstart[r];
-    off[i + 1] = (int32_t)(cstart[r + 1] - cstart[r]);
+    c = (int32_t)(cstart[r + 1] - cstart[r]);
   }
-  for (int i = tid; i <= W3; i += PB_NT) wcnt[i] = 0;
-  if (tid == 0) { ncand = 0; ninner_s = 0; }
+  const int32_t incl = block_incl_scan<NT>(c, wsum);
+  if (i < G::NCELL) off[i + 1] = incl;
+  if (i == 0) off[0] = 0;
+}
+
+// window cell of region particle t (false if outside the window) and its
+// block-local fine coordinates (exact in FP32: at most 20 significant bits)
+template <int PB>
+__device__ __forceinline__ bool pf_cand(const Geo& g, const void* xp, const int64_t* beg, const int32_t* off, int32_t t, int pr,
+                                        int W, int& wi, PPent& e) {
+  using G = PBgeo<PB>;
+  int a = 0, b = G::NCELL;
+  while (b - a > 1) {
+    const int m = (a + b) >> 1;
+    if (off[m] <= t) a = m; else b = m;
+  }
+  const int64_t p = beg[a] + (t - off[a]);
+  const int rel[3] = {a % G::SPAN - 1, (a / G::SPAN) % G::SPAN - 1, a / (G::SPAN * G::SPAN) - 1};
+  int w[3];
+  float x[3];
+  bool keep = true;
+  for (int d = 0; d < 3; d++) {
+    const uint32_t u = load_xu(xp, 3 * p + d, g.izipx);
+    w[d] = CUBE_NCELL * rel[d] + xsub(u, g.izipx) + pr;
+    x[d] = (float)((double)(CUBE_NCELL * rel[d]) + (double)CUBE_NCELL * xoff(u, g.izipx));
+    keep = keep && w[d] >= 0 && w[d] < W;
+  }
+  wi = (w[2] * W + w[1]) * W + w[0];
+  e = PPent{x[0], x[1], x[2], (int32_t)p};
+  return keep;
+}
+
+// exclusive scan of cnt[0..n) in place, cnt[n] = total
+template <int NT>
+__device__ void pf_scan(int32_t* cnt, int n, int32_t* wsum) {
+  const int per = (n + NT - 1) / NT;
+  const int i0 = min(n, (int)threadIdx.x * per), i1 = min(n, i0 + per);
+  int32_t s = 0;
+  for (int i = i0; i < i1; i++) s += cnt[i];
+  int32_t run = block_incl_scan<NT>(s, wsum) - s;
+  for (int i = i0; i < i1; i++) {
+    const int32_t c = cnt[i];
+    cnt[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == NT - 1) cnt[n] = run;
   __syncthreads();
-  if (tid == 0) {
-    off[0] = 0;
-    for (int i = 0; i < NCELL; i++) off[i + 1] += off[i];
-  }
-  __syncthreads();
-  const int32_t ntot = off[NCELL];
-  // pass 1: count candidates per window fine cell
-  for (int32_t t = tid; t < ntot; t += PB_NT) {
-    int a = 0, b = NCELL;
-    while (b - a > 1) { int m = (a + b) >> 1; if (off[m] <= t) a = m; else b = m; }
-    const int64_t p = beg[a] + (t - off[a]);
-    const int rel[3] = {a % SPAN - 1, (a / SPAN) % SPAN - 1, a / (SPAN * SPAN) - 1};
-    int w[3];
-    bool keep = true;
-    for (int e = 0; e < 3; e++) {
-      w[e] = CUBE_NCELL * rel[e] + xsub(load_xu(xp, 3 * p + e, g.izipx), g.izipx) - lo;
-      keep = keep && w[e] >= 0 && w[e] < W;
-    }
-    if (keep) {
-      atomicAdd(&wcnt[(w[2] * W + w[1]) * W + w[0]], 1);
-      atomicAdd(&ncand, 1);
-    }
-  }
-  __syncthreads();
-  const int32_t nc_ = ncand;
-  if (nc_ > PB_CAP) {  // too many candidates for LDS: hand the block to the chunked kernel
-    if (tid == 0) {
-      uint32_t slot = atomicAdd(ovf_n, 1u);
-      ovf_list[slot] = blockIdx.x;
-    }
-    return;
-  }
-  // exclusive scan of wcnt (W3 <= 13824) by one wavefront: serial chunks + shuffle scan
-  if (tid < 64) {
-    const int per = (W3 + 63) / 64;
-    const int i0 = min(W3, tid * per), i1 = min(W3, i0 + per);
-    int32_t s = 0;
-    for (int i = i0; i < i1; i++) s += wcnt[i];
-    int32_t incl = s;
-    for (int o = 1; o < 64; o <<= 1) {
-      int32_t v = __shfl_up(incl, o);
-      if (tid >= o) incl += v;
-    }
-    int32_t run = incl - s;
-    for (int i = i0; i < i1; i++) {
-      int32_t c = wcnt[i];
-      wcnt[i] = run;
-      run += c;
-    }
-    if (tid == 63) wcnt[W3] = incl;
-  }
-  __syncthreads();
-  // pass 2: scatter candidates into fine-cell order (wcnt[i] becomes the end of run i)
-  for (int32_t t = tid; t < ntot; t += PB_NT) {
-    int a = 0, b = NCELL;
-    while (b - a > 1) { int m = (a + b) >> 1; if (off[m] <= t) a = m; else b = m; }
-    const int64_t p = beg[a] + (t - off[a]);
-    const int rel[3] = {a % SPAN - 1, (a / SPAN) % SPAN - 1, a / (SPAN * SPAN) - 1};
-    int w[3];
-    float x[3];
-    bool keep = true;
-    for (int e = 0; e < 3; e++) {
-      uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
-      w[e] = CUBE_NCELL * rel[e] + xsub(u, g.izipx) - lo;
-      x[e] = (float)((double)(CUBE_NCELL * rel[e]) + (double)CUBE_NCELL * xoff(u, g.izipx));
-      keep = keep && w[e] >= 0 && w[e] < W;
-    }
-    if (keep) {
-      int slot = atomicAdd(&wcnt[(w[2] * W + w[1]) * W + w[0]], 1);
-      ent[slot] = PPent{x[0], x[1], x[2], (int32_t)p};
-    }
-  }
-  __syncthreads();
-  // after pass 2, wcnt[i] = end of run i = start of run i+1; start of run 0 is 0
-  float f2loc = 0.f;
-  unsigned long long npl = 0;
-  int32_t ninner = 0;
-  for (int32_t t = tid; t < nc_; t += PB_NT) {
-    const PPent pi = ent[t];
-    const int fi[3] = {(int)floorf(pi.x) - lo, (int)floorf(pi.y) - lo, (int)floorf(pi.z) - lo};
-    if (fi[0] < pr || fi[0] >= pr + FB || fi[1] < pr || fi[1] >= pr + FB || fi[2] < pr || fi[2] >= pr + FB) continue;
-    ninner++;
-    long long fa[3] = {0, 0, 0};
-    for (int kz = fi[2] - pr; kz <= fi[2] + pr; kz++)
-      for (int ky = fi[1] - pr; ky <= fi[1] + pr; ky++) {
-        const int rowc = (kz * W + ky) * W;
-        const int j0 = (rowc + fi[0] - pr) == 0 ? 0 : wcnt[rowc + fi[0] - pr - 1];
-        const int j1 = wcnt[rowc + fi[0] + pr];  // runs of consecutive x cells are contiguous
-        for (int j = j0; j < j1; j++) {
-          const PPent pj = ent[j];
-          npl++;
-          float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-          float r2 = dx * dx + dy * dy + dz * dz;
-          float r = sqrtf(r2);
-          if (!(r > rsoft)) continue;
-          float rc = r * (1.0f / 16.0f);
-          float rc3 = rc * rc * rc;
-          float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
-          float sc = mass * pc / (r2 * r);
-          fa[0] += llrint((double)(sc * dx) * PP_FIX);
-          fa[1] += llrint((double)(sc * dy) * PP_FIX);
-          fa[2] += llrint((double)(sc * dz) * PP_FIX);
-        }
-      }
-    const double f[3] = {(double)fa[0] * (1.0 / PP_FIX), (double)fa[1] * (1.0 / PP_FIX), (double)fa[2] * (1.0 / PP_FIX)};
-    const int64_t p = pi.p;
-    if (probe) {
-      for (int e = 0; e < 3; e++) probe[3 * p + e] = f[e];
-    } else {
-      for (int e = 0; e < 3; e++) {
-        double v = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old);
-        v = v + f[e] * (double)a_mid * (double)dt / 6.0 / (double)PI_F;
-        store_v(vp, 3 * p + e, g.izipv, vencode(venc, kmax, g.izipv, c_old, v));
+}
+
+// force on one target from the candidate runs (start[] exclusive offsets)
+template <typename EntT>
+__device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, int W, int pr, const PPent& ti, float mass,
+                                         float rsoft, long long fa[3], unsigned long long& npl) {
+  const int wx = (int)floorf(ti.x) + pr, wy = (int)floorf(ti.y) + pr, wz = (int)floorf(ti.z) + pr;
+  for (int kz = wz - pr; kz <= wz + pr; kz++)
+    for (int ky = wy - pr; ky <= wy + pr; ky++) {
+      const int row = (kz * W + ky) * W;
+      const int j0 = start[row + wx - pr], j1 = start[row + wx + pr + 1];
+      npl += (unsigned long long)(j1 - j0);
+      for (int j = j0; j < j1; j++) {
+        const PPent pj = ent[j];
+        const float dx = pj.x - ti.x, dy = pj.y - ti.y, dz = pj.z - ti.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r = sqrtf(r2);
+        if (!(r > rsoft)) continue;
+        const float rc = r * (1.0f / 16.0f);
+        const float rc3 = rc * rc * rc;
+        const float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
+        const float sc = mass * pc / (r2 * r);
+        fa[0] += llrint((double)(sc * dx) * PP_FIX);
+        fa[1] += llrint((double)(sc * dy) * PP_FIX);
+        fa[2] += llrint((double)(sc * dz) * PP_FIX);
       }
     }
-    f2loc = fmaxf(f2loc, (float)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
+}
+
+// kick (ext_pp_force.f90:114-119) from prefetched codes, or probe write;
+// returns |F|^2 in float
+__device__ __forceinline__ float pf_apply(const Geo& g, const PPctx& c, void* vp, double* probe, int64_t p, const int32_t vc[3],
+                                          const long long fa[3]) {
+  const double f[3] = {(double)fa[0] * (1.0 / PP_FIX), (double)fa[1] * (1.0 / PP_FIX), (double)fa[2] * (1.0 / PP_FIX)};
+  if (probe) {
+    for (int e = 0; e < 3; e++) probe[3 * p + e] = f[e];
+  } else {
+    for (int e = 0; e < 3; e++) {
+      double v = vdecode(c.vdec, vc[e], g.izipv, c.c_old);
+      v = v + f[e] * (double)c.a_mid * (double)c.dt / 6.0 / (double)PI_F;
+      store_v(vp, 3 * p + e, g.izipv, vencode(c.venc, c.kmax, g.izipv, c.c_old, v));
+    }
   }
+  return (float)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+}
+
+// per-workgroup totals: f2 max, pair count, targets
+template <int NT>
+__device__ void pf_reduce(float f2loc, unsigned long long npl, int32_t ninner, float* redf, unsigned long long* redn,
+                          int32_t* redi, uint32_t* f2max_bits, unsigned long long* npairs_out, unsigned long long* nupd_out) {
   for (int o = 32; o > 0; o >>= 1) {
     f2loc = fmaxf(f2loc, __shfl_xor(f2loc, o));
     npl += __shfl_xor(npl, o);
+    ninner += __shfl_xor(ninner, o);
   }
-  atomicAdd(&ninner_s, ninner);
-  if ((tid & 63) == 0) { redf[tid >> 6] = f2loc; redn[tid >> 6] = npl; }
+  if ((threadIdx.x & 63) == 0) {
+    redf[threadIdx.x >> 6] = f2loc;
+    redn[threadIdx.x >> 6] = npl;
+    redi[threadIdx.x >> 6] = ninner;
+  }
   __syncthreads();
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     float m = 0.f;
-    unsigned long long nn = 0;
-    for (int i = 0; i < PB_NT / 64; i++) { m = fmaxf(m, redf[i]); nn += redn[i]; }
-    atomicMax(f2max_bits, __float_as_uint(m));
-    atomicAdd(npairs_out, nn);
-    atomicAdd(nupd_out, (unsigned long long)ninner_s);
+    unsigned long long nn = 0, ni = 0;
+    for (int i = 0; i < NT / 64; i++) { m = fmaxf(m, redf[i]); nn += redn[i]; ni += (unsigned long long)redi[i]; }
+    // one of PP_SLOTS slots per workgroup: millions of workgroups must not
+    // meet on one address (max and integer sums are order-independent)
+    const int sl = blockIdx.x & (PP_SLOTS - 1);
+    if (m > 0.f) atomicMax(f2max_bits + sl, __float_as_uint(m));
+    if (nn) atomicAdd(npairs_out + sl, nn);
+    if (ni) atomicAdd(nupd_out + sl, ni);
   }
+  __syncthreads();
+}
+
+template <int PB>
+struct PFcfg;
+template <>
+struct PFcfg<4> { static constexpr int NT = 512, K = 4, CAP = 2560; };
+template <>
+struct PFcfg<2> { static constexpr int NT = 128, K = 4, CAP = 1024; };
+
+__host__ __device__ constexpr int pf_wc_bytes(int W3) { return ((W3 + 2) * 4 + 15) / 16 * 16; }
+
+// LDS (dynamic): wc[W^3+2] int32 (wc[1+i]: count, then start, then end of
+// run i; wc[0] = 0), then CAP entries
+template <int PB>
+__global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const void* __restrict__ xp, void* vp,
+                                                          const int64_t* __restrict__ cstart, uint32_t* f2max_bits,
+                                                          unsigned long long* npairs_out, unsigned long long* nupd_out,
+                                                          double* probe, int32_t* dense, uint32_t* ndense) {
+  using G = PBgeo<PB>;
+  constexpr int NT = PFcfg<PB>::NT, K = PFcfg<PB>::K, CAP = PFcfg<PB>::CAP;
+  extern __shared__ __align__(16) uint8_t pf_dyn[];
+  __shared__ int64_t beg[G::NCELL];
+  __shared__ int32_t off[G::NCELL + 1];
+  __shared__ int32_t ncand_s, wsum[NT / 64], redi[NT / 64];
+  __shared__ float redf[NT / 64];
+  __shared__ unsigned long long redn[NT / 64];
+  int c0x, c0y, c0z;
+  pf_origin<PB>(g, blockIdx.x, c0x, c0y, c0z);
+  const int pr = c.pr, W = G::FB + 2 * pr, W3 = W * W * W;
+  int32_t* wc = (int32_t*)pf_dyn;
+  PPent* ent = (PPent*)(pf_dyn + pf_wc_bytes(W3));
+  for (int i = threadIdx.x; i <= W3 + 1; i += NT) wc[i] = 0;
+  if (threadIdx.x == 0) ncand_s = 0;
+  pf_region<PB, NT>(g, cstart, c0x, c0y, c0z, beg, off, wsum);
+  __syncthreads();
+  const int32_t ntot = off[G::NCELL];
+  // pass 1: the first K*NT region particles are kept in registers
+  PPent re[K];
+  int rw[K];
+  bool rk[K];
+  int32_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int32_t t = threadIdx.x + k * NT;
+    rk[k] = t < ntot && pf_cand<PB>(g, xp, beg, off, t, pr, W, rw[k], re[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (rk[k]) {
+      atomicAdd(&wc[1 + rw[k]], 1);
+      mine++;
+    }
+  for (int32_t t = threadIdx.x + K * NT; t < ntot; t += NT) {
+    int wi;
+    PPent e;
+    if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) {
+      atomicAdd(&wc[1 + wi], 1);
+      mine++;
+    }
+  }
+  if (mine) atomicAdd(&ncand_s, mine);
+  __syncthreads();
+  const int32_t ncand = ncand_s;
+  if (ncand > CAP) {
+    if (threadIdx.x == 0) dense[atomicAdd(ndense, 1u)] = (int32_t)blockIdx.x;
+    return;
+  }
+  pf_scan<NT>(wc + 1, W3, wsum);
+  // scatter; afterwards wc[1+i] is the end of run i, so wc[i] is the start of run i
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (rk[k]) ent[atomicAdd(&wc[1 + rw[k]], 1)] = re[k];
+  for (int32_t t = threadIdx.x + K * NT; t < ntot; t += NT) {
+    int wi;
+    PPent e;
+    if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) ent[atomicAdd(&wc[1 + wi], 1)] = e;
+  }
+  __syncthreads();
+  float f2loc = 0.f;
+  unsigned long long npl = 0;
+  int32_t ninner = 0;
+  for (int32_t t = threadIdx.x; t < ncand; t += NT) {
+    const PPent ti = ent[t];
+    const int fx = (int)floorf(ti.x), fy = (int)floorf(ti.y), fz = (int)floorf(ti.z);
+    if ((unsigned)fx >= (unsigned)G::FB || (unsigned)fy >= (unsigned)G::FB || (unsigned)fz >= (unsigned)G::FB) continue;
+    ninner++;
+    int32_t vc[3] = {0, 0, 0};
+    if (!probe)
+      for (int e = 0; e < 3; e++) vc[e] = load_v(vp, 3 * (int64_t)ti.p + e, g.izipv);
+    long long fa[3] = {0, 0, 0};
+    pf_force(ent, wc, W, pr, ti, c.mass, c.rsoft, fa, npl);
+    f2loc = fmaxf(f2loc, pf_apply(g, c, vp, probe, ti.p, vc, fa));
+  }
+  pf_reduce<NT>(f2loc, npl, ninner, redf, redn, redi, f2max_bits, npairs_out, nupd_out);
+}
+
+// Dense blocks, step 1 (one workgroup per listed block, grid-stride): sort
+// the candidates into the pool as [start table: W^3+1 int32][entries], and
+// cut the block's targets into work items of PD_NT.
+template <int PB>
+__global__ __launch_bounds__(PD_NT) void k_pp_dense_prep(Geo g, int pr, const void* __restrict__ xp,
+                                                         const int64_t* __restrict__ cstart, const int32_t* __restrict__ dense,
+                                                         const uint32_t* __restrict__ ndense, PPent* pool, uint64_t pool_cap,
+                                                         unsigned long long* pool_used, int64_t* info, int64_t* items,
+                                                         uint32_t* nitems, int32_t* fall, uint32_t* nfall) {
+  using G = PBgeo<PB>;
+  constexpr int NR = G::FB * G::FB;  // inner rows
+  extern __shared__ __align__(16) uint8_t pf_dyn[];
+  __shared__ int64_t beg[G::NCELL];
+  __shared__ int32_t off[G::NCELL + 1];
+  __shared__ int32_t wsum[PD_NT / 64];
+  __shared__ int32_t rowc[NR + 1];
+  __shared__ unsigned long long base_s;
+  const int W = G::FB + 2 * pr, W3 = W * W * W;
+  int32_t* wcnt = (int32_t*)pf_dyn;
+  const uint64_t tbl_units = (uint64_t)(W3 + 1 + 3) / 4;  // the table in 16-byte units
+  const uint32_t nd = *ndense;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const int32_t b = dense[di];
+    int c0x, c0y, c0z;
+    pf_origin<PB>(g, b, c0x, c0y, c0z);
+    for (int i = threadIdx.x; i <= W3; i += PD_NT) wcnt[i] = 0;
+    pf_region<PB, PD_NT>(g, cstart, c0x, c0y, c0z, beg, off, wsum);
+    __syncthreads();
+    const int32_t ntot = off[G::NCELL];
+    for (int32_t t = threadIdx.x; t < ntot; t += PD_NT) {
+      int wi;
+      PPent e;
+      if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) atomicAdd(&wcnt[wi], 1);
+    }
+    __syncthreads();
+    pf_scan<PD_NT>(wcnt, W3, wsum);
+    const int32_t ncand = wcnt[W3];
+    // targets: inner rows (y, z in [pr, pr+FB)), x-runs [pr, pr+FB)
+    int32_t n = 0;
+    if ((int)threadIdx.x < NR) {
+      const int y = (threadIdx.x % G::FB) + pr, z = (threadIdx.x / G::FB) + pr;
+      const int row = (z * W + y) * W;
+      n = wcnt[row + pr + G::FB] - wcnt[row + pr];
+    }
+    const int32_t incl = block_incl_scan<PD_NT>(n, wsum);
+    if ((int)threadIdx.x < NR) rowc[threadIdx.x + 1] = incl;
+    if (threadIdx.x == 0) {
+      rowc[0] = 0;
+      const uint64_t units = tbl_units + (uint64_t)ncand;
+      const unsigned long long o = atomicAdd(pool_used, (unsigned long long)units);
+      base_s = o + units <= pool_cap ? o : ~0ull;
+    }
+    __syncthreads();
+    const unsigned long long base = base_s;
+    const int32_t ntarg = rowc[NR];
+    if (base == ~0ull) {
+      if (threadIdx.x == 0) fall[atomicAdd(nfall, 1u)] = b;
+      __syncthreads();
+      continue;
+    }
+    int32_t* tbl = (int32_t*)(pool + base);
+    PPent* ent = pool + base + tbl_units;
+    for (int i = threadIdx.x; i <= W3; i += PD_NT) tbl[i] = wcnt[i];
+    __syncthreads();
+    for (int32_t t = threadIdx.x; t < ntot; t += PD_NT) {
+      int wi;
+      PPent e;
+      if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) ent[atomicAdd(&wcnt[wi], 1)] = e;
+    }
+    const int nch = (ntarg + PD_NT - 1) / PD_NT;
+    if (threadIdx.x == 0) {
+      info[2 * di] = (int64_t)base;
+      info[2 * di + 1] = ((int64_t)b << 32) | (int64_t)ntarg;
+      const uint32_t i0 = atomicAdd(nitems, (uint32_t)nch);
+      for (int ch = 0; ch < nch; ch++) items[i0 + ch] = ((int64_t)di << 32) | ch;
+    }
+    __syncthreads();
+  }
+}
+
+// Dense blocks, step 2: one work item (PD_NT targets of one block) per
+// workgroup iteration; candidates are read from the pool.
+template <int PB>
+__global__ __launch_bounds__(PD_NT) void k_pp_dense_pairs(Geo g, PPctx c, void* vp, const PPent* __restrict__ pool,
+                                                          const int64_t* __restrict__ info, const int64_t* __restrict__ items,
+                                                          const uint32_t* __restrict__ nitems, uint32_t* f2max_bits,
+                                                          unsigned long long* npairs_out, unsigned long long* nupd_out,
+                                                          double* probe) {
+  using G = PBgeo<PB>;
+  constexpr int NR = G::FB * G::FB;
+  extern __shared__ __align__(16) uint8_t pf_dyn[];
+  __shared__ int32_t rowc[NR + 1];
+  __shared__ int32_t wsum[PD_NT / 64];
+  __shared__ float redf[PD_NT / 64];
+  __shared__ unsigned long long redn[PD_NT / 64];
+  __shared__ int32_t redi[PD_NT / 64];
+  const int pr = c.pr, W = G::FB + 2 * pr, W3 = W * W * W;
+  int32_t* st = (int32_t*)pf_dyn;
+  const uint64_t tbl_units = (uint64_t)(W3 + 1 + 3) / 4;
+  const uint32_t ni = *nitems;
+  for (uint32_t it = blockIdx.x; it < ni; it += gridDim.x) {
+    const int64_t item = items[it];
+    const int64_t di = item >> 32, ch = item & 0xffffffff;
+    const uint64_t base = (uint64_t)info[2 * di];
+    const int32_t ntarg = (int32_t)(info[2 * di + 1] & 0xffffffff);
+    const int32_t* tbl = (const int32_t*)(pool + base);
+    const PPent* ent = pool + base + tbl_units;
+    for (int i = threadIdx.x; i <= W3; i += PD_NT) st[i] = tbl[i];
+    __syncthreads();
+    int32_t n = 0;
+    if ((int)threadIdx.x < NR) {
+      const int y = (threadIdx.x % G::FB) + pr, z = (threadIdx.x / G::FB) + pr;
+      const int row = (z * W + y) * W;
+      n = st[row + pr + G::FB] - st[row + pr];
+    }
+    const int32_t incl = block_incl_scan<PD_NT>(n, wsum);
+    if ((int)threadIdx.x < NR) rowc[threadIdx.x + 1] = incl;
+    if (threadIdx.x == 0) rowc[0] = 0;
+    __syncthreads();
+    float f2loc = 0.f;
+    unsigned long long npl = 0;
+    int32_t ninner = 0;
+    const int32_t k = (int32_t)ch * PD_NT + threadIdx.x;
+    if (k < ntarg) {
+      int a = 0, b = NR;
+      while (b - a > 1) {
+        const int m = (a + b) >> 1;
+        if (rowc[m] <= k) a = m; else b = m;
+      }
+      const int y = (a % G::FB) + pr, z = (a / G::FB) + pr;
+      const int32_t e = st[(z * W + y) * W + pr] + (k - rowc[a]);
+      const PPent ti = ent[e];
+      int32_t vc[3] = {0, 0, 0};
+      if (!probe)
+        for (int q = 0; q < 3; q++) vc[q] = load_v(vp, 3 * (int64_t)ti.p + q, g.izipv);
+      long long fa[3] = {0, 0, 0};
+      pf_force(ent, st, W, pr, ti, c.mass, c.rsoft, fa, npl);
+      f2loc = pf_apply(g, c, vp, probe, ti.p, vc, fa);
+      ninner = 1;
+    }
+    pf_reduce<PD_NT>(f2loc, npl, ninner, redf, redn, redi, f2max_bits, npairs_out, nupd_out);
+  }
+}
+
+template <int PB>
+static int pp_launch(cube_handle* h, const Geo& g, const PPctx& c, uint32_t* f2, unsigned long long* npr, unsigned long long* nup,
+                     double* probe, int32_t* dense, uint32_t* ndense, int64_t* info, int64_t* items, uint32_t* nitems,
+                     int32_t* fall, uint32_t* nfall, unsigned long long* pool_used) {
+  using G = PBgeo<PB>;
+  const int W = G::FB + 2 * c.pr, W3 = W * W * W;
+  const int64_t nbx = h->nc / PB, nb = nbx * nbx * nbx;
+  const size_t lds_fast = pf_wc_bytes(W3) + sizeof(PPent) * PFcfg<PB>::CAP;
+  const size_t lds_tab = sizeof(int32_t) * (W3 + 1);
+  const uint64_t pool_cap = h->fine.bytes_R() / sizeof(PPent);
+  HIP_TRY(h, hipFuncSetAttribute((const void*)k_pp_fast<PB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast));
+  k_pp_fast<PB><<<(unsigned)nb, PFcfg<PB>::NT, lds_fast, h->stream>>>(g, c, h->xp, h->vp, h->cstart, f2, npr, nup, probe, dense,
+                                                                      ndense);
+  HIP_TRY(h, hipGetLastError());
+  k_pp_dense_prep<PB><<<2048, PD_NT, lds_tab, h->stream>>>(g, c.pr, h->xp, h->cstart, dense, ndense, (PPent*)h->fine.W, pool_cap,
+                                                            pool_used, info, items, nitems, fall, nfall);
+  HIP_TRY(h, hipGetLastError());
+  k_pp_dense_pairs<PB><<<8192, PD_NT, lds_tab, h->stream>>>(g, c, h->vp, (const PPent*)h->fine.W, info, items, nitems, f2, npr, nup,
+                                                             probe);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
 }
 
 int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe) {
   Geo g = geo_of(h);
-  if (h->nc % PB) return cube_fail(h, CUBE_ERR_ARG, "nc must be a multiple of %d for PP", PB);
-  if (h->p.pp_range < 1 || h->p.pp_range > 4)
-    return cube_fail(h, CUBE_ERR_ARG, "pp_range=%d outside 1..4 (parameters.f90:62)", h->p.pp_range);
-  uint32_t* f2 = h->uflags + 8;
-  unsigned long long* cnt = (unsigned long long*)(h->uflags + 16);
-  uint32_t* novf = h->uflags + 20;
-  HIP_TRY(h, hipMemsetAsync(h->uflags + 8, 0, sizeof(uint32_t) * 16, h->stream));
-  const int64_t nb = (h->nc / PB) * (h->nc / PB) * (h->nc / PB);
-  int32_t* ovf = (int32_t*)h->order;  // idle outside the drift
-  uint32_t nov = 0;
+  const int prange = h->p.pp_range;
+  if (prange < 1 || prange > 4)
+    return cube_fail(h, CUBE_ERR_ARG, "pp_range=%d outside 1..4 (parameters.f90:62)", prange);
+  const int PBsel = prange <= 2 ? 4 : 2;
+  if (h->nc % PBsel) return cube_fail(h, CUBE_ERR_ARG, "nc must be a multiple of %d for PP", PBsel);
+  // uflags[9] dense blocks; [10] work items; [11] pool fallbacks; [16..17] pool units used.
+  // Per-slot totals in pp_red: pairs and targets [PP_SLOTS] u64 each, f2 max bits [PP_SLOTS] u32.
+  uint32_t* ndense = h->uflags + 9;
+  uint32_t* nitems = h->uflags + 10;
+  uint32_t* nfall = h->uflags + 11;
+  unsigned long long* pool_used = (unsigned long long*)(h->uflags + 16);
+  unsigned long long* npr = (unsigned long long*)h->pp_red;
+  unsigned long long* nup = npr + PP_SLOTS;
+  uint32_t* f2 = (uint32_t*)(nup + PP_SLOTS);
+  HIP_TRY(h, hipMemsetAsync(h->uflags + 8, 0, sizeof(uint32_t) * 12, h->stream));
+  HIP_TRY(h, hipMemsetAsync(h->pp_red, 0, PP_SLOTS * (2 * sizeof(unsigned long long) + sizeof(uint32_t)), h->stream));
+  // pp_meta: per 2^3-cell block (the finer of the two block sizes) 2 int64 of
+  // info and 2 int32 list entries, plus the work items
+  const int64_t nb2 = (h->nc / 2) * (h->nc / 2) * (h->nc / 2);
+  int64_t* info = (int64_t*)h->pp_meta;
+  int64_t* items = info + 2 * nb2;
+  int32_t* dense = (int32_t*)(items + h->p.np_max / PD_NT + nb2 + 1);
+  int32_t* fall = dense + nb2;
+  PPctx c{h->mass_p, h->p.rsoft, a_mid, dt, c_old, h->vdec, h->venc, h->vkmax, prange};
   int pr = prof_begin(h, PS_PP);
-  if (h->p.pp_range > 2) {  // window beyond the binned kernel's LDS table: chunked kernel for every block
-    const int nbs = (int)((h->nc / PP_BP) * (h->nc / PP_BP) * (h->nc / PP_BP));
-    k_pp<<<(unsigned)nbs, PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, h->p.pp_range, a_mid, dt,
-                                                 c_old, h->vdec, h->venc, h->vkmax, f2, cnt, cnt + 1, probe, nullptr, 1);
-    HIP_TRY(h, hipGetLastError());
-  } else {
-  k_pp_binned<<<(unsigned)nb, PB_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, h->p.pp_range, a_mid, dt,
-                                                      c_old, h->vdec, h->venc, h->vkmax, f2, cnt, cnt + 1, probe, ovf, novf);
-  HIP_TRY(h, hipGetLastError());
-  HIP_TRY(h, hipMemcpyAsync(&nov, novf, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+  int rc = PBsel == 4 ? pp_launch<4>(h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used)
+                      : pp_launch<2>(h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used);
+  if (rc) return rc;
+  uint32_t hv[12];
+  HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 8, sizeof(hv), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  if (nov) {
-    const int bscale = PB / PP_BP, nsub = bscale * bscale * bscale;
-    k_pp<<<(unsigned)((int64_t)nov * nsub), PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, h->p.pp_range,
-                                                                   a_mid, dt, c_old, h->vdec, h->venc, h->vkmax, f2, cnt, cnt + 1,
-                                                                   probe, ovf, bscale);
+  const uint32_t nfb = hv[3];
+  if (nfb) {  // blocks the pool could not hold: chunked kernel on their PP_BP^3 sub-blocks
+    const int bscale = PBsel / PP_BP, nsub = bscale * bscale * bscale;
+    k_pp<<<(unsigned)((int64_t)nfb * nsub), PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, prange, a_mid,
+                                                                   dt, c_old, h->vdec, h->venc, h->vkmax, f2, npr, nup, probe,
+                                                                   fall, bscale);
     HIP_TRY(h, hipGetLastError());
-  }
   }
   prof_end(h, pr);
-  uint32_t hv[12];
-  HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 8, sizeof(uint32_t) * 12, hipMemcpyDeviceToHost, h->stream));
+  std::vector<unsigned long long> red(2 * PP_SLOTS);
+  std::vector<uint32_t> f2b(PP_SLOTS);
+  HIP_TRY(h, hipMemcpyAsync(red.data(), npr, sizeof(unsigned long long) * 2 * PP_SLOTS, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(f2b.data(), f2, sizeof(uint32_t) * PP_SLOTS, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  unsigned long long npairs = 0, nupd = 0;
+  uint32_t f2m = 0;
+  for (int i = 0; i < PP_SLOTS; i++) {
+    npairs += red[i];
+    nupd += red[PP_SLOTS + i];
+    f2m = std::max(f2m, f2b[i]);  // non-negative floats order as their bit patterns
+  }
   float f2max;
-  memcpy(&f2max, &hv[0], 4);
-  unsigned long long c2[2];
-  memcpy(c2, &hv[8], sizeof(c2));
+  memcpy(&f2max, &f2m, 4);
   if (out) {
     out->f2_max_pp = f2max;
-    out->npairs = (int64_t)c2[0];
-    out->itest1 = (int64_t)c2[1];
+    out->npairs = (int64_t)npairs;
+    out->itest1 = (int64_t)nupd;
+    out->dense_blocks = (int64_t)hv[1];
   }
-  h->pp_overflow_blocks = nov;
+  h->pp_overflow_blocks = nfb;
   return 0;
 }

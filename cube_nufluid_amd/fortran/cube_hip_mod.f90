@@ -40,7 +40,7 @@ module cube_hip
 
   type, bind(C) :: cube_pp_out
     real(c_float) :: dt_pp, f2_max_pp
-    integer(c_int64_t) :: npairs, itest1
+    integer(c_int64_t) :: npairs, itest1, dense_blocks
   end type
 
   type, bind(C) :: cube_pm_out

@@ -92,6 +92,7 @@ typedef struct cube_pp_out {
   float f2_max_pp;
   int64_t npairs;        /* pair evaluations incl. self (:98)                */
   int64_t itest1;        /* particles updated (:87)                          */
+  int64_t dense_blocks;  /* blocks too crowded for the LDS path (diagnostic)  */
 } cube_pp_out;
 
 /* Results of one particle_mesh call (pm.f90:427-446). */

@@ -33,5 +33,5 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.SimHeader) == 224   # parameters.f90:129-153
     assert ctypes.sizeof(_lib.CubeParams) == 80
     assert ctypes.sizeof(_lib.DriftOut) == 56
-    assert ctypes.sizeof(_lib.PPOut) == 24
+    assert ctypes.sizeof(_lib.PPOut) == 32
     assert ctypes.sizeof(_lib.PMOut) == 56
