@@ -89,6 +89,7 @@ SIGNATURES = {
     "cube_probe_pm_forces": (c_i32, [_P, _P, _P]),
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
+    "cube_power_spectrum": (c_i32, [_P, c_i32, _P, _P, _P]),
     "cube_ic_zeldovich": (c_i32, [_P, ctypes.POINTER(ICParams), ctypes.POINTER(SimHeader)]),
     "cube_profile_enable": (c_i32, [_P, c_i32]),
     "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),

@@ -26,6 +26,7 @@ SOURCES = {
     "scan.hip": [],
     "fft.hip": [],
     "ic.hip": ["-ffp-contract=off"],
+    "pk.hip": [],
 }
 
 COMMON = [

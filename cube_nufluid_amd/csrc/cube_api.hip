@@ -109,8 +109,8 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
     return fail(cube_fail(h, CUBE_ERR_ARG, "ncb=%lld outside [1, nt]", (long long)p->ncb));
   if ((p->izipx != 1 && p->izipx != 2) || (p->izipv != 1 && p->izipv != 2))
     return fail(cube_fail(h, CUBE_ERR_ARG, "izipx/izipv must be 1 or 2"));
-  if (p->precision != 4)
-    return fail(cube_fail(h, CUBE_ERR_ARG, "precision=%d: this build runs the FP32 path (4)", p->precision));
+  if (p->precision != 4 && p->precision != 8)
+    return fail(cube_fail(h, CUBE_ERR_ARG, "precision=%d: 4 (FP32 production path) or 8 (FP64 oracle)", p->precision));
   if (p->np_max < 1 || p->np_max >= ((int64_t)1 << 31))
     return fail(cube_fail(h, CUBE_ERR_ARG, "np_max=%lld outside [1, 2^31)", (long long)p->np_max));
   h->prec = p->precision;
@@ -439,7 +439,14 @@ extern "C" int cube_probe_density(cube_handle* h, float* rho) {
   if (r) return r;
   const int64_t N = h->fine.N;
   HIP_TRY(h, hipStreamSynchronize(h->stream));
-  HIP_TRY(h, hipMemcpy2D(rho, sizeof(float) * N, h->fine.R, sizeof(float) * (N + 2), sizeof(float) * N, N * N,
-                         hipMemcpyDeviceToHost));
+  if (h->prec == 4) {
+    HIP_TRY(h, hipMemcpy2D(rho, sizeof(float) * N, h->fine.R, sizeof(float) * (N + 2), sizeof(float) * N, N * N,
+                           hipMemcpyDeviceToHost));
+  } else {
+    std::vector<double> row((size_t)N * N * N);
+    HIP_TRY(h, hipMemcpy2D(row.data(), sizeof(double) * N, h->fine.R, sizeof(double) * (N + 2), sizeof(double) * N, N * N,
+                           hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < row.size(); i++) rho[i] = (float)row[i];
+  }
   return CUBE_OK;
 }

@@ -63,8 +63,9 @@ __device__ __forceinline__ void dft_small(cpx<T>* v) {
   }
 }
 
-// One Stockham pass over all W columns held in s[n*W + w].
-template <typename T, int N, int W, bool INV, int R, int NS>
+// One Stockham pass over all W columns held in s[n*LD + w] (LD >= W; a
+// padded LD keeps transposed loads free of LDS bank conflicts).
+template <typename T, int N, int W, int LD, bool INV, int R, int NS>
 __device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ tw, int tws) {
   constexpr int B = N / R;
   constexpr int TOT = B * W;
@@ -79,7 +80,7 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ t
       int k = j % NS;
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        cpx<T> a = s[(j + r * B) * W + w];
+        cpx<T> a = s[(j + r * B) * LD + w];
         if (NS > 1 && r > 0) {
           cpx<T> t = tw[(k * r * (N / (NS * R))) * tws];
           if (INV) t.y = -t.y;
@@ -98,27 +99,42 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ t
       int w = b % W, j = b / W;
       int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-      for (int r = 0; r < R; r++) s[(idxD + r * NS) * W + w] = v[q][r];
+      for (int r = 0; r < R; r++) s[(idxD + r * NS) * LD + w] = v[q][r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int W, bool INV, int NS>
+template <typename T, int N, int W, int LD, bool INV, int NS>
 struct FftPasses {
   __device__ __forceinline__ static void run(cpx<T>* s, const cpx<T>* tw, int tws) {
     if constexpr (NS < N) {
       constexpr int REM = N / NS;
       constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
-      fft_pass<T, N, W, INV, R, NS>(s, tw, tws);
-      FftPasses<T, N, W, INV, NS * R>::run(s, tw, tws);
+      fft_pass<T, N, W, LD, INV, R, NS>(s, tw, tws);
+      FftPasses<T, N, W, LD, INV, NS * R>::run(s, tw, tws);
     }
   }
 };
 // In-LDS FFT of W columns of length N (twiddles tw[m*tws] = exp(-2 pi i m/N)).
-template <typename T, int N, int W, bool INV>
+template <typename T, int N, int W, bool INV, int LD = W>
 __device__ __forceinline__ void lds_fft(cpx<T>* s, const cpx<T>* tw, int tws) {
-  FftPasses<T, N, W, INV, 1>::run(s, tw, tws);
+  FftPasses<T, N, W, LD, INV, 1>::run(s, tw, tws);
+}
+
+// Workgroups go to the 8 XCDs round-robin by linear id and each XCD has its
+// own L2.  Logical block ids are assigned so that every XCD runs one
+// contiguous range: blocks that share cache lines (neighbouring columns of a
+// strided pass) then run on the same XCD at about the same time.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t L, uint32_t T) {
+  const uint32_t q = T >> 3, r = T & 7, x = L & 7, sl = L >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + sl;
+}
+__device__ __forceinline__ void grid_block_xy(int& bx, int& by) {
+  const uint32_t L = blockIdx.x + gridDim.x * blockIdx.y, T = gridDim.x * gridDim.y;
+  const uint32_t lg = xcd_remap(L, T);
+  bx = (int)(lg % gridDim.x);
+  by = (int)(lg / gridDim.x);
 }
 
 template <typename T>
@@ -132,21 +148,21 @@ constexpr int pick_w(int logn) {
 // rows of N reals (pitch N+2) -> N/2+1 complex, in place.  nrows = N*N.
 template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_fft_x_r2c(cpx<T>* data, const cpx<T>* __restrict__ tw, int64_t nrows) {
-  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1);
-  __shared__ cpx<T> s[H * W];
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
+  __shared__ cpx<T> s[H * LD];
   const int64_t r0 = (int64_t)blockIdx.x * W;
   for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
     int w = idx / H, m = idx % H;
     int64_t row = r0 + w;
-    s[m * W + w] = row < nrows ? data[row * (H + 1) + m] : cpx<T>{0, 0};
+    s[m * LD + w] = row < nrows ? data[row * (H + 1) + m] : cpx<T>{0, 0};
   }
   __syncthreads();
-  lds_fft<T, H, W, false>(s, tw, 2);
+  lds_fft<T, H, W, false, LD>(s, tw, 2);
   for (int idx = threadIdx.x; idx < (H + 1) * W; idx += NT_FFT) {
     int w = idx / (H + 1), k = idx % (H + 1);
     int64_t row = r0 + w;
-    cpx<T> zk = s[(k % H) * W + w];
-    cpx<T> zc = s[((H - k) % H) * W + w];
+    cpx<T> zk = s[(k % H) * LD + w];
+    cpx<T> zc = s[((H - k) % H) * LD + w];
     zc.y = -zc.y;
     cpx<T> e{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
     cpx<T> dd = csub(zk, zc);
@@ -163,36 +179,49 @@ template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst, int64_t opitch,
                                                        const cpx<T>* __restrict__ tw, int64_t nrows,
                                                        int smode, float* S, uint32_t* f2max) {
-  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1);
-  __shared__ cpx<T> xs[(H + 1) * W];
-  __shared__ cpx<T> s[H * W];
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
+  __shared__ cpx<T> s[H * LD];
+  __shared__ cpx<T> xnyq[W];
   __shared__ float red[NT_FFT / 64];
   const int64_t r0 = (int64_t)blockIdx.x * W;
   for (int idx = threadIdx.x; idx < (H + 1) * W; idx += NT_FFT) {
     int w = idx / (H + 1), k = idx % (H + 1);
     int64_t row = r0 + w;
-    xs[k * W + w] = row < nrows ? src[row * (H + 1) + k] : cpx<T>{0, 0};
+    cpx<T> v = row < nrows ? src[row * (H + 1) + k] : cpx<T>{0, 0};
+    if (k < H) s[k * LD + w] = v; else xnyq[w] = v;
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
-    int w = idx / H, k = idx % H;
-    cpx<T> a = xs[k * W + w];
-    cpx<T> b = xs[(H - k) * W + w];
-    b.y = -b.y;
-    cpx<T> sum = cadd(a, b), dif = csub(a, b);
-    cpx<T> t = tw[k];
-    t.y = -t.y;                                // exp(+2 pi i k/N)
-    cpx<T> td = cmul(t, dif);
-    s[k * W + w] = cpx<T>{sum.x - td.y, sum.y + td.x};  // sum + i*t*dif
+  // Z[k] = (X[k] + conj X[H-k]) + i exp(+2 pi i k/N) (X[k] - conj X[H-k]),
+  // in place: one thread per pair (k, H-k), k = 0..H/2
+  for (int idx = threadIdx.x; idx < (H / 2 + 1) * W; idx += NT_FFT) {
+    int w = idx / (H / 2 + 1), k = idx % (H / 2 + 1);
+    auto zof = [&](cpx<T> a, cpx<T> b, int kk) {
+      b.y = -b.y;
+      cpx<T> sum = cadd(a, b), dif = csub(a, b);
+      cpx<T> t = tw[kk];
+      t.y = -t.y;  // exp(+2 pi i kk/N)
+      cpx<T> td = cmul(t, dif);
+      return cpx<T>{sum.x - td.y, sum.y + td.x};  // sum + i*t*dif
+    };
+    cpx<T> a = s[k * LD + w];
+    if (k == 0) {
+      s[w] = zof(a, xnyq[w], 0);
+    } else if (k == H / 2) {
+      s[k * LD + w] = zof(a, a, k);
+    } else {
+      cpx<T> b = s[(H - k) * LD + w];
+      s[k * LD + w] = zof(a, b, k);
+      s[(H - k) * LD + w] = zof(b, a, H - k);
+    }
   }
   __syncthreads();
-  lds_fft<T, H, W, true>(s, tw, 2);
+  lds_fft<T, H, W, true, LD>(s, tw, 2);
   float mymax = 0.f;
   for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
     int w = idx / H, m = idx % H;
     int64_t row = r0 + w;
     if (row >= nrows) continue;
-    cpx<T> z = s[m * W + w];
+    cpx<T> z = s[m * LD + w];
     *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
     if (smode) {
       float f0 = (float)z.x, f1 = (float)z.y;
@@ -220,15 +249,17 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst,
 }
 
 // ----------------------------------------------------------- strided Y / Z
-// AXIS 1: columns along y (stride H+1) in plane blockIdx.y = z.
-// AXIS 2: columns along z (stride N(H+1)) at blockIdx.y = y.
+// AXIS 1: columns along y (stride H+1) in plane by = z.
+// AXIS 2: columns along z (stride N(H+1)) at by = y; (bx, by) is the XCD-remapped block.
 template <typename T, int LOGN, int AXIS, bool INV>
 __global__ __launch_bounds__(NT_FFT) void k_fft_strided(cpx<T>* data, const cpx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN, KX = N / 2 + 1, W = pick_w<T>(LOGN);
   __shared__ cpx<T> s[N * W];
-  const int kx0 = blockIdx.x * W;
+  int bx, by;
+  grid_block_xy(bx, by);
+  const int kx0 = bx * W;
   const int64_t stride = AXIS == 1 ? (int64_t)KX : (int64_t)N * KX;
-  const int64_t base = AXIS == 1 ? (int64_t)blockIdx.y * N * KX + kx0 : (int64_t)blockIdx.y * KX + kx0;
+  const int64_t base = AXIS == 1 ? (int64_t)by * N * KX + kx0 : (int64_t)by * KX + kx0;
   for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
     int n = idx / W, w = idx % W;
     s[idx] = (kx0 + w < KX) ? data[base + n * stride + w] : cpx<T>{0, 0};
@@ -242,7 +273,7 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_strided(cpx<T>* data, const cpx<
 }
 
 // ------------------------------------------------------ fused Z convolution
-// dst = Zinv( i*K_d . Zfwd(src) ) along z for columns (kx0.., ky=blockIdx.y).
+// dst = Zinv( i*K_d . Zfwd(src) ) along z for columns (kx0.., ky = by).
 // K is the imaginary part of the kernel transform over one octant,
 // [d][ky'][kx][kz'] with ky'=min(ky,N-ky), kz'=min(kz,N-kz); component d is
 // odd in its own axis and even in the other two.
@@ -251,8 +282,10 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
                                                        int d, const cpx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN, KX = N / 2 + 1, H = N / 2 + 1, W = pick_w<T>(LOGN);
   __shared__ cpx<T> s[N * W];
-  const int kx0 = blockIdx.x * W;
-  const int ky = blockIdx.y;
+  int bx, by;
+  grid_block_xy(bx, by);
+  const int kx0 = bx * W;
+  const int ky = by;
   const int64_t stride = (int64_t)N * KX;
   const int64_t base = (int64_t)ky * KX + kx0;
   for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {

@@ -53,9 +53,10 @@ __device__ __forceinline__ int seg_find(const int32_t* off, int n, int32_t t) {
 // of the order in which threads add them; the node value is the sum rounded
 // once to float.
 #define CIC_FIX 1099511627776.0  // 2^40
-template <int BC>
+template <int BC, typename T>
 __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
-                                                     float mass, float* R, int64_t N, double* part) {
+                                                     float mass_f, T* R, int64_t N, double* part) {
+  const T mass = (T)mass_f;
   constexpr int F = 4 * BC, SPAN = BC + 1, NCELL = SPAN * SPAN * SPAN;
   __shared__ unsigned long long s[F * F * F];
   __shared__ int64_t beg[NCELL];
@@ -74,21 +75,21 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
     int64_t p = beg[ci] + (t - off[ci]);
     int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
     int l0[3] = {l0x, l0y, l0z};
-    float dx1[3], dx2[3];
+    T dx1[3], dx2[3];
     int n1[3];
     for (int e = 0; e < 3; e++) {
       int il = l0[e] + rel[e] + 1;  // tile-local 1-based cell index (0 = left ghost)
       uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
-      float tx = (float)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
-      int a1 = (int)floorf(tx) + 1;
-      dx1[e] = (float)a1 - tx;
+      T tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
+      int a1 = (int)floor(tx) + 1;
+      dx1[e] = (T)a1 - tx;
       dx2[e] = 1 - dx1[e];
       n1[e] = (a1 - 1) - CUBE_NCELL * l0[e];  // block-local node of tap 1
     }
-    const float w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
-                        dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
-                        dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
-                        dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
+    const T w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
+                    dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
+                    dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
+                    dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
     const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
     int x = i % F, y = (i / F) % F, z = i / (F * F);
     int64_t gx = 4 * c0x + x, gy = 4 * c0y + y, gz = 4 * c0z + z;
     double vd = (double)(long long)s[i] * (1.0 / CIC_FIX);
-    R[(gz * N + gy) * (N + 2) + gx] = (float)vd;
+    R[(gz * N + gy) * (N + 2) + gx] = (T)vd;
     ms += vd;
   }
   for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
@@ -119,9 +120,10 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
 // ------------------------------------------------------------ coarse CIC
 // Nodes at coarse-cell centres (pm.f90:255 '-0.5'); a particle in cell c
 // touches nodes c-1..c+1, so the block gathers cells [c0-1, c0+BC].
-template <int BC>
+template <int BC, typename T>
 __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
-                                                       float mass, float* R, double* part) {
+                                                       float mass_f, T* R, double* part) {
+  const T mass = (T)mass_f;
   constexpr int SPAN = BC + 2, NCELL = SPAN * SPAN * SPAN;
   __shared__ unsigned long long s[BC * BC * BC];
   __shared__ int64_t beg[NCELL];
@@ -139,21 +141,21 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
     int64_t p = beg[ci] + (t - off[ci]);
     int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
     int l0[3] = {l0x, l0y, l0z};
-    float dx1[3], dx2[3];
+    T dx1[3], dx2[3];
     int n1[3];
     for (int e = 0; e < 3; e++) {
       int il = l0[e] + rel[e] + 1;
       uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
-      float tx = (float)(((double)(il - 1) + xoff(u, g.izipx)) - 0.5);
-      int a1 = (int)floorf(tx) + 1;  // 1-based tile-local node
-      dx1[e] = (float)a1 - tx;
+      T tx = (T)(((double)(il - 1) + xoff(u, g.izipx)) - 0.5);
+      int a1 = (int)floor(tx) + 1;  // 1-based tile-local node
+      dx1[e] = (T)a1 - tx;
       dx2[e] = 1 - dx1[e];
       n1[e] = (a1 - 1) - l0[e];
     }
-    const float w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
-                        dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
-                        dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
-                        dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
+    const T w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
+                    dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
+                    dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
+                    dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
     const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
   for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) {
     int x = i % BC, y = (i / BC) % BC, z = i / (BC * BC);
     double vd = (double)(long long)s[i] * (1.0 / CIC_FIX);
-    R[((int64_t)(c0z + z) * N + (c0y + y)) * (N + 2) + (c0x + x)] = (float)vd;
+    R[((int64_t)(c0z + z) * N + (c0y + y)) * (N + 2) + (c0x + x)] = (T)vd;
     ms += vd;
   }
   for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
@@ -187,7 +189,11 @@ int cic_fine(cube_handle* h, double* mass) {
   int64_t nb = (h->nc / 4) * (h->nc / 4) * (h->nc / 4);
   if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   int pr = prof_begin(h, PS_CIC_F);
-  k_cic_fine<4><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
+  if (h->prec == 4)
+    k_cic_fine<4, float><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
+  else
+    k_cic_fine<4, double><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (double*)h->fine.R, h->fine.N,
+                                                                 h->dred);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
@@ -197,7 +203,10 @@ int cic_coarse(cube_handle* h, double* mass) {
   Geo g = geo_of(h);
   int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
   int pr = prof_begin(h, PS_CIC_C);
-  k_cic_coarse<8><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->coarse.R, h->dred);
+  if (h->prec == 4)
+    k_cic_coarse<8, float><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->coarse.R, h->dred);
+  else
+    k_cic_coarse<8, double><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (double*)h->coarse.R, h->dred);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
@@ -211,10 +220,11 @@ int cic_coarse(cube_handle* h, double* mass) {
 // independent in both kicks, so running them per component reproduces the
 // reference's sequence exactly.  With probe arrays set, the kick is
 // replaced by writing the interpolated forces.
+template <typename T>
 __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                                   int64_t n3c, const float* __restrict__ vfield,
-                                                  const float* __restrict__ Ff, int64_t Nf,
-                                                  const float* __restrict__ Fc,
+                                                  const T* __restrict__ Ff, int64_t Nf,
+                                                  const T* __restrict__ Fc,
                                                   float a_mid, float dt, double c_old, double c_new,
                                                   const float* __restrict__ vdec, const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
@@ -238,22 +248,22 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __re
     int64_t r = r0 + lo;
     int32_t c[3];
     cell_of_rank(g, r, c);
-    float dx1[3], dx2[3], cx1[3], cx2[3];
+    T dx1[3], dx2[3], cx1[3], cx2[3];
     int64_t f1[3], f2[3], k1[3], k2[3];
     for (int e = 0; e < 3; e++) {
       int torg = (c[e] / g.nt) * g.nt;
       int il = c[e] - torg + 1;
       double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
-      float tx = (float)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xo);
-      int a1 = (int)floorf(tx) + 1;
-      dx1[e] = (float)a1 - tx;
+      T tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xo);
+      int a1 = (int)floor(tx) + 1;
+      dx1[e] = (T)a1 - tx;
       dx2[e] = 1 - dx1[e];
       // a tile-local float coordinate may round up to 4*nt; wrap both taps
       f1[e] = ((int64_t)CUBE_NCELL * torg + a1 - 1) % Nf;
       f2[e] = (f1[e] + 1) % Nf;
-      float tc = (float)(((double)(torg + il - 1) + xo) - 0.5);
-      int b1 = (int)floorf(tc) + 1;  // 1-based global node, 0..nc (0 and nc+1 are halo)
-      cx1[e] = (float)b1 - tc;
+      T tc = (T)(((double)(torg + il - 1) + xo) - 0.5);
+      int b1 = (int)floor(tc) + 1;  // 1-based global node, 0..nc (0 and nc+1 are halo)
+      cx1[e] = (T)b1 - tc;
       cx2[e] = 1 - cx1[e];
       k1[e] = b1 - 1 < 0 ? Nc - 1 : b1 - 1;
       k2[e] = b1 >= Nc ? b1 - Nc : b1;
@@ -284,26 +294,27 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __re
       probe_c[3 * p + d] = s;
       continue;
     }
-    const float A = a_mid, T = dt;
+    // FP32 meshes: the reference's float expression F*a_mid*dt/6/pi*w
+    const T A = (T)a_mid, DT = (T)dt, SIX = (T)6, PI = (T)PI_F;
     double v = vdecode(vdec, load_v(vp, 3 * p + d, g.izipv), g.izipv, c_old);
-    v = v + (double)(FFI(f1[0], f1[1], f1[2]) * A * T / 6.0f / PI_F * dx1[0] * dx1[1] * dx1[2]);
-    v = v + (double)(FFI(f2[0], f1[1], f1[2]) * A * T / 6.0f / PI_F * dx2[0] * dx1[1] * dx1[2]);
-    v = v + (double)(FFI(f1[0], f2[1], f1[2]) * A * T / 6.0f / PI_F * dx1[0] * dx2[1] * dx1[2]);
-    v = v + (double)(FFI(f1[0], f1[1], f2[2]) * A * T / 6.0f / PI_F * dx1[0] * dx1[1] * dx2[2]);
-    v = v + (double)(FFI(f1[0], f2[1], f2[2]) * A * T / 6.0f / PI_F * dx1[0] * dx2[1] * dx2[2]);
-    v = v + (double)(FFI(f2[0], f1[1], f2[2]) * A * T / 6.0f / PI_F * dx2[0] * dx1[1] * dx2[2]);
-    v = v + (double)(FFI(f2[0], f2[1], f1[2]) * A * T / 6.0f / PI_F * dx2[0] * dx2[1] * dx1[2]);
-    v = v + (double)(FFI(f2[0], f2[1], f2[2]) * A * T / 6.0f / PI_F * dx2[0] * dx2[1] * dx2[2]);
+    v = v + (double)(FFI(f1[0], f1[1], f1[2]) * A * DT / SIX / PI * dx1[0] * dx1[1] * dx1[2]);
+    v = v + (double)(FFI(f2[0], f1[1], f1[2]) * A * DT / SIX / PI * dx2[0] * dx1[1] * dx1[2]);
+    v = v + (double)(FFI(f1[0], f2[1], f1[2]) * A * DT / SIX / PI * dx1[0] * dx2[1] * dx1[2]);
+    v = v + (double)(FFI(f1[0], f1[1], f2[2]) * A * DT / SIX / PI * dx1[0] * dx1[1] * dx2[2]);
+    v = v + (double)(FFI(f1[0], f2[1], f2[2]) * A * DT / SIX / PI * dx1[0] * dx2[1] * dx2[2]);
+    v = v + (double)(FFI(f2[0], f1[1], f2[2]) * A * DT / SIX / PI * dx2[0] * dx1[1] * dx2[2]);
+    v = v + (double)(FFI(f2[0], f2[1], f1[2]) * A * DT / SIX / PI * dx2[0] * dx2[1] * dx1[2]);
+    v = v + (double)(FFI(f2[0], f2[1], f2[2]) * A * DT / SIX / PI * dx2[0] * dx2[1] * dx2[2]);
     int32_t q = vencode(venc, kmax, g.izipv, c_new, v);
     v = vdecode(vdec, q, g.izipv, c_new);
-    v = v + (double)(FCI(k1[0], k1[1], k1[2]) * A * T / 6.0f / PI_F * cx1[0] * cx1[1] * cx1[2]);
-    v = v + (double)(FCI(k2[0], k1[1], k1[2]) * A * T / 6.0f / PI_F * cx2[0] * cx1[1] * cx1[2]);
-    v = v + (double)(FCI(k1[0], k2[1], k1[2]) * A * T / 6.0f / PI_F * cx1[0] * cx2[1] * cx1[2]);
-    v = v + (double)(FCI(k1[0], k1[1], k2[2]) * A * T / 6.0f / PI_F * cx1[0] * cx1[1] * cx2[2]);
-    v = v + (double)(FCI(k1[0], k2[1], k2[2]) * A * T / 6.0f / PI_F * cx1[0] * cx2[1] * cx2[2]);
-    v = v + (double)(FCI(k2[0], k1[1], k2[2]) * A * T / 6.0f / PI_F * cx2[0] * cx1[1] * cx2[2]);
-    v = v + (double)(FCI(k2[0], k2[1], k1[2]) * A * T / 6.0f / PI_F * cx2[0] * cx2[1] * cx1[2]);
-    v = v + (double)(FCI(k2[0], k2[1], k2[2]) * A * T / 6.0f / PI_F * cx2[0] * cx2[1] * cx2[2]);
+    v = v + (double)(FCI(k1[0], k1[1], k1[2]) * A * DT / SIX / PI * cx1[0] * cx1[1] * cx1[2]);
+    v = v + (double)(FCI(k2[0], k1[1], k1[2]) * A * DT / SIX / PI * cx2[0] * cx1[1] * cx1[2]);
+    v = v + (double)(FCI(k1[0], k2[1], k1[2]) * A * DT / SIX / PI * cx1[0] * cx2[1] * cx1[2]);
+    v = v + (double)(FCI(k1[0], k1[1], k2[2]) * A * DT / SIX / PI * cx1[0] * cx1[1] * cx2[2]);
+    v = v + (double)(FCI(k1[0], k2[1], k2[2]) * A * DT / SIX / PI * cx1[0] * cx2[1] * cx2[2]);
+    v = v + (double)(FCI(k2[0], k1[1], k2[2]) * A * DT / SIX / PI * cx2[0] * cx1[1] * cx2[2]);
+    v = v + (double)(FCI(k2[0], k2[1], k1[2]) * A * DT / SIX / PI * cx2[0] * cx2[1] * cx1[2]);
+    v = v + (double)(FCI(k2[0], k2[1], k2[2]) * A * DT / SIX / PI * cx2[0] * cx2[1] * cx2[2]);
 #undef FFI
 #undef FCI
     float av = (float)fabs(v + (double)vfield[r * 3 + d]);
@@ -326,11 +337,16 @@ int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, d
                    uint32_t* vmax_bits, double* probe_f, double* probe_c) {
   Geo g = geo_of(h);
   int64_t nb = (h->n3c + KICK_CH - 1) / KICK_CH;
-  const float* Fc = (const float*)h->fc + (int64_t)d * h->n3c;
+  const char* Fc = (const char*)h->fc + (size_t)d * h->n3c * h->prec;
   int pr = prof_begin(h, PS_KICK);
-  k_kick<<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
-                                                   (const float*)h->fine.W, h->fine.N, Fc, a_mid, dt, c_old, c_new,
-                                                   h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+  if (h->prec == 4)
+    k_kick<float><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
+                                                            (const float*)h->fine.W, h->fine.N, (const float*)Fc, a_mid, dt, c_old,
+                                                            c_new, h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+  else
+    k_kick<double><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
+                                                             (const double*)h->fine.W, h->fine.N, (const double*)Fc, a_mid, dt,
+                                                             c_old, c_new, h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   return 0;

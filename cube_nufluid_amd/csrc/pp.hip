@@ -297,11 +297,30 @@ __device__ void pf_scan(int32_t* cnt, int n, int32_t* wsum) {
   __syncthreads();
 }
 
-// force on one target from the candidate runs (start[] exclusive offsets)
-template <typename EntT>
-__device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, int W, int pr, const PPent& ti, float mass,
-                                         float rsoft, long long fa[3], unsigned long long& npl) {
+// Pair-force accumulators: FP32 pairs sum in 2^-32 fixed point (exact and
+// order-independent); the FP64 oracle path sums doubles.
+template <typename RT>
+struct PPacc;
+template <>
+struct PPacc<float> {
+  long long a[3] = {0, 0, 0};
+  __device__ __forceinline__ void add(int e, float v) { a[e] += llrint((double)v * PP_FIX); }
+  __device__ __forceinline__ double get(int e) const { return (double)a[e] * (1.0 / PP_FIX); }
+};
+template <>
+struct PPacc<double> {
+  double a[3] = {0, 0, 0};
+  __device__ __forceinline__ void add(int e, double v) { a[e] += v; }
+  __device__ __forceinline__ double get(int e) const { return a[e]; }
+};
+
+// force on one target from the candidate runs (start[] exclusive offsets);
+// candidate coordinates are exact in FP32, so RT only sets the pair arithmetic
+template <typename RT, typename EntT>
+__device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, int W, int pr, const PPent& ti, float mass_f,
+                                         float rsoft_f, PPacc<RT>& acc, unsigned long long& npl) {
   const int wx = (int)floorf(ti.x) + pr, wy = (int)floorf(ti.y) + pr, wz = (int)floorf(ti.z) + pr;
+  const RT mass = (RT)mass_f, rsoft = (RT)rsoft_f, tx = (RT)ti.x, ty = (RT)ti.y, tz = (RT)ti.z;
   for (int kz = wz - pr; kz <= wz + pr; kz++)
     for (int ky = wy - pr; ky <= wy + pr; ky++) {
       const int row = (kz * W + ky) * W;
@@ -309,26 +328,27 @@ __device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, 
       npl += (unsigned long long)(j1 - j0);
       for (int j = j0; j < j1; j++) {
         const PPent pj = ent[j];
-        const float dx = pj.x - ti.x, dy = pj.y - ti.y, dz = pj.z - ti.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float r = sqrtf(r2);
+        const RT dx = (RT)pj.x - tx, dy = (RT)pj.y - ty, dz = (RT)pj.z - tz;
+        const RT r2 = dx * dx + dy * dy + dz * dz;
+        const RT r = sqrt(r2);
         if (!(r > rsoft)) continue;
-        const float rc = r * (1.0f / 16.0f);
-        const float rc3 = rc * rc * rc;
-        const float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
-        const float sc = mass * pc / (r2 * r);
-        fa[0] += llrint((double)(sc * dx) * PP_FIX);
-        fa[1] += llrint((double)(sc * dy) * PP_FIX);
-        fa[2] += llrint((double)(sc * dz) * PP_FIX);
+        const RT rc = r * ((RT)1 / (RT)16);
+        const RT rc3 = rc * rc * rc;
+        const RT pc = (RT)1 - (RT)1.75 * rc3 + (RT)0.75 * rc3 * rc * rc;
+        const RT sc = mass * pc / (r2 * r);
+        acc.add(0, sc * dx);
+        acc.add(1, sc * dy);
+        acc.add(2, sc * dz);
       }
     }
 }
 
 // kick (ext_pp_force.f90:114-119) from prefetched codes, or probe write;
 // returns |F|^2 in float
+template <typename RT>
 __device__ __forceinline__ float pf_apply(const Geo& g, const PPctx& c, void* vp, double* probe, int64_t p, const int32_t vc[3],
-                                          const long long fa[3]) {
-  const double f[3] = {(double)fa[0] * (1.0 / PP_FIX), (double)fa[1] * (1.0 / PP_FIX), (double)fa[2] * (1.0 / PP_FIX)};
+                                          const PPacc<RT>& acc) {
+  const double f[3] = {acc.get(0), acc.get(1), acc.get(2)};
   if (probe) {
     for (int e = 0; e < 3; e++) probe[3 * p + e] = f[e];
   } else {
@@ -381,7 +401,7 @@ __host__ __device__ constexpr int pf_wc_bytes(int W3) { return ((W3 + 2) * 4 + 1
 
 // LDS (dynamic): wc[W^3+2] int32 (wc[1+i]: count, then start, then end of
 // run i; wc[0] = 0), then CAP entries
-template <int PB>
+template <int PB, typename RT>
 __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const void* __restrict__ xp, void* vp,
                                                           const int64_t* __restrict__ cstart, uint32_t* f2max_bits,
                                                           unsigned long long* npairs_out, unsigned long long* nupd_out,
@@ -457,9 +477,9 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
     int32_t vc[3] = {0, 0, 0};
     if (!probe)
       for (int e = 0; e < 3; e++) vc[e] = load_v(vp, 3 * (int64_t)ti.p + e, g.izipv);
-    long long fa[3] = {0, 0, 0};
-    pf_force(ent, wc, W, pr, ti, c.mass, c.rsoft, fa, npl);
-    f2loc = fmaxf(f2loc, pf_apply(g, c, vp, probe, ti.p, vc, fa));
+    PPacc<RT> acc;
+    pf_force<RT>(ent, wc, W, pr, ti, c.mass, c.rsoft, acc, npl);
+    f2loc = fmaxf(f2loc, pf_apply<RT>(g, c, vp, probe, ti.p, vc, acc));
   }
   pf_reduce<NT>(f2loc, npl, ninner, redf, redn, redi, f2max_bits, npairs_out, nupd_out);
 }
@@ -546,7 +566,7 @@ __global__ __launch_bounds__(PD_NT) void k_pp_dense_prep(Geo g, int pr, const vo
 
 // Dense blocks, step 2: one work item (PD_NT targets of one block) per
 // workgroup iteration; candidates are read from the pool.
-template <int PB>
+template <int PB, typename RT>
 __global__ __launch_bounds__(PD_NT) void k_pp_dense_pairs(Geo g, PPctx c, void* vp, const PPent* __restrict__ pool,
                                                           const int64_t* __restrict__ info, const int64_t* __restrict__ items,
                                                           const uint32_t* __restrict__ nitems, uint32_t* f2max_bits,
@@ -599,16 +619,16 @@ __global__ __launch_bounds__(PD_NT) void k_pp_dense_pairs(Geo g, PPctx c, void* 
       int32_t vc[3] = {0, 0, 0};
       if (!probe)
         for (int q = 0; q < 3; q++) vc[q] = load_v(vp, 3 * (int64_t)ti.p + q, g.izipv);
-      long long fa[3] = {0, 0, 0};
-      pf_force(ent, st, W, pr, ti, c.mass, c.rsoft, fa, npl);
-      f2loc = pf_apply(g, c, vp, probe, ti.p, vc, fa);
+      PPacc<RT> acc;
+      pf_force<RT>(ent, st, W, pr, ti, c.mass, c.rsoft, acc, npl);
+      f2loc = pf_apply<RT>(g, c, vp, probe, ti.p, vc, acc);
       ninner = 1;
     }
     pf_reduce<PD_NT>(f2loc, npl, ninner, redf, redn, redi, f2max_bits, npairs_out, nupd_out);
   }
 }
 
-template <int PB>
+template <int PB, typename RT>
 static int pp_launch(cube_handle* h, const Geo& g, const PPctx& c, uint32_t* f2, unsigned long long* npr, unsigned long long* nup,
                      double* probe, int32_t* dense, uint32_t* ndense, int64_t* info, int64_t* items, uint32_t* nitems,
                      int32_t* fall, uint32_t* nfall, unsigned long long* pool_used) {
@@ -618,14 +638,14 @@ static int pp_launch(cube_handle* h, const Geo& g, const PPctx& c, uint32_t* f2,
   const size_t lds_fast = pf_wc_bytes(W3) + sizeof(PPent) * PFcfg<PB>::CAP;
   const size_t lds_tab = sizeof(int32_t) * (W3 + 1);
   const uint64_t pool_cap = h->fine.bytes_R() / sizeof(PPent);
-  HIP_TRY(h, hipFuncSetAttribute((const void*)k_pp_fast<PB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast));
-  k_pp_fast<PB><<<(unsigned)nb, PFcfg<PB>::NT, lds_fast, h->stream>>>(g, c, h->xp, h->vp, h->cstart, f2, npr, nup, probe, dense,
+  HIP_TRY(h, hipFuncSetAttribute((const void*)k_pp_fast<PB, RT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast));
+  k_pp_fast<PB, RT><<<(unsigned)nb, PFcfg<PB>::NT, lds_fast, h->stream>>>(g, c, h->xp, h->vp, h->cstart, f2, npr, nup, probe, dense,
                                                                       ndense);
   HIP_TRY(h, hipGetLastError());
   k_pp_dense_prep<PB><<<2048, PD_NT, lds_tab, h->stream>>>(g, c.pr, h->xp, h->cstart, dense, ndense, (PPent*)h->fine.W, pool_cap,
                                                             pool_used, info, items, nitems, fall, nfall);
   HIP_TRY(h, hipGetLastError());
-  k_pp_dense_pairs<PB><<<8192, PD_NT, lds_tab, h->stream>>>(g, c, h->vp, (const PPent*)h->fine.W, info, items, nitems, f2, npr, nup,
+  k_pp_dense_pairs<PB, RT><<<8192, PD_NT, lds_tab, h->stream>>>(g, c, h->vp, (const PPent*)h->fine.W, info, items, nitems, f2, npr, nup,
                                                              probe);
   HIP_TRY(h, hipGetLastError());
   return 0;
@@ -658,8 +678,11 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
   int32_t* fall = dense + nb2;
   PPctx c{h->mass_p, h->p.rsoft, a_mid, dt, c_old, h->vdec, h->venc, h->vkmax, prange};
   int pr = prof_begin(h, PS_PP);
-  int rc = PBsel == 4 ? pp_launch<4>(h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used)
-                      : pp_launch<2>(h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used);
+  int rc;
+#define PP_ARGS h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used
+  if (h->prec == 4) rc = PBsel == 4 ? pp_launch<4, float>(PP_ARGS) : pp_launch<2, float>(PP_ARGS);
+  else rc = PBsel == 4 ? pp_launch<4, double>(PP_ARGS) : pp_launch<2, double>(PP_ARGS);
+#undef PP_ARGS
   if (rc) return rc;
   uint32_t hv[12];
   HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 8, sizeof(hv), hipMemcpyDeviceToHost, h->stream));

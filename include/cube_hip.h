@@ -157,6 +157,15 @@ int cube_probe_pp_forces(cube_handle* h, double* f_pp);
    x fastest) -- the periodic field pm.f90:76-107 builds tile by tile. */
 int cube_probe_density(cube_handle* h, float* rho);
 
+/* Matter power spectrum of the current positions on the fine mesh -- the
+   estimator of utilities/cicpower.f90:85-257 with powerspectrum.f90:27-137
+   (CIC density, delta = rho/mean - 1, transform / nf^3, sinc^4 window
+   correction, linear bins nint(|k|) in grid units, Delta^2 = 4 pi k^3 P).
+   nbin must be nint(nf/2*sqrt(3)); for bins 1..nbin (array index b-1) it
+   writes the mode count, the sum of |k| and the sum of Delta^2 over modes.
+   Overwrites the fine mesh (recomputed by the next cube_pm). */
+int cube_power_spectrum(cube_handle* h, int nbin, double* modes, double* ksum, double* delta2sum);
+
 /* Seeded Zel'dovich initial conditions written straight into the resident
    state -- the role of the reference's IC tool (util/initial_conditions.f90:
    214-954), which writes CUBE's first checkpoint.  Noise: unit Gaussian per
