@@ -76,6 +76,23 @@ assert ctypes.sizeof(CubeParams) == 80
 
 # exported symbols and their signatures (include/cube_hip.h)
 _P = ctypes.c_void_p
+class CubeClock(ctypes.Structure):
+    _fields_ = [("a", c_f32), ("t", c_f32), ("tau", c_f32), ("dt", c_f32), ("dt_old", c_f32), ("da", c_f32),
+                ("a_mid", c_f32), ("dt_neu", c_f32), ("cur_checkpoint", c_i32), ("n_checkpoint", c_i32),
+                ("checkpoint_step", c_i32), ("final_step", c_i32), ("checkpoint_at", c_i32), ("pad", c_i32),
+                ("istep", c_i64), ("z_checkpoint", c_f32 * 64)]
+
+
+class EnsParams(ctypes.Structure):
+    _fields_ = [("device", c_i32), ("max_live", c_i32), ("mem_budget", c_i64)]
+
+
+class EnsResult(ctypes.Structure):
+    _fields_ = [("id", c_i64), ("status", c_i32), ("steps", c_i32), ("t_submit", c_f64), ("t_admit", c_f64),
+                ("t_done", c_f64), ("nparticles", c_i64), ("a_final", c_f32), ("pad", c_i32),
+                ("state_hash", ctypes.c_uint64), ("message", ctypes.c_char * 128)]
+
+
 SIGNATURES = {
     "cube_init": (c_i32, [ctypes.POINTER(CubeParams), _P, _P, ctypes.POINTER(_P)]),
     "cube_upload_state": (c_i32, [_P, ctypes.POINTER(SimHeader), _P, _P, _P, _P, _P]),
@@ -90,6 +107,16 @@ SIGNATURES = {
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
     "cube_power_spectrum": (c_i32, [_P, c_i32, _P, _P, _P]),
+    "cube_clock_init": (c_i32, [ctypes.POINTER(CubeClock), _P, c_i32]),
+    "cube_step": (c_i32, [_P, ctypes.POINTER(CubeClock)]),
+    "cube_footprint": (c_i64, [ctypes.POINTER(CubeParams), c_i32]),
+    "cube_ens_create": (c_i32, [ctypes.POINTER(EnsParams), ctypes.POINTER(_P)]),
+    "cube_ens_submit": (c_i32, [_P, ctypes.POINTER(CubeParams), _P, _P, ctypes.POINTER(ICParams), _P, c_i32,
+                                ctypes.POINTER(c_i64)]),
+    "cube_ens_poll": (c_i32, [_P, ctypes.POINTER(EnsResult), c_i32, ctypes.POINTER(c_i32)]),
+    "cube_ens_pending": (c_i32, [_P, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
+    "cube_ens_clock": (c_f64, [_P]),
+    "cube_ens_destroy": (c_i32, [_P]),
     "cube_ic_zeldovich": (c_i32, [_P, ctypes.POINTER(ICParams), ctypes.POINTER(SimHeader)]),
     "cube_profile_enable": (c_i32, [_P, c_i32]),
     "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),

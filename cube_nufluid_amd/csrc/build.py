@@ -27,6 +27,7 @@ SOURCES = {
     "fft.hip": [],
     "ic.hip": ["-ffp-contract=off"],
     "pk.hip": [],
+    "serve.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

@@ -76,10 +76,30 @@ static const CodecTables& codec_tables(int izipv) {
   });
   return tabs[izipv];
 }
+__global__ void k_vtab_fill(const float* __restrict__ dec, int n, double c, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)dec[i] / c;
+}
+const double* vtab_for(cube_handle* h, double c) {
+  const int n = h->p.izipv == 1 ? 256 : 65536;
+  // two slots, least recently used replaced: the kick needs the old- and
+  // new-sigma tables at once
+  for (int s = 0; s < 2; s++)
+    if (h->vdt_c[s] == c) {
+      h->vdt_next = s ^ 1;
+      return h->vdt + (size_t)s * n;
+    }
+  const int s = h->vdt_next;
+  h->vdt_next = s ^ 1;
+  k_vtab_fill<<<(n + 255) / 256, 256, 0, h->stream>>>(h->vdec, n, c, h->vdt + (size_t)s * n);
+  h->vdt_c[s] = c;
+  return h->vdt + (size_t)s * n;
+}
 int codec_build(cube_handle* h) {
   const CodecTables& t = codec_tables((int)h->p.izipv);
   HIP_TRY(h, hipMalloc(&h->vdec, sizeof(float) * t.dec.size()));
   HIP_TRY(h, hipMalloc(&h->venc, sizeof(double) * t.thr.size()));
+  HIP_TRY(h, hipMalloc(&h->vdt, sizeof(double) * 2 * t.dec.size()));
   HIP_TRY(h, hipMemcpy(h->vdec, t.dec.data(), sizeof(float) * t.dec.size(), hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(h->venc, t.thr.data(), sizeof(double) * t.thr.size(), hipMemcpyHostToDevice));
   h->vkmax = t.kmax;
@@ -222,7 +242,7 @@ extern "C" int cube_finalize(cube_handle* h) {
   }
   for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
   void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->order, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
-                  h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc, h->dbig,
+                  h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc, h->vdt, h->dbig,
                   h->dscratch_own ? h->dscratch : nullptr, h->pp_meta, h->pp_red};
   for (void* b : bufs)
     if (b) (void)hipFree(b);

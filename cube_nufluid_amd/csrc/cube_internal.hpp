@@ -60,6 +60,9 @@ struct cube_handle {
   int32_t* rhoc2 = nullptr; float* vfield2 = nullptr; int64_t* cstart2 = nullptr;
   // codec tables
   float* vdec = nullptr;      // tanf((pi*vp)/(nvbin-1)) for every code
+  double* vdt = nullptr;      // two decoded-velocity tables (vdec/c), see vtab_for
+  double vdt_c[2] = {0, 0};
+  int vdt_next = 0;
   double* venc = nullptr;     // encode thresholds (see codec.cpp)
   int32_t vkmax = 0;          // largest |code| produced by the encoder
   // meshes
@@ -176,18 +179,20 @@ __device__ __forceinline__ int32_t xsub(uint32_t u, int izipx) {  // fine sub-ce
 __device__ __forceinline__ double nint_d(double y) {
   return y >= 0 ? floor(y + 0.5) : -floor(-y + 0.5);
 }
-// velocity decode: tan table (exact host tanf) then the double divide
-__device__ __forceinline__ double vdecode(const float* tab, int32_t v, int izipv, double c) {
-  int32_t half = izipv == 1 ? 128 : 32768;
-  return (double)tab[v + half] / c;
+// velocity decode (update_particle.f90:69, pm.f90:145): a table per sigma
+// holding (double)tanf(pi*v/(nvbin-1)) / c for every code v, filled on the
+// device by the reference's own double division (vtab_for)
+__device__ __forceinline__ double vdecode(const double* dtab, int32_t v, int izipv) {
+  return dtab[v + (izipv == 1 ? 128 : 32768)];
 }
 // velocity encode with host-libm-exact thresholds.  thr[k + kmax - 1] is the
-// smallest x with enc(x) >= k, for k in [-kmax+1, kmax].
+// smallest x with enc(x) >= k, for k in [-kmax+1, kmax].  An FP32 atan gives
+// the code to within one step; the thresholds settle it exactly.
 __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int izipv, double c, double v) {
-  const double nv1 = izipv == 1 ? 255.0 : 65535.0;
-  double x = c * v;
-  double y = nv1 * atan(x) / (double)PI_F;
-  int32_t n = (int32_t)nint_d(y);
+  const float nv1 = izipv == 1 ? 255.0f : 65535.0f;
+  const double x = c * v;
+  const float y = nv1 * atanf((float)x) * (1.0f / PI_F);
+  int32_t n = (int32_t)rintf(y);
   if (n > kmax) n = kmax;
   if (n < -kmax) n = -kmax;
   for (int it = 0; it < 4; it++) {
@@ -199,6 +204,8 @@ __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int 
 }
 
 // ------------------------------------------------------------- launchers
+// decoded-velocity table for coefficient c (one of two cached slots)
+const double* vtab_for(cube_handle* h, double c);
 // scan.hip
 int scan_counts(cube_handle* h, const int32_t* cnt, int64_t n, int64_t* cstart);
 int reduce_doubles(cube_handle* h, const double* part, int64_t nparts, int nfields, double* host_out);

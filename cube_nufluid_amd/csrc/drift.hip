@@ -56,7 +56,7 @@ __device__ __forceinline__ WaveRun wave_run(int64_t key) {
 __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __restrict__ xp, const void* __restrict__ vp,
                                                        const int64_t* __restrict__ cstart, int64_t n3c,
                                                        const float* __restrict__ vfield, float dt_mid, double c_old,
-                                                       const float* __restrict__ vdec, int dmax, uint16_t* disp,
+                                                       const double* __restrict__ vdt, int dmax, uint16_t* disp,
                                                        int32_t* cnt_new, uint32_t* flags) {
   __shared__ int64_t cs[DR_CH + 1];
   __shared__ int smax[3];
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
         int il = (c[e] & (g.nt - 1)) + 1;  // tile-local 1-based source index
         if (g.lnt < 0) il = c[e] % g.nt + 1;
         double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
-        double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[r * 3 + e];
+        double vreal = vdecode(vdt, load_v(vp, 3 * p + e, g.izipv), g.izipv) + (double)vfield[r * 3 + e];
         double deltax = ((double)dt_mid * vreal) / (double)CUBE_NCELL;
         int ig = (int)ceil(((double)il - 1.0) + xo + deltax);
         int d = ig - il;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* _
                                                         const int64_t* __restrict__ cstart, const void* __restrict__ xp,
                                                         const void* __restrict__ vp, const int32_t* __restrict__ pid,
                                                         const float* __restrict__ vfield, float dt_mid, double c_old,
-                                                        const float* __restrict__ vdec, void* xp_o, int32_t* pid_o,
+                                                        const double* __restrict__ vdt, void* xp_o, int32_t* pid_o,
                                                         double* vr) {
   const double xscale = (g.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0) * (double)CUBE_NCELL;
   for (int64_t q = (int64_t)blockIdx.x * DR_NT + threadIdx.x; q < np; q += (int64_t)gridDim.x * DR_NT) {
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* _
     const int64_t rs = key_rank(key);
     const int64_t p = cstart[rs] + key_l(key);
     for (int e = 0; e < 3; e++) {
-      const double vreal = vdecode(vdec, load_v(vp, 3 * p + e, g.izipv), g.izipv, c_old) + (double)vfield[rs * 3 + e];
+      const double vreal = vdecode(vdt, load_v(vp, 3 * p + e, g.izipv), g.izipv) + (double)vfield[rs * 3 + e];
       int64_t xn = (int64_t)load_xu(xp, 3 * p + e, g.izipx);
       xn = g.izipx == 1 ? (int64_t)(int8_t)xn : (int64_t)(int16_t)xn;
       xn += (int64_t)nint_d((double)dt_mid * vreal / xscale);
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* _
 // (:119) and the sigma_v sums (:174-200).  part: 3 doubles per workgroup.
 __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __restrict__ cs_new, int64_t n3c,
                                                        const float* __restrict__ vfield, const double* __restrict__ vr,
-                                                       double c_old, const float* __restrict__ vdec,
+                                                       double c_old, const double* __restrict__ vdt,
                                                        const double* __restrict__ venc, int32_t kmax, void* vp_o,
                                                        float* vfield_o, int32_t* rhoc_o, double* part) {
   __shared__ int64_t scs[DR_NT + 1];
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
       const float vfn = svf[lo * 3 + e];
       const int32_t code = vencode(venc, kmax, g.izipv, c_old, vr[3 * q + e] - (double)vfn);
       store_v(vp_o, 3 * q + e, g.izipv, code);
-      v[e] = vdecode(vdec, code, g.izipv, c_old);
+      v[e] = vdecode(vdt, code, g.izipv);
     }
     sum_r += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     for (int e = 0; e < 3; e++) v[e] += (double)svf[lo * 3 + e];
@@ -437,13 +437,14 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   const float dt_mid = (dt_old + dt) / 2;
   const int nlayer = 2 * (int)ceilf(dt_mid * h->vz_max / (float)CUBE_NCELL) + 1;
   const double c_old = vcoef(h->sigma_vi);
+  const double* vdt = vtab_for(h, c_old);
   const int dmax = (int)(h->ncb < DMAX ? h->ncb : DMAX);
   uint32_t* flags = h->uflags;  // [0..2] max disp, [3] too far / crowded, [4] frame conflicts, [6] crowded cells
   HIP_TRY(h, hipMemsetAsync(flags, 0, sizeof(uint32_t) * 8, h->stream));
   HIP_TRY(h, hipMemsetAsync(h->rhoc2, 0, sizeof(int32_t) * h->n3c, h->stream));
   int64_t nb1 = (h->n3c + DR_CH - 1) / DR_CH;
   int pid1 = prof_begin(h, PS_DRIFT1);
-  k_drift_pass1<<<(unsigned)nb1, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, dt_mid, c_old, h->vdec, dmax,
+  k_drift_pass1<<<(unsigned)nb1, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, dt_mid, c_old, vdt, dmax,
                                                          h->disp, h->rhoc2, flags);
   prof_end(h, pid1);
   HIP_TRY(h, hipGetLastError());
@@ -476,10 +477,10 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   int pid5 = prof_begin(h, PS_DRIFT_WRITE);
   const int64_t ngather = std::min<int64_t>((h->np + DR_NT - 1) / DR_NT, 1 << 20);
   k_drift_gather<<<(unsigned)std::max<int64_t>(ngather, 1), DR_NT, 0, h->stream>>>(
-      g, keys, h->np, h->cstart, h->xp, h->vp, h->pid, h->vfield, dt_mid, c_old, h->vdec, h->xp2, h->pid2,
+      g, keys, h->np, h->cstart, h->xp, h->vp, h->pid, h->vfield, dt_mid, c_old, vdt, h->xp2, h->pid2,
       (double*)h->dscratch);
   k_drift_cells<<<(unsigned)nb3, DR_NT, 0, h->stream>>>(g, h->cstart2, h->n3c, h->vfield, (const double*)h->dscratch, c_old,
-                                                         h->vdec, h->venc, h->vkmax, h->vp2, h->vfield2, h->rhoc2, h->dred);
+                                                         vdt, h->venc, h->vkmax, h->vp2, h->vfield2, h->rhoc2, h->dred);
   prof_end(h, pid5);
   HIP_TRY(h, hipGetLastError());
   double sums[3];

@@ -151,10 +151,20 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_r2c(cpx<T>* data, const cpx<T>
   constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
   __shared__ cpx<T> s[H * LD];
   const int64_t r0 = (int64_t)blockIdx.x * W;
-  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
-    int w = idx / H, m = idx % H;
-    int64_t row = r0 + w;
-    s[m * LD + w] = row < nrows ? data[row * (H + 1) + m] : cpx<T>{0, 0};
+  {
+    constexpr int PER = (H * W + NT_FFT - 1) / NT_FFT;
+    cpx<T> r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = threadIdx.x + k * NT_FFT, w = idx / H, m = idx % H;
+      const int64_t row = r0 + w;
+      r[k] = (idx < H * W && row < nrows) ? data[row * (H + 1) + m] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = threadIdx.x + k * NT_FFT, w = idx / H, m = idx % H;
+      if (idx < H * W) s[m * LD + w] = r[k];
+    }
   }
   __syncthreads();
   lds_fft<T, H, W, false, LD>(s, tw, 2);
@@ -248,6 +258,24 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst,
   }
 }
 
+// Load W columns of N complex values at stride `stride` into s[n*W + w]
+// (columns w >= wvalid read as zero).  All loads of a thread are issued
+// before any LDS store so that they are in flight together.
+template <typename T, int N, int W>
+__device__ __forceinline__ void col_load(cpx<T>* s, const cpx<T>* __restrict__ src, int64_t stride, int wvalid) {
+  constexpr int TOT = N * W, PER = TOT / NT_FFT;
+  static_assert(TOT % NT_FFT == 0, "whole loads per thread");
+  cpx<T> r[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int idx = threadIdx.x + k * NT_FFT, n = idx / W, w = idx % W;
+    r[k] = w < wvalid ? src[n * stride + w] : cpx<T>{0, 0};
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) s[threadIdx.x + k * NT_FFT] = r[k];
+  __syncthreads();
+}
+
 // ----------------------------------------------------------- strided Y / Z
 // AXIS 1: columns along y (stride H+1) in plane by = z.
 // AXIS 2: columns along z (stride N(H+1)) at by = y; (bx, by) is the XCD-remapped block.
@@ -260,11 +288,7 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_strided(cpx<T>* data, const cpx<
   const int kx0 = bx * W;
   const int64_t stride = AXIS == 1 ? (int64_t)KX : (int64_t)N * KX;
   const int64_t base = AXIS == 1 ? (int64_t)by * N * KX + kx0 : (int64_t)by * KX + kx0;
-  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
-    int n = idx / W, w = idx % W;
-    s[idx] = (kx0 + w < KX) ? data[base + n * stride + w] : cpx<T>{0, 0};
-  }
-  __syncthreads();
+  col_load<T, N, W>(s, data + base, stride, KX - kx0);
   lds_fft<T, N, W, INV>(s, tw, 1);
   for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
     int n = idx / W, w = idx % W;
@@ -288,11 +312,7 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
   const int ky = by;
   const int64_t stride = (int64_t)N * KX;
   const int64_t base = (int64_t)ky * KX + kx0;
-  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
-    int n = idx / W, w = idx % W;
-    s[idx] = (kx0 + w < KX) ? src[base + n * stride + w] : cpx<T>{0, 0};
-  }
-  __syncthreads();
+  col_load<T, N, W>(s, src + base, stride, KX - kx0);
   lds_fft<T, N, W, false>(s, tw, 1);
   const int kyp = ky <= N / 2 ? ky : N - ky;
   const T sy = (d == 1 && ky > N / 2) ? (T)-1 : (T)1;

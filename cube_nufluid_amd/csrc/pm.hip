@@ -218,15 +218,19 @@ int cic_coarse(cube_handle* h, double* mass) {
 // sigma_new; pm.f90:145-154) followed by component d of the coarse kick
 // (decode/encode with sigma_new; pm.f90:353-363).  Components are
 // independent in both kicks, so running them per component reproduces the
-// reference's sequence exactly.  With probe arrays set, the kick is
-// replaced by writing the interpolated forces.
+// reference's sequence.  Each tap adds (double)(F*k*w1*w2*w3) with
+// k = a_mid*dt/6/pi formed once in the mesh precision (the reference forms
+// F*a_mid*dt/6/pi per tap; the two agree to rounding).  With probe arrays
+// set, the kick is replaced by writing the interpolated forces.  Mesh
+// sizes are powers of two, so tap indices wrap with masks.
 template <typename T>
 __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                                   int64_t n3c, const float* __restrict__ vfield,
                                                   const T* __restrict__ Ff, int64_t Nf,
                                                   const T* __restrict__ Fc,
                                                   float a_mid, float dt, double c_old, double c_new,
-                                                  const float* __restrict__ vdec, const double* __restrict__ venc, int32_t kmax,
+                                                  const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
+                                                  const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
   __shared__ int64_t cs[KICK_CH + 1];
   __shared__ float red[KICK_NT / 64];
@@ -237,7 +241,9 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __re
   }
   __syncthreads();
   const int64_t p0 = cs[0], p1 = cs[KICK_CH];
-  const int64_t Nc = g.nc;
+  const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
+  const int64_t P = Nf + 2;  // fine rows are padded to Nf+2 reals
+  const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax = 0.f;
   for (int64_t p = p0 + threadIdx.x; p < p1; p += KICK_NT) {
     int lo = 0, hi = KICK_CH;
@@ -245,78 +251,60 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __re
       int mid = (lo + hi) >> 1;
       if (cs[mid] <= p) lo = mid; else hi = mid;
     }
-    int64_t r = r0 + lo;
+    const int64_t r = r0 + lo;
     int32_t c[3];
     cell_of_rank(g, r, c);
     T dx1[3], dx2[3], cx1[3], cx2[3];
-    int64_t f1[3], f2[3], k1[3], k2[3];
+    uint32_t f1[3], f2[3], k1[3], k2[3];
     for (int e = 0; e < 3; e++) {
-      int torg = (c[e] / g.nt) * g.nt;
-      int il = c[e] - torg + 1;
-      double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
-      T tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xo);
-      int a1 = (int)floor(tx) + 1;
+      const int il = (g.lnt >= 0 ? (c[e] & (g.nt - 1)) : c[e] % g.nt) + 1;
+      const int torg = c[e] - (il - 1);
+      const double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
+      const T tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xo);
+      const int a1 = (int)floor(tx) + 1;
       dx1[e] = (T)a1 - tx;
       dx2[e] = 1 - dx1[e];
-      // a tile-local float coordinate may round up to 4*nt; wrap both taps
-      f1[e] = ((int64_t)CUBE_NCELL * torg + a1 - 1) % Nf;
-      f2[e] = (f1[e] + 1) % Nf;
-      T tc = (T)(((double)(torg + il - 1) + xo) - 0.5);
-      int b1 = (int)floor(tc) + 1;  // 1-based global node, 0..nc (0 and nc+1 are halo)
+      // a tile-local float coordinate may round up to 4*nt; the mask wraps it
+      f1[e] = (uint32_t)(CUBE_NCELL * torg + a1 - 1) & mf;
+      f2[e] = (f1[e] + 1) & mf;
+      const T tc = (T)(((double)(torg + il - 1) + xo) - 0.5);
+      const int b1 = (int)floor(tc) + 1;  // 1-based global node, 0..nc (0 and nc+1 are halo)
       cx1[e] = (T)b1 - tc;
       cx2[e] = 1 - cx1[e];
-      k1[e] = b1 - 1 < 0 ? Nc - 1 : b1 - 1;
-      k2[e] = b1 >= Nc ? b1 - Nc : b1;
+      k1[e] = (uint32_t)(b1 - 1) & mc;
+      k2[e] = (uint32_t)b1 & mc;
     }
-    const int64_t P = Nf + 2;
-#define FFI(x, y, z) Ff[((z) * Nf + (y)) * P + (x)]
-#define FCI(x, y, z) Fc[((z) * Nc + (y)) * Nc + (x)]
+    // the 8 taps: bit 0 picks x, bit 1 y, bit 2 z (order of pm.f90:146-153)
+    const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
+    T Fv[8], Fw[8], Cv[8], Cw[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+      const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
+      Fv[q] = Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])];
+      Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
+      const uint32_t rowc = (oz ? k2[2] : k1[2]) * Nc + (oy ? k2[1] : k1[1]);
+      Cv[q] = Fc[(int64_t)rowc * Nc + (ox ? k2[0] : k1[0])];
+      Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
+    }
     if (probe_f) {
-      double s = 0;
-      s += (double)(FFI(f1[0], f1[1], f1[2]) * dx1[0] * dx1[1] * dx1[2]);
-      s += (double)(FFI(f2[0], f1[1], f1[2]) * dx2[0] * dx1[1] * dx1[2]);
-      s += (double)(FFI(f1[0], f2[1], f1[2]) * dx1[0] * dx2[1] * dx1[2]);
-      s += (double)(FFI(f1[0], f1[1], f2[2]) * dx1[0] * dx1[1] * dx2[2]);
-      s += (double)(FFI(f1[0], f2[1], f2[2]) * dx1[0] * dx2[1] * dx2[2]);
-      s += (double)(FFI(f2[0], f1[1], f2[2]) * dx2[0] * dx1[1] * dx2[2]);
-      s += (double)(FFI(f2[0], f2[1], f1[2]) * dx2[0] * dx2[1] * dx1[2]);
-      s += (double)(FFI(f2[0], f2[1], f2[2]) * dx2[0] * dx2[1] * dx2[2]);
-      probe_f[3 * p + d] = s;
-      s = 0;
-      s += (double)(FCI(k1[0], k1[1], k1[2]) * cx1[0] * cx1[1] * cx1[2]);
-      s += (double)(FCI(k2[0], k1[1], k1[2]) * cx2[0] * cx1[1] * cx1[2]);
-      s += (double)(FCI(k1[0], k2[1], k1[2]) * cx1[0] * cx2[1] * cx1[2]);
-      s += (double)(FCI(k1[0], k1[1], k2[2]) * cx1[0] * cx1[1] * cx2[2]);
-      s += (double)(FCI(k1[0], k2[1], k2[2]) * cx1[0] * cx2[1] * cx2[2]);
-      s += (double)(FCI(k2[0], k1[1], k2[2]) * cx2[0] * cx1[1] * cx2[2]);
-      s += (double)(FCI(k2[0], k2[1], k1[2]) * cx2[0] * cx2[1] * cx1[2]);
-      s += (double)(FCI(k2[0], k2[1], k2[2]) * cx2[0] * cx2[1] * cx2[2]);
-      probe_c[3 * p + d] = s;
+      double sf = 0, sc = 0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        sf += (double)(Fv[q] * Fw[q]);
+        sc += (double)(Cv[q] * Cw[q]);
+      }
+      probe_f[3 * p + d] = sf;
+      probe_c[3 * p + d] = sc;
       continue;
     }
-    // FP32 meshes: the reference's float expression F*a_mid*dt/6/pi*w
-    const T A = (T)a_mid, DT = (T)dt, SIX = (T)6, PI = (T)PI_F;
-    double v = vdecode(vdec, load_v(vp, 3 * p + d, g.izipv), g.izipv, c_old);
-    v = v + (double)(FFI(f1[0], f1[1], f1[2]) * A * DT / SIX / PI * dx1[0] * dx1[1] * dx1[2]);
-    v = v + (double)(FFI(f2[0], f1[1], f1[2]) * A * DT / SIX / PI * dx2[0] * dx1[1] * dx1[2]);
-    v = v + (double)(FFI(f1[0], f2[1], f1[2]) * A * DT / SIX / PI * dx1[0] * dx2[1] * dx1[2]);
-    v = v + (double)(FFI(f1[0], f1[1], f2[2]) * A * DT / SIX / PI * dx1[0] * dx1[1] * dx2[2]);
-    v = v + (double)(FFI(f1[0], f2[1], f2[2]) * A * DT / SIX / PI * dx1[0] * dx2[1] * dx2[2]);
-    v = v + (double)(FFI(f2[0], f1[1], f2[2]) * A * DT / SIX / PI * dx2[0] * dx1[1] * dx2[2]);
-    v = v + (double)(FFI(f2[0], f2[1], f1[2]) * A * DT / SIX / PI * dx2[0] * dx2[1] * dx1[2]);
-    v = v + (double)(FFI(f2[0], f2[1], f2[2]) * A * DT / SIX / PI * dx2[0] * dx2[1] * dx2[2]);
-    int32_t q = vencode(venc, kmax, g.izipv, c_new, v);
-    v = vdecode(vdec, q, g.izipv, c_new);
-    v = v + (double)(FCI(k1[0], k1[1], k1[2]) * A * DT / SIX / PI * cx1[0] * cx1[1] * cx1[2]);
-    v = v + (double)(FCI(k2[0], k1[1], k1[2]) * A * DT / SIX / PI * cx2[0] * cx1[1] * cx1[2]);
-    v = v + (double)(FCI(k1[0], k2[1], k1[2]) * A * DT / SIX / PI * cx1[0] * cx2[1] * cx1[2]);
-    v = v + (double)(FCI(k1[0], k1[1], k2[2]) * A * DT / SIX / PI * cx1[0] * cx1[1] * cx2[2]);
-    v = v + (double)(FCI(k1[0], k2[1], k2[2]) * A * DT / SIX / PI * cx1[0] * cx2[1] * cx2[2]);
-    v = v + (double)(FCI(k2[0], k1[1], k2[2]) * A * DT / SIX / PI * cx2[0] * cx1[1] * cx2[2]);
-    v = v + (double)(FCI(k2[0], k2[1], k1[2]) * A * DT / SIX / PI * cx2[0] * cx2[1] * cx1[2]);
-    v = v + (double)(FCI(k2[0], k2[1], k2[2]) * A * DT / SIX / PI * cx2[0] * cx2[1] * cx2[2]);
-#undef FFI
-#undef FCI
+    double v = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
+#pragma unroll
+    for (int q = 0; q < 8; q++) v = v + (double)(Fv[q] * kf * Fw[q]);
+    const int32_t qn = vencode(venc, kmax, g.izipv, c_new, v);
+    v = vdecode(vdt_new, qn, g.izipv);
+#pragma unroll
+    for (int q = 0; q < 8; q++) v = v + (double)(Cv[q] * kf * Cw[q]);
     float av = (float)fabs(v + (double)vfield[r * 3 + d]);
     vmax = fmaxf(vmax, av);
     store_v(vp, 3 * p + d, g.izipv, vencode(venc, kmax, g.izipv, c_new, v));
@@ -338,15 +326,17 @@ int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, d
   Geo g = geo_of(h);
   int64_t nb = (h->n3c + KICK_CH - 1) / KICK_CH;
   const char* Fc = (const char*)h->fc + (size_t)d * h->n3c * h->prec;
+  const double* vo = vtab_for(h, c_old);
+  const double* vn = vtab_for(h, c_new);
   int pr = prof_begin(h, PS_KICK);
   if (h->prec == 4)
     k_kick<float><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
                                                             (const float*)h->fine.W, h->fine.N, (const float*)Fc, a_mid, dt, c_old,
-                                                            c_new, h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+                                                            c_new, vo, vn, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
   else
     k_kick<double><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
                                                              (const double*)h->fine.W, h->fine.N, (const double*)Fc, a_mid, dt,
-                                                             c_old, c_new, h->vdec, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+                                                             c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   return 0;

@@ -10,13 +10,14 @@
 // pair set is identical.  Positions relative to the block are exact in
 // FP32 (they carry at most 19 significant bits), the pair force is FP32 and
 // the per-particle sum is a 64-bit fixed-point integer in 2^-32 units, so it
-// does not depend on the order in which candidates landed in LDS and the
-// result is the same on every run.  The kick and re-quantisation with the
+// does not depend on the order in which candidates landed in LDS.  This
+// chunked kernel handles only blocks the faster kernels below cannot hold.  The kick and re-quantisation with the
 // current sigma (ext_pp_force.f90:90, 114-119) are applied in place.
 #include "cube_internal.hpp"
 #include <string.h>
 #include <algorithm>
 #include <vector>
+#include <type_traits>
 
 #define PP_NT 64
 #define PP_BP 2
@@ -31,7 +32,7 @@ __device__ __forceinline__ int32_t pack_fc(int a, int b, int c) {  // each in [-
 
 __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                               float mass, float rsoft, int pp_range, float a_mid, float dt, double c_old,
-                                              const float* __restrict__ vdec, const double* __restrict__ venc, int32_t kmax,
+                                              const double* __restrict__ vdt, const double* __restrict__ venc, int32_t kmax,
                                               uint32_t* f2max_bits, unsigned long long* npairs_out,
                                               unsigned long long* nupd_out, double* probe, const int32_t* blist, int bscale) {
   constexpr int SPAN = PP_BP + 2, NCELL = SPAN * SPAN * SPAN, FB = 4 * PP_BP;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
         for (int e = 0; e < 3; e++) probe[3 * pi + e] = f[e];
       } else {
         for (int e = 0; e < 3; e++) {
-          double v = vdecode(vdec, load_v(vp, 3 * pi + e, g.izipv), g.izipv, c_old);
+          double v = vdecode(vdt, load_v(vp, 3 * pi + e, g.izipv), g.izipv);
           v = v + f[e] * (double)a_mid * (double)dt / 6.0 / (double)PI_F;
           store_v(vp, 3 * pi + e, g.izipv, vencode(venc, kmax, g.izipv, c_old, v));
         }
@@ -189,17 +190,20 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
 //               between drift and PM) and cuts their targets into work items
 //               of PD_NT, which a pairs kernel walks grid-stride.  A block the
 //               pool cannot hold goes to the chunked kernel above.
-// Pair sums are fixed point, so the result does not depend on the order of
-// candidates inside a fine cell or on the path that handled the block.
+// Every sum has a fixed value whatever the thread timing: the fast path
+// sorts each fine cell's candidates by particle index and sums in double in
+// that order; the dense path sums in 2^-32 fixed point.  Pair forces are
+// FP32 with exact FP32 relative positions (FP64 in the oracle precision).
 // PB = 4 serves pp_range <= 2, PB = 2 pp_range 3..4 (window <= 20 cells).
 #define PD_NT 256
 #define PP_SLOTS 1024
+#define PF_OWN 4096  // region particles covered by the LDS owner table
 struct PPent { float x, y, z; int32_t p; };
 
 struct PPctx {  // per-call constants
   float mass, rsoft, a_mid, dt;
   double c_old;
-  const float* vdec;
+  const double* vdt;  // decoded velocities for c_old
   const double* venc;
   int32_t kmax, pr;
 };
@@ -255,24 +259,33 @@ __device__ __forceinline__ void pf_region(const Geo& g, const int64_t* cstart, i
 
 // window cell of region particle t (false if outside the window) and its
 // block-local fine coordinates (exact in FP32: at most 20 significant bits)
+// region cell of region particle t: from the owner table when given, else
+// by binary search of the offsets
 template <int PB>
 __device__ __forceinline__ bool pf_cand(const Geo& g, const void* xp, const int64_t* beg, const int32_t* off, int32_t t, int pr,
-                                        int W, int& wi, PPent& e) {
+                                        int W, int& wi, PPent& e, const uint8_t* owner = nullptr) {
   using G = PBgeo<PB>;
-  int a = 0, b = G::NCELL;
-  while (b - a > 1) {
-    const int m = (a + b) >> 1;
-    if (off[m] <= t) a = m; else b = m;
+  int a = 0;
+  if (owner) {
+    a = owner[t];
+  } else {
+    int b = G::NCELL;
+    while (b - a > 1) {
+      const int m = (a + b) >> 1;
+      if (off[m] <= t) a = m; else b = m;
+    }
   }
   const int64_t p = beg[a] + (t - off[a]);
   const int rel[3] = {a % G::SPAN - 1, (a / G::SPAN) % G::SPAN - 1, a / (G::SPAN * G::SPAN) - 1};
   int w[3];
   float x[3];
   bool keep = true;
+  // 4*(u+0.5)/2^(8 izipx) = (2u+1)*2^(1-8 izipx): exact in FP32, as is the sum
+  const float res = g.izipx == 1 ? 0.0078125f : 3.0517578125e-05f;
   for (int d = 0; d < 3; d++) {
     const uint32_t u = load_xu(xp, 3 * p + d, g.izipx);
     w[d] = CUBE_NCELL * rel[d] + xsub(u, g.izipx) + pr;
-    x[d] = (float)((double)(CUBE_NCELL * rel[d]) + (double)CUBE_NCELL * xoff(u, g.izipx));
+    x[d] = (float)(CUBE_NCELL * rel[d]) + (float)(2 * u + 1) * res;
     keep = keep && w[d] >= 0 && w[d] < W;
   }
   wi = (w[2] * W + w[1]) * W + w[0];
@@ -307,6 +320,13 @@ struct PPacc<float> {
   __device__ __forceinline__ void add(int e, float v) { a[e] += llrint((double)v * PP_FIX); }
   __device__ __forceinline__ double get(int e) const { return (double)a[e] * (1.0 / PP_FIX); }
 };
+// FP32 pairs summed in double in a fixed candidate order (runs sorted by
+// particle index), which is as reproducible as the fixed-point sum
+struct PPaccOrd {
+  double a[3] = {0, 0, 0};
+  __device__ __forceinline__ void add(int e, float v) { a[e] += (double)v; }
+  __device__ __forceinline__ double get(int e) const { return a[e]; }
+};
 template <>
 struct PPacc<double> {
   double a[3] = {0, 0, 0};
@@ -314,46 +334,78 @@ struct PPacc<double> {
   __device__ __forceinline__ double get(int e) const { return a[e]; }
 };
 
+// pair term of ext_pp_force.f90:99-108 for r > rsoft: m pc(r/16) dx/r^3
+__device__ __forceinline__ bool pf_pair(float dx, float dy, float dz, float mass, float rsoft2, float& sc) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  if (!(r2 > rsoft2)) return false;
+  const float ri = rsqrtf(r2);
+  const float r = r2 * ri;
+  const float rc = r * (1.0f / 16.0f);
+  const float rc3 = rc * rc * rc;
+  const float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
+  sc = mass * pc * (ri * ri * ri);
+  return true;
+}
+__device__ __forceinline__ bool pf_pair(double dx, double dy, double dz, double mass, double rsoft2, double& sc) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  if (!(r2 > rsoft2)) return false;
+  const double r = sqrt(r2);
+  const double rc = r * (1.0 / 16.0);
+  const double rc3 = rc * rc * rc;
+  const double pc = 1.0 - 1.75 * rc3 + 0.75 * rc3 * rc * rc;
+  sc = mass * pc / (r2 * r);
+  return true;
+}
+
 // force on one target from the candidate runs (start[] exclusive offsets);
-// candidate coordinates are exact in FP32, so RT only sets the pair arithmetic
-template <typename RT, typename EntT>
+// candidate coordinates are exact in FP32, so RT only sets the pair
+// arithmetic.  ROWSUM: candidates are in a fixed order, so each x-run is
+// summed in RT and added to the accumulator once per run.
+template <typename RT, bool ROWSUM, typename Acc, typename EntT>
 __device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, int W, int pr, const PPent& ti, float mass_f,
-                                         float rsoft_f, PPacc<RT>& acc, unsigned long long& npl) {
+                                         float rsoft_f, Acc& acc, unsigned long long& npl) {
   const int wx = (int)floorf(ti.x) + pr, wy = (int)floorf(ti.y) + pr, wz = (int)floorf(ti.z) + pr;
-  const RT mass = (RT)mass_f, rsoft = (RT)rsoft_f, tx = (RT)ti.x, ty = (RT)ti.y, tz = (RT)ti.z;
+  const RT mass = (RT)mass_f, rsoft2 = (RT)rsoft_f * (RT)rsoft_f, tx = (RT)ti.x, ty = (RT)ti.y, tz = (RT)ti.z;
   for (int kz = wz - pr; kz <= wz + pr; kz++)
     for (int ky = wy - pr; ky <= wy + pr; ky++) {
       const int row = (kz * W + ky) * W;
       const int j0 = start[row + wx - pr], j1 = start[row + wx + pr + 1];
       npl += (unsigned long long)(j1 - j0);
+      RT sx = 0, sy = 0, sz = 0;
       for (int j = j0; j < j1; j++) {
         const PPent pj = ent[j];
         const RT dx = (RT)pj.x - tx, dy = (RT)pj.y - ty, dz = (RT)pj.z - tz;
-        const RT r2 = dx * dx + dy * dy + dz * dz;
-        const RT r = sqrt(r2);
-        if (!(r > rsoft)) continue;
-        const RT rc = r * ((RT)1 / (RT)16);
-        const RT rc3 = rc * rc * rc;
-        const RT pc = (RT)1 - (RT)1.75 * rc3 + (RT)0.75 * rc3 * rc * rc;
-        const RT sc = mass * pc / (r2 * r);
-        acc.add(0, sc * dx);
-        acc.add(1, sc * dy);
-        acc.add(2, sc * dz);
+        RT sc;
+        if (!pf_pair(dx, dy, dz, mass, rsoft2, sc)) continue;
+        if (ROWSUM) {
+          sx += sc * dx;
+          sy += sc * dy;
+          sz += sc * dz;
+        } else {
+          acc.add(0, sc * dx);
+          acc.add(1, sc * dy);
+          acc.add(2, sc * dz);
+        }
+      }
+      if (ROWSUM && j1 > j0) {
+        acc.add(0, sx);
+        acc.add(1, sy);
+        acc.add(2, sz);
       }
     }
 }
 
 // kick (ext_pp_force.f90:114-119) from prefetched codes, or probe write;
 // returns |F|^2 in float
-template <typename RT>
+template <typename Acc>
 __device__ __forceinline__ float pf_apply(const Geo& g, const PPctx& c, void* vp, double* probe, int64_t p, const int32_t vc[3],
-                                          const PPacc<RT>& acc) {
+                                          const Acc& acc) {
   const double f[3] = {acc.get(0), acc.get(1), acc.get(2)};
   if (probe) {
     for (int e = 0; e < 3; e++) probe[3 * p + e] = f[e];
   } else {
     for (int e = 0; e < 3; e++) {
-      double v = vdecode(c.vdec, vc[e], g.izipv, c.c_old);
+      double v = vdecode(c.vdt, vc[e], g.izipv);
       v = v + f[e] * (double)c.a_mid * (double)c.dt / 6.0 / (double)PI_F;
       store_v(vp, 3 * p + e, g.izipv, vencode(c.venc, c.kmax, g.izipv, c.c_old, v));
     }
@@ -414,6 +466,7 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
   __shared__ int32_t ncand_s, wsum[NT / 64], redi[NT / 64];
   __shared__ float redf[NT / 64];
   __shared__ unsigned long long redn[NT / 64];
+  __shared__ uint8_t owner[PF_OWN];
   int c0x, c0y, c0z;
   pf_origin<PB>(g, blockIdx.x, c0x, c0y, c0z);
   const int pr = c.pr, W = G::FB + 2 * pr, W3 = W * W * W;
@@ -424,6 +477,13 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
   pf_region<PB, NT>(g, cstart, c0x, c0y, c0z, beg, off, wsum);
   __syncthreads();
   const int32_t ntot = off[G::NCELL];
+  // owner table: region cell of each of the first PF_OWN region particles
+  static_assert(G::NCELL <= 256, "owner entries are bytes");
+  const bool use_owner = ntot <= PF_OWN;
+  if (use_owner && (int)threadIdx.x < G::NCELL)
+    for (int32_t t = off[threadIdx.x]; t < off[threadIdx.x + 1]; t++) owner[t] = (uint8_t)threadIdx.x;
+  __syncthreads();
+  const uint8_t* own = use_owner ? owner : nullptr;
   // pass 1: the first K*NT region particles are kept in registers
   PPent re[K];
   int rw[K];
@@ -432,7 +492,7 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int32_t t = threadIdx.x + k * NT;
-    rk[k] = t < ntot && pf_cand<PB>(g, xp, beg, off, t, pr, W, rw[k], re[k]);
+    rk[k] = t < ntot && pf_cand<PB>(g, xp, beg, off, t, pr, W, rw[k], re[k], own);
   }
 #pragma unroll
   for (int k = 0; k < K; k++)
@@ -466,6 +526,17 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
     if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) ent[atomicAdd(&wc[1 + wi], 1)] = e;
   }
   __syncthreads();
+  // runs in particle-index order: the pair sums below then have a fixed order
+  for (int i = threadIdx.x; i < W3; i += NT) {
+    const int a = wc[i], b = wc[i + 1];
+    for (int m = a + 1; m < b; m++) {
+      const PPent v = ent[m];
+      int j = m - 1;
+      while (j >= a && ent[j].p > v.p) { ent[j + 1] = ent[j]; j--; }
+      ent[j + 1] = v;
+    }
+  }
+  __syncthreads();
   float f2loc = 0.f;
   unsigned long long npl = 0;
   int32_t ninner = 0;
@@ -477,9 +548,9 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
     int32_t vc[3] = {0, 0, 0};
     if (!probe)
       for (int e = 0; e < 3; e++) vc[e] = load_v(vp, 3 * (int64_t)ti.p + e, g.izipv);
-    PPacc<RT> acc;
-    pf_force<RT>(ent, wc, W, pr, ti, c.mass, c.rsoft, acc, npl);
-    f2loc = fmaxf(f2loc, pf_apply<RT>(g, c, vp, probe, ti.p, vc, acc));
+    typename std::conditional<sizeof(RT) == 4, PPaccOrd, PPacc<double>>::type acc;
+    pf_force<RT, true>(ent, wc, W, pr, ti, c.mass, c.rsoft, acc, npl);
+    f2loc = fmaxf(f2loc, pf_apply(g, c, vp, probe, ti.p, vc, acc));
   }
   pf_reduce<NT>(f2loc, npl, ninner, redf, redn, redi, f2max_bits, npairs_out, nupd_out);
 }
@@ -620,8 +691,8 @@ __global__ __launch_bounds__(PD_NT) void k_pp_dense_pairs(Geo g, PPctx c, void* 
       if (!probe)
         for (int q = 0; q < 3; q++) vc[q] = load_v(vp, 3 * (int64_t)ti.p + q, g.izipv);
       PPacc<RT> acc;
-      pf_force<RT>(ent, st, W, pr, ti, c.mass, c.rsoft, acc, npl);
-      f2loc = pf_apply<RT>(g, c, vp, probe, ti.p, vc, acc);
+      pf_force<RT, false>(ent, st, W, pr, ti, c.mass, c.rsoft, acc, npl);
+      f2loc = pf_apply(g, c, vp, probe, ti.p, vc, acc);
       ninner = 1;
     }
     pf_reduce<PD_NT>(f2loc, npl, ninner, redf, redn, redi, f2max_bits, npairs_out, nupd_out);
@@ -676,7 +747,7 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
   int64_t* items = info + 2 * nb2;
   int32_t* dense = (int32_t*)(items + h->p.np_max / PD_NT + nb2 + 1);
   int32_t* fall = dense + nb2;
-  PPctx c{h->mass_p, h->p.rsoft, a_mid, dt, c_old, h->vdec, h->venc, h->vkmax, prange};
+  PPctx c{h->mass_p, h->p.rsoft, a_mid, dt, c_old, vtab_for(h, c_old), h->venc, h->vkmax, prange};
   int pr = prof_begin(h, PS_PP);
   int rc;
 #define PP_ARGS h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used
@@ -691,7 +762,7 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
   if (nfb) {  // blocks the pool could not hold: chunked kernel on their PP_BP^3 sub-blocks
     const int bscale = PBsel / PP_BP, nsub = bscale * bscale * bscale;
     k_pp<<<(unsigned)((int64_t)nfb * nsub), PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, prange, a_mid,
-                                                                   dt, c_old, h->vdec, h->venc, h->vkmax, f2, npr, nup, probe,
+                                                                   dt, c_old, c.vdt, h->venc, h->vkmax, f2, npr, nup, probe,
                                                                    fall, bscale);
     HIP_TRY(h, hipGetLastError());
   }

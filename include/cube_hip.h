@@ -185,6 +185,62 @@ typedef struct cube_ic_params {
 } cube_ic_params;
 int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_sim_header* sim_out);
 
+/* ---- host loop in native code: main.f90:48-97 with timestep.f90:8-95 in
+   real(4).  cube_clock holds the time-stepping state the reference keeps in
+   module variables (a, t, tau, dt, dt_old, da, a_mid, dt_neu,
+   cur_checkpoint, checkpoint_step, final_step, z_checkpoint). */
+#define CUBE_MAX_CHECKPOINTS 64
+typedef struct cube_clock {
+  float a, t, tau, dt, dt_old, da, a_mid, dt_neu;
+  int32_t cur_checkpoint, n_checkpoint, checkpoint_step, final_step;
+  int32_t checkpoint_at;  /* checkpoint index reached by the last step, else 0 */
+  int32_t pad;
+  int64_t istep;
+  float z_checkpoint[CUBE_MAX_CHECKPOINTS];
+} cube_clock;
+/* initialize.f90:40-53: a from z_checkpoint[0], cur_checkpoint = 2. */
+int cube_clock_init(cube_clock* c, const float* z_checkpoint, int n);
+/* One pass of the main-loop body: timestep, drift (update_x), PP if
+   enabled, PM, and at a checkpoint step the closing half drift
+   (main.f90:74-75).  After the final checkpoint, c->final_step and
+   c->checkpoint_step are both set. */
+int cube_step(cube_handle* h, cube_clock* c);
+/* Device bytes a box of these parameters holds (plus the Zel'dovich
+   generator's temporary mesh when ic_np_nc > 0), as used for admission. */
+int64_t cube_footprint(const cube_params* p, int32_t ic_np_nc);
+
+/* ---- ensemble serving (no counterpart in the reference): independent
+   boxes of uneven size submitted at any time; each is admitted as soon as
+   its footprint fits mem_budget and fewer than max_live boxes run, then
+   stepped from its Zel'dovich start to the last z_checkpoint on its own
+   stream and host thread, concurrently with the other live boxes. */
+typedef struct cube_ensemble cube_ensemble;
+typedef struct cube_ens_params {
+  int32_t device;
+  int32_t max_live;     /* concurrent boxes                                  */
+  int64_t mem_budget;   /* device bytes for resident boxes; <= 0: 90% free  */
+} cube_ens_params;
+typedef struct cube_ens_result {
+  int64_t id;
+  int32_t status;       /* cube_status of the run                            */
+  int32_t steps;
+  double t_submit, t_admit, t_done;  /* seconds on the engine clock          */
+  int64_t nparticles;
+  float a_final;
+  int32_t pad;
+  uint64_t state_hash;  /* FNV-1a of the final xp, vp, pid                   */
+  char message[128];
+} cube_ens_result;
+int cube_ens_create(const cube_ens_params* p, cube_ensemble** out);
+int cube_ens_submit(cube_ensemble* e, const cube_params* p, const float* kern_f_table, const float* kern_c_table,
+                    const cube_ic_params* ic, const float* z_checkpoint, int n_z, int64_t* id);
+/* Moves up to max_results finished runs into out; *n_out = how many. */
+int cube_ens_poll(cube_ensemble* e, cube_ens_result* out, int max_results, int* n_out);
+int cube_ens_pending(cube_ensemble* e, int* queued, int* live);
+double cube_ens_clock(cube_ensemble* e);
+/* Runs everything submitted to completion, then frees the engine. */
+int cube_ens_destroy(cube_ensemble* e);
+
 /* Per-kernel timing with HIP events recorded on the handle's stream (no
    counterpart in the reference, whose only timers are the per-phase
    system_clock prints, e.g. pm.f90:45,182-183).  cube_profile_read returns

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(ROOT, "include", "cube_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\bint\s+(cube_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(?:int|int64_t|double)\s+(cube_\w+)\s*\(", src)))
 
 
 def test_header_declares_expected_entry_points():
@@ -35,3 +35,6 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.DriftOut) == 56
     assert ctypes.sizeof(_lib.PPOut) == 32
     assert ctypes.sizeof(_lib.PMOut) == 56
+    assert ctypes.sizeof(_lib.CubeClock) == 320
+    assert ctypes.sizeof(_lib.EnsParams) == 16
+    assert ctypes.sizeof(_lib.EnsResult) == 192
