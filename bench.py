@@ -101,6 +101,31 @@ def cpu_baseline(cfg, st, steps_dt, sample_tiles, threads):
                                                                          t1 - t0, t2 - t1, t3 - t2, t_glob)}
 
 
+def ensemble_bench(n, rate):
+    """Ensemble serving: n seeded LCDM PM+PP boxes drawn uniformly from
+    {32,64,128,256}^3 particles (mesh 2x particles per dimension), Zel'dovich
+    start at z=49, run to z=0, arriving as a Poisson process; served
+    concurrently by the native engine, then the same boxes one at a time."""
+    from cube_nufluid_amd.serving import poisson_requests, run_ensemble
+    reqs = poisson_requests(n, rate, seed=5)
+    log("ensemble: %d requests, sizes %s, Poisson rate %.1f/s" % (n, sorted(set(r["npd"] for r in reqs)), rate))
+    mix = run_ensemble(reqs, max_live=64, log=log)
+    log("ensemble (concurrent): p50 %.3f s, p99 %.3f s, %.3g particle-steps/s"
+        % (mix["p50"], mix["p99"], mix["particle_steps_per_s"]))
+    plain = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1)
+    log("one at a time: %.3g particle-steps/s over %.2f s" % (plain["particle_steps_per_s"], plain["span"]))
+    key = lambda r: (r["npd"], r["seed"])
+    same = {key(r): r["hash"] for r in mix["results"]} == {key(r): r["hash"] for r in plain["results"]}
+    return {"workload": "C5: %d seeded LCDM PM+PP boxes, {32,64,128,256}^3 particles, z=49 to 0, Poisson arrivals "
+                        "%.1f/s" % (n, rate),
+            "p50_s": round(mix["p50"], 4), "p99_s": round(mix["p99"], 4),
+            "particle_steps_per_s": mix["particle_steps_per_s"],
+            "plain_particle_steps_per_s": plain["particle_steps_per_s"],
+            "speedup_vs_one_at_a_time": mix["particle_steps_per_s"] / plain["particle_steps_per_s"],
+            "steps": sorted(set(r["steps"] for r in mix["results"])),
+            "same_result_alone_and_in_mix": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +135,9 @@ def main():
     ap.add_argument("--z", type=float, default=0.0, help="redshift of the Zel'dovich start state")
     ap.add_argument("--seed", type=int, default=1003)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
+    ap.add_argument("--ens-requests", type=int, default=16)
+    ap.add_argument("--ens-rate", type=float, default=6.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--cpu-tiles", type=int, default=0,
                     help="tiles in the CPU sample (0: one tile when tiles hold >4M particles, else one per thread)")
     args = ap.parse_args()
@@ -186,15 +214,18 @@ def main():
         "roofline": roofline,
         "per_kernel_ms": per_kernel,
     }
+    st_cpu = None
     if not args.no_cpu_baseline:
         st_cpu = cube.download()
-        cube.close()
+    last_dt = float(cube.dt) or 0.05
+    cube.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ens_requests, args.ens_rate)
+    if st_cpu is not None:
         threads = min(16, os.cpu_count() or 1)
         per_tile = npart / cfg.nnt ** 3
         tiles = args.cpu_tiles or (1 if per_tile > 4e6 else min(threads, cfg.nnt ** 3))
-        out["cpu_baseline"] = cpu_baseline(cfg, st_cpu, float(cube.dt) or 0.05, tiles, threads)
-    else:
-        cube.close()
+        out["cpu_baseline"] = cpu_baseline(cfg, st_cpu, last_dt, tiles, threads)
     print(json.dumps(out), flush=True)
 
 

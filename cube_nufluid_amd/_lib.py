@@ -83,6 +83,11 @@ class CubeClock(ctypes.Structure):
                 ("istep", c_i64), ("z_checkpoint", c_f32 * 64)]
 
 
+class NuParams(ctypes.Structure):
+    _fields_ = [("m_ev", c_f32), ("f_neu", c_f32), ("z_i_nu", c_f32), ("gamma", c_f32), ("box", c_f32), ("h0", c_f32),
+                ("omega_m", c_f32), ("neutrino_ic", c_i32)]
+
+
 class EnsParams(ctypes.Structure):
     _fields_ = [("device", c_i32), ("max_live", c_i32), ("mem_budget", c_i64)]
 
@@ -107,6 +112,12 @@ SIGNATURES = {
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
     "cube_power_spectrum": (c_i32, [_P, c_i32, _P, _P, _P]),
+    "cube_nu_init": (c_i32, [_P, ctypes.POINTER(NuParams)]),
+    "cube_nu_info": (c_i32, [_P, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_f32), ctypes.POINTER(c_f32),
+                             ctypes.POINTER(c_f32)]),
+    "cube_nu_set_fluid": (c_i32, [_P, c_i32, _P]),
+    "cube_nu_get_fluid": (c_i32, [_P, c_i32, _P]),
+    "cube_fluid_evolve": (c_i32, [_P, c_f32, c_i32, c_f32, ctypes.POINTER(c_f32)]),
     "cube_clock_init": (c_i32, [ctypes.POINTER(CubeClock), _P, c_i32]),
     "cube_step": (c_i32, [_P, ctypes.POINTER(CubeClock)]),
     "cube_footprint": (c_i64, [ctypes.POINTER(CubeParams), c_i32]),

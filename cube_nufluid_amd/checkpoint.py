@@ -93,3 +93,40 @@ def read_z_list(path):
     if not zs:
         raise ValueError("%s empty" % os.path.basename(path))
     return zs
+
+
+# ------------------------------------------------------------ neutrino fluid
+def neu_ncb(dx, ncb=6, ncell=4):
+    """Halo width of a fluid of cell size dx (hydrogr.f90:53)."""
+    return max(2, 1 + int(np.floor(ncb * ncell / dx)))
+
+
+def write_neu_checkpoint(fn, m_ev, gamma, isothermal, weights, fluids, dxs, ncb=6):
+    """dfld_checkpoint (hydrodf.f90:143-166): m/eV, gamma, isothermal
+    (logical, 4 bytes), the three density weights, then every fluid's
+    fld(5, 1-ncb:nc+ncb, ...) with its periodic halo, first index fastest.
+    fluids: [5][nc][nc][nc] float32 arrays (z, y, x order)."""
+    with open(fn, "wb") as f:
+        np.array([m_ev, gamma], dtype=np.float32).tofile(f)
+        np.array([1 if isothermal else 0], dtype=np.int32).tofile(f)
+        np.asarray(weights, dtype=np.float32).tofile(f)
+        for fl, dx in zip(fluids, dxs):
+            b = neu_ncb(dx, ncb)
+            ext = np.pad(np.asarray(fl, dtype=np.float32), ((0, 0), (b, b), (b, b), (b, b)), mode="wrap")
+            np.ascontiguousarray(np.moveaxis(ext, 0, -1)).tofile(f)   # (k, j, i, q): q fastest
+
+
+def read_neu_checkpoint(fn, ncs, dxs, ncb=6):
+    """Inverse of write_neu_checkpoint: (m_ev, gamma, isothermal, weights,
+    fluids as [5][nc][nc][nc] physical cells)."""
+    with open(fn, "rb") as f:
+        m_ev, gamma = np.fromfile(f, dtype=np.float32, count=2)
+        iso = bool(np.fromfile(f, dtype=np.int32, count=1)[0])
+        w = np.fromfile(f, dtype=np.float32, count=3)
+        fluids = []
+        for nc, dx in zip(ncs, dxs):
+            b = neu_ncb(dx, ncb)
+            ne = nc + 2 * b
+            ext = np.fromfile(f, dtype=np.float32, count=5 * ne ** 3).reshape(ne, ne, ne, 5)
+            fluids.append(np.ascontiguousarray(np.moveaxis(ext, -1, 0)[:, b:b + nc, b:b + nc, b:b + nc]))
+    return float(m_ev), float(gamma), iso, w, fluids

@@ -28,6 +28,7 @@ SOURCES = {
     "ic.hip": ["-ffp-contract=off"],
     "pk.hip": [],
     "serve.hip": ["-ffp-contract=off"],
+    "fluid.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

@@ -202,7 +202,7 @@ extern "C" int cube_profile_enable(cube_handle* h, int on) {
 }
 
 static const char* kSlotNames[PS_NSLOT] = {"drift_pass1", "drift_scan", "drift_scatter", "drift_sort_big", "drift_write",
-                                           "pp", "cic_coarse",
+                                           "fluid", "pp", "cic_coarse",
                                            "fft_coarse", "cic_fine", "fft_x_r2c", "fft_y_fwd", "fft_zconv",
                                            "fft_y_inv", "fft_x_c2r", "kick"};
 
@@ -248,6 +248,7 @@ extern "C" int cube_finalize(cube_handle* h) {
     if (b) (void)hipFree(b);
   mesh_free(h->fine);
   mesh_free(h->coarse);
+  fl_free(h->nu);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CUBE_OK;
@@ -348,8 +349,10 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   uint32_t* f2f = h->uflags + 33;
   uint32_t* vmx = h->uflags + 34;  // 3 entries
   HIP_TRY(h, hipMemsetAsync(h->uflags + 32, 0, sizeof(uint32_t) * 8, h->stream));
+  const bool nu = h->nu.on;
   // coarse mesh: force_c for all three components (pm.f90:188-326)
   if ((r = cic_coarse(h, &mass_c))) return r;
+  if (nu && (r = fl_density_coarse(h, a_mid))) return r;
   if ((r = mesh_fwd_xy(h, h->coarse))) return r;
   for (int d = 0; d < 3; d++) {
     if ((r = mesh_zconv(h, h->coarse, d))) return r;
@@ -358,13 +361,17 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   }
   // fine mesh, one component at a time, each followed by its kick
   if ((r = cic_fine(h, &mass_f))) return r;
+  if (nu && (r = fl_density_fine(h, a_mid))) return r;
   if ((r = mesh_fwd_xy(h, h->fine))) return r;
   const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
+  const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
   for (int d = 0; d < 3; d++) {
     if ((r = mesh_zconv(h, h->fine, d))) return r;
     if ((r = mesh_inv_yx(h, h->fine, h->fine.W, d + 1, f2f))) return r;
     if ((r = kick_component(h, d, a_mid, dt, c_old, c_new, vmx + d, probe_f, probe_c))) return r;
+    if (nu_kick && (r = fl_gravity_fine(h, d, a_mid, dt))) return r;
   }
+  if (nu_kick && (r = fl_gravity_finish(h, a_mid, dt))) return r;
   uint32_t hv[5];
   HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 32, sizeof(hv), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));

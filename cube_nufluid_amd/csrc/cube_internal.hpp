@@ -37,10 +37,22 @@ struct Mesh {
 
 // Per-kernel timing with HIP events on the handle's stream (diagnostic).
 enum ProfSlot {
-  PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT_SCATTER, PS_DRIFT_SORT, PS_DRIFT_WRITE, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
+  PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT_SCATTER, PS_DRIFT_SORT, PS_DRIFT_WRITE, PS_FLUID, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
   PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_NSLOT
 };
 struct ProfRec { int slot; hipEvent_t a, b; };
+
+// Neutrino dispersive fluid: three fluids of nf/2, nf/4, nf/8 cells per
+// dimension, fields [5][nc^3] (rho, mx, my, mz, E) and gravity accumulators
+// [4][nc^3] (J0, J1, J2, Q) in double.
+struct NuState {
+  bool on = false;
+  cube_nu_params par{};
+  int nc[3] = {0, 0, 0};
+  float dx[3] = {0, 0, 0}, cs2[3] = {0, 0, 0}, w[3] = {0, 0, 0}, ai[3] = {0, 0, 0};
+  float* F[3] = {nullptr, nullptr, nullptr};
+  double* acc[3] = {nullptr, nullptr, nullptr};
+};
 
 struct cube_handle {
   cube_params p;
@@ -80,6 +92,7 @@ struct cube_handle {
   uint32_t pp_overflow_blocks = 0;  // PP blocks handled by the chunked fallback in the last call
   void* pp_meta = nullptr;    // PP dense-block lists and work items
   void* pp_red = nullptr;     // PP per-slot totals (1024 slots)
+  NuState nu;
   cube_sim_header sim{};
   std::string err;
   // profiling
@@ -229,6 +242,12 @@ int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, d
 int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe);
 // drift.hip
 int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
+// fluid.hip: PM coupling of the neutrino fluid
+int fl_density_coarse(cube_handle* h, float a_mid);
+int fl_density_fine(cube_handle* h, float a_mid);
+int fl_gravity_fine(cube_handle* h, int d, float a_mid, float dt);
+int fl_gravity_finish(cube_handle* h, float a_mid, float dt);
+void fl_free(NuState& s);
 
 static inline Geo geo_of(const cube_handle* h) {
   Geo g;

@@ -88,9 +88,12 @@ extern "C" int cube_step(cube_handle* h, cube_clock* c) {
   clock_timestep(h, c);
   c->istep += 1;
   int r;
+  // main.f90:51-55, 66-70: the fluid is evolved half a step on each side
+  if (h->nu.on && (r = cube_fluid_evolve(h, c->dt / 2, 1, c->a_mid, &c->dt_neu))) return r;
   if ((r = cube_drift(h, c->dt_old, c->dt, nullptr))) return r;
   if (h->p.pp_on && (r = cube_pp(h, c->a_mid, c->dt, nullptr))) return r;
   if ((r = cube_pm(h, c->a_mid, c->dt, nullptr))) return r;
+  if (h->nu.on && (r = cube_fluid_evolve(h, c->dt / 2, -1, c->a_mid, &c->dt_neu))) return r;
   if (c->checkpoint_step) {
     c->dt_old = 0;  // main.f90:74-75: the half drift closing the step
     if ((r = cube_drift(h, c->dt_old, c->dt, nullptr))) return r;

@@ -130,6 +130,39 @@ class Cube:
         except Exception:
             pass
 
+    # ----------------------------------------------------- neutrino fluid
+    def nu_init(self, m_ev=0.1, f_neu=None, z_i_nu=10.0, gamma=5.0 / 3.0, neutrino_ic=True):
+        """dfld_setup for one neutrino mass; f_neu defaults to
+        omega_nu/omega_m with omega_nu = 3 m/93.14/h^2 (parameters.f90:83-92)."""
+        c = self.cfg
+        if f_neu is None:
+            f_neu = 3 * m_ev / 93.14 / c.h0 ** 2 / c.omega_m
+        p = _lib.NuParams(m_ev, f_neu, z_i_nu, gamma, c.box, c.h0, c.omega_m, int(neutrino_ic))
+        _lib.check(self.lib, self.h, self.lib.cube_nu_init(self.h, ctypes.byref(p)))
+        self.nu_params = p
+        self.nu_on = True
+
+    def nu_info(self, n):
+        nc, dx, cs2, w = ctypes.c_int32(), ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        _lib.check(self.lib, self.h, self.lib.cube_nu_info(self.h, n, ctypes.byref(nc), ctypes.byref(dx), ctypes.byref(cs2),
+                                                           ctypes.byref(w)))
+        return dict(nc=nc.value, dx=dx.value, cs2=cs2.value, weight=w.value)
+
+    def nu_get(self, n):
+        nc = self.nu_info(n)["nc"]
+        f = np.empty((5, nc, nc, nc), dtype=np.float32)
+        _lib.check(self.lib, self.h, self.lib.cube_nu_get_fluid(self.h, n, _lib.ptr(f)))
+        return f
+
+    def nu_set(self, n, f):
+        f = np.ascontiguousarray(f, dtype=np.float32)
+        _lib.check(self.lib, self.h, self.lib.cube_nu_set_fluid(self.h, n, _lib.ptr(f)))
+
+    def fluid_evolve(self, dt, direction, a):
+        dtn = ctypes.c_float(float(self.dt_neu))
+        _lib.check(self.lib, self.h, self.lib.cube_fluid_evolve(self.h, float(dt), int(direction), float(a), ctypes.byref(dtn)))
+        self.dt_neu = f32(dtn.value)
+
     # ------------------------------------------------------- subroutines
     def expansion(self, a, dt):
         return _lib.expansion(a, dt, self.cfg.omega_m, self.cfg.omega_l, self.cfg.wde)
@@ -205,10 +238,15 @@ class Cube:
         """One pass of main.f90:48-97; returns True after the final checkpoint."""
         t0 = time.perf_counter()
         self.timestep()
+        nu = getattr(self, "nu_on", False)
+        if nu:  # main.f90:51-55
+            self.fluid_evolve(f32(self.dt / 2), 1, self.a_mid)
         self.update_x()
         if self.cfg.pp_on:
             self.ext_pp_force()
         self.particle_mesh()
+        if nu:  # main.f90:66-70
+            self.fluid_evolve(f32(self.dt / 2), -1, self.a_mid)
         if self.checkpoint_step:
             self.dt_old = f32(0)
             self.update_x()

@@ -185,6 +185,35 @@ typedef struct cube_ic_params {
 } cube_ic_params;
 int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_sim_header* sim_out);
 
+/* ---- neutrino fluid (hydrodf.f90, hydrogr.f90, hydro3d.f90, hydro1d.f90):
+   the dispersive sum of three fluids at the Gauss-Laguerre velocities
+   v_n/beta*c, beta = 5966.34 m/eV, on nf/2, nf/4, nf/8 cells per dimension;
+   one neutrino mass (parameters.f90:80-83, Nneu = 1). */
+typedef struct cube_nu_params {
+  float m_ev;          /* neutrino mass, eV (Mneu)                          */
+  float f_neu;         /* omega_nu/omega_m (parameters.f90:92)              */
+  float z_i_nu;        /* redshift where the fluid starts (parameters.f90:73) */
+  float gamma;         /* adiabatic index, 5/3 in main.f90:38               */
+  float box, h0, omega_m;  /* for the velocity unit hg_dv (hydrogr.f90:14) */
+  int32_t neutrino_ic; /* 1: fluids start at z_i_nu (hydrodf.f90:1,14-16)  */
+} cube_nu_params;
+/* dfld_setup (hydrodf.f90:54-86) + hg_setup (hydrogr.f90:47-81): uniform
+   fluids rho=1, m=0, E=cs2/(gamma-1); normalised density weights.  From
+   then on cube_pm adds the fluid density to both PM meshes and the PM
+   forces to the fluid (pm.f90:55-74, 157-171, 198-242, 374-422). */
+int cube_nu_init(cube_handle* h, const cube_nu_params* p);
+/* Geometry of fluid n (0..2): cells per dimension, cell size in fine cells,
+   cs^2 in simulation units, density weight. */
+int cube_nu_info(cube_handle* h, int n, int32_t* nc, float* dx, float* cs2, float* weight);
+/* Fields of fluid n as [5][nc^3] float (rho, mx, my, mz, E), x fastest --
+   the physical cells of hg_checkpoint/hg_restart (hydrogr.f90:395-446). */
+int cube_nu_set_fluid(cube_handle* h, int n, const float* fld);
+int cube_nu_get_fluid(cube_handle* h, int n, float* fld);
+/* dfld_evolve (hydrodf.f90:90-110): fluids with a >= a_i swept x,y,z
+   (dir > 0) or z,y,x (dir < 0) with step dt; *dt_neu = min(*dt_neu, CFL
+   step of each evolved fluid) (hydrogr.f90:104-124). */
+int cube_fluid_evolve(cube_handle* h, float dt, int dir, float a, float* dt_neu);
+
 /* ---- host loop in native code: main.f90:48-97 with timestep.f90:8-95 in
    real(4).  cube_clock holds the time-stepping state the reference keeps in
    module variables (a, t, tau, dt, dt_old, da, a_mid, dt_neu,

@@ -40,6 +40,12 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* fluid_oracle.c */
+int ofl_evolve(float* P, int nc, int ncb, float g, float dt, float dx, int dir, float* dtg);
+void ofl_gravity_fine(float* P, int nc, float dx, const float* F, int nf, float a_mid, float dt);
+void ofl_gravity_coarse(float* P, int nc, float dx, const float* Fc, int ncc, int ncell, float a_mid, float dt);
+void ofl_density_centres(float* out, int nf, int nfl, const float** rho, const int* ncf, const float* dxf, const float* w);
 #endif
 
 typedef struct { double re, im; } cpx;
@@ -60,6 +66,11 @@ typedef struct orc {
   float ftemp_pp; /* ext_pp_force.f90:18 'real ftemp' is SAVEd, never reset */
   int64_t tile_limit; /* timing samples only: process tiles with index < tile_limit */
   double t_global;    /* seconds of the last PM call spent on box-wide coarse mesh work */
+  /* neutrino fluid (fluid_oracle.c): three fluids, fields [5][nc^3] */
+  int nu_on;
+  float nu_fneu, nu_zi, nu_gamma, nu_w[3], nu_dx[3], nu_ai[3];
+  int nu_nc[3], nu_ncb[3];
+  float* nu_F[3];
   float *kern_f;  /* (nfe/2+1, nfe, nfe, 3) Fortran order */
   float *kern_c;  /* (nc/2+1, nc, nc, 3) */
   float fk_table[16 * 16 * 16 * 3];
@@ -395,7 +406,9 @@ orc_t* orc_create(const int64_t* ip, const float* fp, const float* fk_table, con
 void orc_destroy(orc_t* o) {
   if (!o) return;
   free(o->xp); free(o->vp); free(o->pid); free(o->rhoc); free(o->vfield); free(o->cstart);
-  free(o->kern_f); free(o->kern_c); free(o);
+  free(o->kern_f); free(o->kern_c);
+  for (int n = 0; n < 3; n++) free(o->nu_F[n]);
+  free(o);
 }
 const char* orc_error(orc_t* o) { return o->err; }
 int64_t orc_np(orc_t* o) { return o->np; }
@@ -409,6 +422,33 @@ void orc_scalars(orc_t* o, float* out) {
 }
 void orc_set_tile_limit(orc_t* o, int64_t n) { o->tile_limit = n; }
 double orc_global_seconds(const orc_t* o) { return o->t_global; }
+
+/* neutrino fluid set-up: fpar = {f_neu, z_i_nu, gamma, w[3], dx[3], a_i[3]},
+   ipar = {nc[3], ncb[3]}; fields start as given by orc_nu_set */
+void orc_nu_setup(orc_t* o, const float* fpar, const int* ipar) {
+  o->nu_on = 1;
+  o->nu_fneu = fpar[0]; o->nu_zi = fpar[1]; o->nu_gamma = fpar[2];
+  for (int n = 0; n < 3; n++) {
+    o->nu_w[n] = fpar[3 + n]; o->nu_dx[n] = fpar[6 + n]; o->nu_ai[n] = fpar[9 + n];
+    o->nu_nc[n] = ipar[n]; o->nu_ncb[n] = ipar[3 + n];
+    free(o->nu_F[n]);
+    o->nu_F[n] = (float*)calloc((size_t)5 * ipar[n] * ipar[n] * ipar[n], sizeof(float));
+  }
+}
+void orc_nu_set(orc_t* o, int n, const float* f) {
+  memcpy(o->nu_F[n], f, sizeof(float) * 5 * (size_t)o->nu_nc[n] * o->nu_nc[n] * o->nu_nc[n]);
+}
+void orc_nu_get(const orc_t* o, int n, float* f) {
+  memcpy(f, o->nu_F[n], sizeof(float) * 5 * (size_t)o->nu_nc[n] * o->nu_nc[n] * o->nu_nc[n]);
+}
+/* dfld_evolve (hydrodf.f90:90-110) */
+int orc_nu_evolve(orc_t* o, float dt, int dir, float a, float* dt_neu) {
+  for (int n = 0; n < 3; n++) {
+    if (!(a >= o->nu_ai[n])) continue;
+    if (ofl_evolve(o->nu_F[n], o->nu_nc[n], o->nu_ncb[n], o->nu_gamma, dt, o->nu_dx[n], dir, dt_neu)) return 1;
+  }
+  return 0;
+}
 void orc_set_scalars(orc_t* o, const float* in) {
   o->sigma_vi = in[0]; o->sigma_vi_new = in[1]; o->vz_max = in[2]; o->mass_p = in[3]; o->ftemp_pp = in[4];
 }
@@ -743,6 +783,22 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
   if (inner < 1) inner = 1;
   int outer = threads / inner;
   if (outer < 1) outer = 1;
+  /* neutrino fluid: density at fine-cell centres, global fine force for the
+     fluid's gravity (pm.f90:57-72, 157-171) */
+  const int64_t nf = nc * ncell;
+  const int nu = o->nu_on;
+  const int nu_homog = nu && 1.0f / a_mid - 1.0f > o->nu_zi;
+  int nu_grav = 0;
+  for (int n = 0; nu && n < 3; n++) nu_grav = nu_grav || a_mid >= o->nu_ai[n];
+  if (probe) nu_grav = 0;
+  float* Dc = NULL;
+  float* Fg = NULL;
+  if (nu && !nu_homog) {
+    Dc = (float*)malloc(sizeof(float) * nf * nf * nf);
+    const float* rr[3] = {o->nu_F[0], o->nu_F[1], o->nu_F[2]};
+    ofl_density_centres(Dc, (int)nf, 3, rr, o->nu_nc, o->nu_dx, o->nu_w);
+  }
+  if (nu_grav) Fg = (float*)malloc(sizeof(float) * 3 * nf * nf * nf);
   /* ---- fine PM per tile */
 #pragma omp parallel for num_threads(outer) schedule(dynamic, 1) collapse(3) reduction(+ : testrhof) reduction(max : f2f)
   for (int64_t tz = 0; tz < nnt; tz++)
@@ -756,6 +812,15 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
         float* tim = (float*)malloc(sizeof(float) * hf * nfe * nfe);
         float* force = (float*)malloc(sizeof(float) * 3 * nfo * nfo * nfo);
 #define RF(i, j, k) rho[((int64_t)(k)-1) * nfe * nfe + ((int64_t)(j)-1) * nfe + ((i)-1)]
+        if (nu) /* node i of the extended tile sits at global fine position origin+i-1-nfb */
+          for (int64_t k = 1; k <= nfe; k++)
+            for (int64_t j = 1; j <= nfe; j++)
+              for (int64_t i = 1; i <= nfe; i++) {
+                if (nu_homog) { RF(i, j, k) += o->nu_fneu; continue; }
+                const int64_t gx = wrapi(tx * nt * ncell + i - 1 - nfb, nf), gy = wrapi(ty * nt * ncell + j - 1 - nfb, nf),
+                              gz = wrapi(tz * nt * ncell + k - 1 - nfb, nf);
+                RF(i, j, k) += Dc[(gz * nf + gy) * nf + gx] * o->nu_fneu;
+              }
         for (int il = 0; il < 3; il++)
           for (int64_t k = 2 - ncb + il; k <= nt + ncb - 1; k += 3)
             for (int64_t j = 2 - ncb; j <= nt + ncb - 1; j++)
@@ -802,6 +867,13 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
                 force[((k * nfo + j) * nfo + i) * 3 + d] = RF(nfb + i, nfb + j, nfb + k);
         }
 #undef RF
+        if (Fg) /* force index i covers global nodes origin+i-1; keep the physical ones */
+          for (int64_t k = 1; k <= nt * ncell; k++)
+            for (int64_t j = 1; j <= nt * ncell; j++)
+              for (int64_t i = 1; i <= nt * ncell; i++)
+                for (int d = 0; d < 3; d++)
+                  Fg[(size_t)d * nf * nf * nf + ((tz * nt * ncell + k - 1) * nf + (ty * nt * ncell + j - 1)) * nf +
+                     (tx * nt * ncell + i - 1)] = force[((k * nfo + j) * nfo + i) * 3 + d];
         for (int64_t q = 0; q < nfo * nfo * nfo; q++) {
           float f = force[q * 3] * force[q * 3] + force[q * 3 + 1] * force[q * 3 + 1] + force[q * 3 + 2] * force[q * 3 + 2];
           if (f > f2f) f2f = f;
@@ -855,6 +927,11 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
 #undef FF
         free(rho); free(cre); free(cim); free(tre); free(tim); free(force);
       }
+  if (Fg) {
+    for (int n = 0; n < 3; n++)
+      if (a_mid >= o->nu_ai[n]) ofl_gravity_fine(o->nu_F[n], o->nu_nc[n], o->nu_dx[n], Fg, (int)nf, a_mid, dt);
+    free(Fg);
+  }
   if (!probe) o->sigma_vi = o->sigma_vi_new; /* pm.f90:180 */
   const float sig_c = probe ? sig_new : o->sigma_vi;
 
@@ -902,6 +979,32 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
 #undef RT
         free(r3t);
       }
+  if (nu) { /* pm.f90:198-242: fine-cell-centre density CIC onto coarse nodes */
+    for (int64_t q = 0; nu_homog && q < n3c; q++) r3[q] += o->nu_fneu * (float)(ncell * ncell * ncell);
+    if (!nu_homog)
+      for (int64_t mz = 0; mz < nf; mz++)
+        for (int64_t my = 0; my < nf; my++)
+          for (int64_t mx = 0; mx < nf; mx++) {
+            const float rnu = Dc[(mz * nf + my) * nf + mx] * o->nu_fneu;
+            const int64_t m[3] = {mx, my, mz};
+            float d1[3], d2[3];
+            int64_t k1[3], k2[3];
+            for (int d = 0; d < 3; d++) {
+              const float xc = ((float)m[d] + 0.5f) / (float)ncell - 0.5f;
+              const int64_t f = (int64_t)floorf(xc);
+              d1[d] = (float)(f + 1) - xc;
+              d2[d] = 1.0f - d1[d];
+              k1[d] = wrapi(f, nc);
+              k2[d] = wrapi(f + 1, nc);
+            }
+            for (int t = 0; t < 8; t++) {
+              const int ox = t & 1, oy = (t >> 1) & 1, oz = t >> 2;
+              r3[((oz ? k2[2] : k1[2]) * nc + (oy ? k2[1] : k1[1])) * nc + (ox ? k2[0] : k1[0])] +=
+                  rnu * (ox ? d2[0] : d1[0]) * (oy ? d2[1] : d1[1]) * (oz ? d2[2] : d1[2]);
+            }
+          }
+  }
+  free(Dc);
   const double t_glob0 = omp_get_wtime();
   double testrhoc = 0;
   for (int64_t q = 0; q < n3c; q++) testrhoc += (double)r3[q];
@@ -975,6 +1078,14 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
   }
 #undef FC
   vmax[0] = vm0; vmax[1] = vm1; vmax[2] = vm2;
+  if (nu_grav) { /* pm.f90:374-422 */
+    float* fcs = (float*)malloc(sizeof(float) * 3 * n3c);
+    for (int64_t q = 0; q < n3c; q++)
+      for (int d = 0; d < 3; d++) fcs[(size_t)d * n3c + q] = fc[q * 3 + d];
+    for (int n = 0; n < 3; n++)
+      if (a_mid >= o->nu_ai[n]) ofl_gravity_coarse(o->nu_F[n], o->nu_nc[n], o->nu_dx[n], fcs, (int)nc, (int)ncell, a_mid, dt);
+    free(fcs);
+  }
   free(r3); free(cre); free(cim); free(tre); free(tim); free(fc);
   float dt_fine = sqrtf(1.0f / (sqrtf(f2f) * a_mid * GG));
   float dt_coarse = sqrtf((float)ncell / (sqrtf(f2c) * a_mid * GG));

@@ -51,6 +51,11 @@ def load():
         "orc_density": (ctypes.c_int, [_P, _P, ctypes.c_int]),
         "orc_set_tile_limit": (None, [_P, _i64]),
         "orc_global_seconds": (_f64, [_P]),
+        "orc_nu_setup": (None, [_P, _P, _P]),
+        "orc_nu_set": (None, [_P, ctypes.c_int, _P]),
+        "orc_nu_get": (None, [_P, ctypes.c_int, _P]),
+        "orc_nu_evolve": (ctypes.c_int, [_P, _f32, ctypes.c_int, _f32, _P]),
+        "ofl_evolve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _f32, _f32, _f32, ctypes.c_int, _P]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
@@ -187,3 +192,35 @@ def fft_r2c(r):
     im = np.zeros_like(re)
     load().orc_fft_r2c(p(r), n, p(re), p(im), 4)
     return re + 1j * im.astype(np.float64)
+
+
+    # neutrino fluid
+    def nu_setup(self, f_neu, z_i_nu, gamma, weights, dxs, ais, ncs, ncbs, fluids):
+        fp = np.array([f_neu, z_i_nu, gamma, *weights, *dxs, *ais], dtype=np.float32)
+        ip = np.array([*ncs, *ncbs], dtype=np.int32)
+        self.lib.orc_nu_setup(self.o, p(fp), p(ip))
+        self.nu_ncs = list(ncs)
+        for n, f in enumerate(fluids):
+            self.lib.orc_nu_set(self.o, n, p(np.ascontiguousarray(f, dtype=np.float32)))
+
+    def nu_get(self, n):
+        nc = self.nu_ncs[n]
+        f = np.empty((5, nc, nc, nc), dtype=np.float32)
+        self.lib.orc_nu_get(self.o, n, p(f))
+        return f
+
+    def nu_evolve(self, dt, direction, a, dt_neu=1000.0):
+        d = np.array([dt_neu], dtype=np.float32)
+        self._chk(self.lib.orc_nu_evolve(self.o, dt, direction, a, p(d)))
+        return float(d[0])
+
+
+def fluid_evolve(fld, ncb, gamma, dt, dx, direction, dtg=1000.0):
+    """ofl_evolve on one fluid's physical fields [5][nc][nc][nc] (in place copy returned)."""
+    lib = load()
+    f = np.ascontiguousarray(fld, dtype=np.float32).copy()
+    d = np.array([dtg], dtype=np.float32)
+    rc = lib.ofl_evolve(p(f), f.shape[1], ncb, gamma, dt, dx, direction, p(d))
+    if rc:
+        raise RuntimeError("ofl_evolve failed")
+    return f, float(d[0])

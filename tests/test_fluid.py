@@ -1,0 +1,192 @@
+"""Neutrino fluid: the CPU restatement (oracle/fluid_oracle.c) pinned by
+exact invariants of the finite-volume scheme, the order-free form of the
+gravity update checked against the sequential one, the 'neu' checkpoint
+format, and the GPU fluid (through the C ABI) against the restatement."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from cube_nufluid_amd.checkpoint import neu_ncb, read_neu_checkpoint, write_neu_checkpoint
+
+G = np.float32(5.0 / 3.0)
+
+
+def smooth_state(nc, seed, amp=0.2):
+    rng = np.random.default_rng(seed)
+    k = 2 * np.pi / nc
+    z, y, x = np.meshgrid(np.arange(nc), np.arange(nc), np.arange(nc), indexing="ij")
+    f = np.empty((5, nc, nc, nc), dtype=np.float32)
+    ph = rng.uniform(0, 2 * np.pi, size=8)
+    f[0] = 1 + amp * np.sin(k * x + ph[0]) * np.cos(k * y + ph[1]) + 0.5 * amp * np.sin(2 * k * z + ph[2])
+    for d in range(3):
+        f[1 + d] = f[0] * 0.3 * amp * np.sin(k * (x + y + z) + ph[3 + d])
+    kin = 0.5 * (f[1] ** 2 + f[2] ** 2 + f[3] ** 2) / f[0]
+    f[4] = kin + (0.5 + 0.2 * np.cos(k * y + ph[7])) / (G - 1)
+    return f
+
+
+def test_uniform_flow_is_invariant():
+    f = np.zeros((5, 8, 8, 8), dtype=np.float32)
+    f[0], f[1], f[2], f[3] = 1.3, 0.2, -0.1, 0.05
+    f[4] = 0.5 * (0.2 ** 2 + 0.1 ** 2 + 0.05 ** 2) / 1.3 + 0.7
+    for d in (1, -1):
+        g, dt = ol.fluid_evolve(f, 2, G, 0.1, 2.0, d)
+        np.testing.assert_array_equal(g, f)
+        assert dt < 1000
+
+
+def test_conservation_and_mirror_symmetry():
+    f = smooth_state(16, 3)
+    g, _ = ol.fluid_evolve(f, 2, G, 0.2, 2.0, 1)
+    for q in (0, 1, 2, 3, 4):
+        s0, s1 = f[q].astype(np.float64).sum(), g[q].astype(np.float64).sum()
+        assert abs(s1 - s0) <= 2e-6 * np.abs(f[q]).astype(np.float64).sum(), q
+    # mirror x -> -x (cells i -> nc-1-i) with mx -> -mx commutes with the update
+    m = f[:, :, :, ::-1].copy()
+    m[1] = -m[1]
+    gm, _ = ol.fluid_evolve(m, 2, G, 0.2, 2.0, 1)
+    gb = gm[:, :, :, ::-1].copy()
+    gb[1] = -gb[1]
+    np.testing.assert_allclose(gb, g, rtol=2e-6, atol=2e-7)
+
+
+def test_stationary_contact_keeps_pressure_and_rest():
+    """Density jump at uniform pressure and zero velocity: HLL diffuses mass,
+    but the momentum flux is the uniform pressure and the energy flux E v is
+    zero, so momenta stay exactly zero and E exactly unchanged."""
+    nc = 16
+    f = np.zeros((5, nc, nc, nc), dtype=np.float32)
+    f[0] = 1.0
+    f[0][:, :, nc // 2:] = 3.0
+    f[4] = np.float32(0.6) / (G - 1)
+    g, _ = ol.fluid_evolve(f, 2, G, 0.3, 2.0, 1)
+    assert np.all(g[1:4] == 0)
+    np.testing.assert_array_equal(g[4], f[4])
+    assert not np.array_equal(g[0], f[0])
+    assert abs(g[0].astype(np.float64).sum() - f[0].astype(np.float64).sum()) < 1e-3
+
+
+def test_gravity_closed_form_equals_sequential_impulses():
+    """m += rho J, E += m0.J + rho/2 (|J|^2 + Q) equals applying the impulses
+    dV_k a_k one after another with dE = m.a dV + rho/2 |a|^2 dV."""
+    rng = np.random.default_rng(1)
+    rho = 1.7
+    m0 = rng.normal(size=3)
+    dV = rng.uniform(0, 1, size=40)
+    a = rng.normal(size=(40, 3))
+    m, E = m0.copy(), 2.5
+    for k in rng.permutation(40):
+        E += dV[k] * (m @ a[k] + 0.5 * rho * a[k] @ a[k])
+        m += dV[k] * rho * a[k]
+    J = (dV[:, None] * a).sum(0)
+    Q = (dV * (1 - dV) * (a * a).sum(1)).sum()
+    np.testing.assert_allclose(m, m0 + rho * J, rtol=1e-13)
+    assert abs(E - (2.5 + m0 @ J + 0.5 * rho * (J @ J + Q))) < 1e-12 * abs(E)
+
+
+def test_neu_checkpoint_round_trip(tmp_path):
+    fl = [smooth_state(n, n) for n in (16, 8, 4)]
+    dxs = [2.0, 4.0, 8.0]
+    fn = str(tmp_path / "0.000_neu1_1.bin")
+    write_neu_checkpoint(fn, 0.1, 5.0 / 3.0, True, [0.5, 0.3, 0.2], fl, dxs)
+    b = [neu_ncb(dx) for dx in dxs]
+    assert b == [13, 7, 4]  # ncb = 6, ncell = 4: hydrogr.f90:53
+    size = 4 * (2 + 1 + 3) + sum(4 * 5 * (n + 2 * bb) ** 3 for n, bb in zip((16, 8, 4), b))
+    assert (tmp_path / "0.000_neu1_1.bin").stat().st_size == size
+    m, g, iso, w, back = read_neu_checkpoint(fn, (16, 8, 4), dxs)
+    assert (m, iso) == (np.float32(0.1), True) and abs(g - 5 / 3) < 1e-6
+    for a, b2 in zip(fl, back):
+        np.testing.assert_array_equal(a, b2)
+
+
+# ------------------------------------------------------------------- GPU
+def hydrodf_setup(cfg, m_ev, f_neu):
+    """hydrodf.f90:54-86 and hydrogr.f90:47-81 in float32, independently."""
+    f = np.float32
+    nf = f(cfg.nf)
+    dv = f(2) * nf / (f(3.0) * f(100.0) * np.sqrt(f(cfg.omega_m)) * f(cfg.box))
+    beta = f(5966.34) * f(m_ev)
+    glv = np.array([0.41577456, 2.29428036, 6.28994508], dtype=np.float32)
+    glw = np.array([0.71109301, 0.27851773, 0.01038926], dtype=np.float32)
+    w = glw * (glv ** 2 / (np.exp(glv) + f(1))) * np.exp(glv)
+    w = w / w.sum()
+    cs2 = np.maximum(f(1e-6), (glv / beta * f(2.998e5) * dv) ** 2)
+    ncs = [cfg.nf // 2, cfg.nf // 4, cfg.nf // 8]
+    return w, cs2, ncs
+
+
+@pytest.mark.gpu
+def test_gpu_fluid_setup_and_sweeps_match_restatement():
+    from cube_nufluid_amd.params import BoxConfig
+    from cube_nufluid_amd.ic import zeldovich_ic
+    from cube_nufluid_amd.driver import Cube
+    cfg = BoxConfig(nc=16, nnt=1, ncb=6, box=100.0)
+    cube = Cube(cfg, zeldovich_ic(cfg, seed=3, z=10.0))
+    cube.nu_init(m_ev=0.1, f_neu=0.0075, z_i_nu=10.0)
+    w, cs2, ncs = hydrodf_setup(cfg, 0.1, 0.0075)
+    for n in range(3):
+        info = cube.nu_info(n)
+        assert info["nc"] == ncs[n] and info["dx"] == cfg.nf / ncs[n]
+        assert abs(info["weight"] - w[n]) < 1e-6 and abs(info["cs2"] / cs2[n] - 1) < 1e-5
+        g0 = cube.nu_get(n)
+        assert np.all(g0[0] == 1) and np.all(g0[1:4] == 0)
+        np.testing.assert_allclose(g0[4], info["cs2"] / (5.0 / 3.0 - 1), rtol=1e-6)
+    # sweeps on a smooth state, both directions
+    cur = {}
+    for n in range(3):
+        cur[n] = smooth_state(ncs[n], 10 + n)
+        cube.nu_set(n, cur[n])
+    for direction in (1, -1):
+        cube.dt_neu = np.float32(1000)
+        cube.fluid_evolve(0.15, direction, 0.5)
+        dt_all = 1000.0
+        for n in range(3):
+            dx = cfg.nf / ncs[n]
+            ref, dt_all = ol.fluid_evolve(cur[n], neu_ncb(dx), G, 0.15, dx, direction, dt_all)
+            got = cube.nu_get(n)
+            print("fluid", n, "dir", direction, "max abs diff", float(np.abs(got - ref).max()), flush=True)
+            np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
+            cur[n] = got
+        assert abs(float(cube.dt_neu) / dt_all - 1) < 1e-6
+    cube.close()
+
+
+@pytest.mark.gpu
+def test_gpu_pm_with_fluid_matches_oracle():
+    """One drift + PM with the fluid coupled (density into both meshes,
+    fine and coarse forces into the fluid): fluid fields and CDM velocity
+    codes against the oracle on the same state."""
+    from cube_nufluid_amd.params import BoxConfig
+    from cube_nufluid_amd.ic import zeldovich_ic
+    from cube_nufluid_amd.driver import Cube
+    cfg = BoxConfig(nc=16, nnt=1, ncb=6, box=100.0)
+    st = zeldovich_ic(cfg, seed=5, z=3.0)
+    cube = Cube(cfg, st)
+    cube.nu_init(m_ev=0.1, f_neu=0.02, z_i_nu=10.0)
+    orc = ol.Oracle(cfg)
+    orc.set_state(st)
+    infos = [cube.nu_info(n) for n in range(3)]
+    fl = [smooth_state(i["nc"], 20 + n, amp=0.05) for n, i in enumerate(infos)]
+    for n in range(3):
+        cube.nu_set(n, fl[n])
+    ai = 1 / (1 + 10.0)
+    orc.nu_setup(0.02, 10.0, 5.0 / 3.0, [i["weight"] for i in infos], [i["dx"] for i in infos], [ai] * 3,
+                 [i["nc"] for i in infos], [neu_ncb(i["dx"]) for i in infos], fl)
+    a_mid, dt = np.float32(0.25), np.float32(0.3)
+    cube.a_mid, cube.dt = a_mid, dt
+    cube.particle_mesh()
+    orc.pm(float(a_mid), float(dt))
+    for n in range(3):
+        g, o = cube.nu_get(n), orc.nu_get(n)
+        scale = np.abs(o).reshape(5, -1).max(1)
+        err = (np.abs(g - o).reshape(5, -1).max(1) / scale)
+        print("fluid", n, "max rel err per field", err, flush=True)
+        assert np.all(err < 2e-5), (n, err)
+        assert not np.array_equal(g, fl[n])
+    a, b = cube.download(), orc.get_state()
+    diff = np.abs(a.vp.astype(np.int64) - b.vp.astype(np.int64))
+    print("vp mismatch frac", float(np.mean(diff > 0)), flush=True)
+    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    cube.close()
+
+

@@ -38,3 +38,4 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.CubeClock) == 320
     assert ctypes.sizeof(_lib.EnsParams) == 16
     assert ctypes.sizeof(_lib.EnsResult) == 192
+    assert ctypes.sizeof(_lib.NuParams) == 32
