@@ -168,7 +168,7 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
 #undef TRY
   if ((rc = codec_build(h))) return fail(rc);
   if ((rc = mesh_alloc(h, h->fine, h->nf, h->prec, true, true))) return fail(rc);
-  if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, true, true))) return fail(rc);
+  if ((rc = mesh_alloc(h, h->coarse, h->nc, h->prec, false, true))) return fail(rc);
   // the fine work mesh is idle during the drift and serves as its scratch
   // (crowded-cell merges, then three doubles per particle)
   if (h->fine.bytes_R() >= 3 * sizeof(double) * (size_t)npm) {
@@ -350,27 +350,40 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   uint32_t* vmx = h->uflags + 34;  // 3 entries
   HIP_TRY(h, hipMemsetAsync(h->uflags + 32, 0, sizeof(uint32_t) * 8, h->stream));
   const bool nu = h->nu.on;
-  // coarse mesh: force_c for all three components (pm.f90:188-326)
+  // coarse mesh: force_c for all three components (pm.f90:188-326); the
+  // third inverse also forms max |force_c|^2 from the two already written
   if ((r = cic_coarse(h, &mass_c))) return r;
   if (nu && (r = fl_density_coarse(h, a_mid))) return r;
   if ((r = mesh_fwd_xy(h, h->coarse))) return r;
+  const int64_t Nc = h->coarse.N;
+  const char* fc = (const char*)h->fc;
+  const size_t fcs = (size_t)h->n3c * h->prec;
   for (int d = 0; d < 3; d++) {
-    if ((r = mesh_zconv(h, h->coarse, d))) return r;
-    void* outc = (char*)h->fc + (size_t)d * h->n3c * h->prec;
-    if ((r = mesh_inv_yx(h, h->coarse, outc, d + 1, f2c))) return r;
+    if ((r = mesh_zconv(h, h->coarse, d, h->coarse.W))) return r;
+    if ((r = mesh_inv_yx(h, h->coarse, h->coarse.W, (void*)(fc + d * fcs), Nc, fc, Nc, fc + fcs, Nc,
+                         d == 2 ? f2c : nullptr)))
+      return r;
   }
-  // fine mesh, one component at a time, each followed by its kick
+  // fine mesh: the three force components stay resident for one fused
+  // kick: F0 in S (pitch N), F1 in W and F2 in R (pitch N+2, in place).  R
+  // holds the density transform until the last z convolution reads it.
   if ((r = cic_fine(h, &mass_f))) return r;
   if (nu && (r = fl_density_fine(h, a_mid))) return r;
   if ((r = mesh_fwd_xy(h, h->fine))) return r;
-  const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
-  const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
+  const int64_t Nf = h->fine.N;
+  const void* Ff[3] = {h->fine.S, h->fine.W, h->fine.R};
+  const int64_t pf[3] = {Nf, Nf + 2, Nf + 2};
   for (int d = 0; d < 3; d++) {
-    if ((r = mesh_zconv(h, h->fine, d))) return r;
-    if ((r = mesh_inv_yx(h, h->fine, h->fine.W, d + 1, f2f))) return r;
-    if ((r = kick_component(h, d, a_mid, dt, c_old, c_new, vmx + d, probe_f, probe_c))) return r;
-    if (nu_kick && (r = fl_gravity_fine(h, d, a_mid, dt))) return r;
+    void* buf = d == 2 ? h->fine.R : h->fine.W;
+    if ((r = mesh_zconv(h, h->fine, d, buf))) return r;
+    if ((r = mesh_inv_yx(h, h->fine, buf, (void*)Ff[d], pf[d], Ff[0], pf[0], Ff[1], pf[1], d == 2 ? f2f : nullptr)))
+      return r;
   }
+  const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
+  if ((r = kick_all(h, Ff, pf, a_mid, dt, c_old, c_new, vmx, probe_f, probe_c))) return r;
+  const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
+  for (int d = 0; nu_kick && d < 3; d++)
+    if ((r = fl_gravity_fine(h, d, Ff[d], pf[d], a_mid, dt))) return r;
   if (nu_kick && (r = fl_gravity_finish(h, a_mid, dt))) return r;
   uint32_t hv[5];
   HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 32, sizeof(hv), hipMemcpyDeviceToHost, h->stream));

@@ -29,7 +29,7 @@ struct Mesh {
   int prec = 4;          // sizeof(real)
   void* R = nullptr;     // real N*N*(N+2) padded rows; complex view N*N*(N/2+1)
   void* W = nullptr;     // complex work, same size as R
-  float* S = nullptr;    // f2 accumulator, N^3 float
+  void* S = nullptr;     // unpadded N^3 real mesh (fine: force component 0)
   void* K = nullptr;     // kernel [3][H][H][H], layout [d][ky'][kx][kz'], scaled 1/N^3
   void* tw = nullptr;    // N twiddles exp(-2 pi i m/N), type cpx<real>
   size_t bytes_R() const { return (size_t)N * N * (N + 2) * prec; }
@@ -228,16 +228,21 @@ int mesh_pass(cube_handle* h, Mesh& m, void* buf, int axis, bool inverse);    //
 int mesh_c2r(cube_handle* h, Mesh& m, const void* src, void* dst, int64_t opitch);
 void mesh_free(Mesh& m);
 int mesh_fwd_xy(cube_handle* h, Mesh& m);                  // R: X r2c then Y, in place
-int mesh_zconv(cube_handle* h, Mesh& m, int d);              // W = Zinv(i*K_d * Zfwd(R))
-int mesh_inv_yx(cube_handle* h, Mesh& m, void* out, int smode, uint32_t* f2max); // W: Y inv, X c2r -> out
+// dst (R or W) = Zinv(i*K_d * Zfwd(R))
+int mesh_zconv(cube_handle* h, Mesh& m, int d, void* dst);
+// buf (R or W): Y inverse in place, then X c2r into out (row pitch opitch);
+// with f2max, also max(F0^2+F1^2+F2^2) from the other two components
+int mesh_inv_yx(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0,
+                const void* F1, int64_t p1, uint32_t* f2max);
 int mesh_fft3_r2c(cube_handle* h, Mesh& m);                  // full forward on R (for kernel build)
 int fine_kernel_build(cube_handle* h, const float* table);
 int coarse_kernel_build(cube_handle* h, const float* table);
 // pm.hip
 int cic_fine(cube_handle* h, double* mass);
 int cic_coarse(cube_handle* h, double* mass);
-int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, double c_new,
-                   uint32_t* vmax_bits, double* probe_f, double* probe_c);
+// fine force components Ff[d] (row pitch pf[d]) and the coarse force fc
+int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
+             double c_new, uint32_t* vmax_bits, double* probe_f, double* probe_c);
 // pp.hip
 int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe);
 // drift.hip
@@ -245,7 +250,7 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
 // fluid.hip: PM coupling of the neutrino fluid
 int fl_density_coarse(cube_handle* h, float a_mid);
 int fl_density_fine(cube_handle* h, float a_mid);
-int fl_gravity_fine(cube_handle* h, int d, float a_mid, float dt);
+int fl_gravity_fine(cube_handle* h, int d, const void* F, int64_t pitch, float a_mid, float dt);
 int fl_gravity_finish(cube_handle* h, float a_mid, float dt);
 void fl_free(NuState& s);
 

@@ -183,12 +183,16 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_r2c(cpx<T>* data, const cpx<T>
 }
 
 // ------------------------------------------------------------------ X C2R
-// rows of N/2+1 complex (src) -> N reals at dst[row*opitch + x].  smode:
-// 0 none, 1 S=f*f, 2 S=S+f*f, 3 max(S+f*f) into *f2max (float bits).
+// rows of N/2+1 complex (src) -> N reals at dst[row*opitch + x] (src may be
+// dst: every row is read into LDS before it is written).  With f2max set,
+// the other two force components F0, F1 (row pitches p0, p1) are read at the
+// same nodes and max(F0^2 + F1^2 + f^2), summed in float in component
+// order, goes into *f2max as float bits.
 template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst, int64_t opitch,
                                                        const cpx<T>* __restrict__ tw, int64_t nrows,
-                                                       int smode, float* S, uint32_t* f2max) {
+                                                       const T* F0, int64_t p0, const T* F1, int64_t p1,
+                                                       uint32_t* f2max) {
   constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
   __shared__ cpx<T> s[H * LD];
   __shared__ cpx<T> xnyq[W];
@@ -233,20 +237,18 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst,
     if (row >= nrows) continue;
     cpx<T> z = s[m * LD + w];
     *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
-    if (smode) {
-      float f0 = (float)z.x, f1 = (float)z.y;
-      float* sp = S + row * N + 2 * m;
-      if (smode == 1) {
-        sp[0] = f0 * f0; sp[1] = f1 * f1;
-      } else if (smode == 2) {
-        sp[0] = sp[0] + f0 * f0; sp[1] = sp[1] + f1 * f1;
-      } else {
-        float a = sp[0] + f0 * f0, b = sp[1] + f1 * f1;
-        mymax = fmaxf(mymax, fmaxf(a, b));
-      }
+    if (f2max) {
+      const cpx<T> a = *reinterpret_cast<const cpx<T>*>(F0 + row * p0 + 2 * m);
+      const cpx<T> b = *reinterpret_cast<const cpx<T>*>(F1 + row * p1 + 2 * m);
+      float s0 = (float)a.x * (float)a.x, s1 = (float)a.y * (float)a.y;
+      s0 = fmaf((float)b.x, (float)b.x, s0);
+      s1 = fmaf((float)b.y, (float)b.y, s1);
+      s0 = fmaf((float)z.x, (float)z.x, s0);
+      s1 = fmaf((float)z.y, (float)z.y, s1);
+      mymax = fmaxf(mymax, fmaxf(s0, s1));
     }
   }
-  if (smode == 3) {
+  if (f2max) {
     for (int off = 32; off > 0; off >>= 1) mymax = fmaxf(mymax, __shfl_xor(mymax, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mymax;
     __syncthreads();
@@ -298,6 +300,7 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_strided(cpx<T>* data, const cpx<
 
 // ------------------------------------------------------ fused Z convolution
 // dst = Zinv( i*K_d . Zfwd(src) ) along z for columns (kx0.., ky = by).
+// dst may be src: a workgroup reads all its columns before writing them.
 // K is the imaginary part of the kernel transform over one octant,
 // [d][ky'][kx][kz'] with ky'=min(ky,N-ky), kz'=min(kz,N-kz); component d is
 // odd in its own axis and even in the other two.
@@ -394,49 +397,53 @@ int mesh_fft3_r2c(cube_handle* h, Mesh& m) {
 }
 
 template <typename T>
-static int zconv_t(cube_handle* h, Mesh& m, int d) {
+static int zconv_t(cube_handle* h, Mesh& m, int d, void* dst) {
   int logn = ilog2(m.N);
   int64_t N = m.N;
   int p1 = prof_begin(h, &m == &h->fine ? PS_FFT_ZCONV : PS_FFT_C);
   LOGN_SWITCH(logn, {
     constexpr int WS = pick_w<T>(L);
     k_fft_zconv<T, L><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(
-        (const cpx<T>*)m.R, (cpx<T>*)m.W, (const T*)m.K, d, (const cpx<T>*)m.tw);
+        (const cpx<T>*)m.R, (cpx<T>*)dst, (const T*)m.K, d, (const cpx<T>*)m.tw);
   });
   prof_end(h, p1);
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
-int mesh_zconv(cube_handle* h, Mesh& m, int d) {
-  return m.prec == 4 ? zconv_t<float>(h, m, d) : zconv_t<double>(h, m, d);
+int mesh_zconv(cube_handle* h, Mesh& m, int d, void* dst) {
+  if (dst != m.R && dst != m.W) return cube_fail(h, CUBE_ERR_ARG, "z convolution target must be R or W");
+  return m.prec == 4 ? zconv_t<float>(h, m, d, dst) : zconv_t<double>(h, m, d, dst);
 }
 
 template <typename T>
-static int inv_yx_t(cube_handle* h, Mesh& m, void* out, int64_t opitch, int smode, uint32_t* f2max) {
+static int inv_yx_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0,
+                    const void* F1, int64_t p1, uint32_t* f2max) {
   int logn = ilog2(m.N);
   int64_t N = m.N, nrows = N * N;
-  cpx<T>* c = (cpx<T>*)m.W;
+  cpx<T>* c = (cpx<T>*)buf;
   const cpx<T>* tw = (const cpx<T>*)m.tw;
   const bool fine = &m == &h->fine;
   LOGN_SWITCH(logn, {
     constexpr int WS = pick_w<T>(L);
-    int p1 = prof_begin(h, fine ? PS_FFT_YI : PS_FFT_C);
+    int pa = prof_begin(h, fine ? PS_FFT_YI : PS_FFT_C);
     k_fft_strided<T, L, 1, true><<<dim3((unsigned)((N / 2 + 1 + WS - 1) / WS), (unsigned)N), NT_FFT, 0, h->stream>>>(c, tw);
-    prof_end(h, p1);
+    prof_end(h, pa);
     constexpr int WX = pick_w<T>(L - 1);
-    int p2 = prof_begin(h, fine ? PS_FFT_XI : PS_FFT_C);
+    int pb = prof_begin(h, fine ? PS_FFT_XI : PS_FFT_C);
     k_fft_x_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
-        c, (T*)out, opitch, tw, nrows, smode, m.S, f2max);
-    prof_end(h, p2);
+        c, (T*)out, opitch, tw, nrows, (const T*)F0, p0, (const T*)F1, p1, f2max);
+    prof_end(h, pb);
   });
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
-// Y inverse on W then C2R into out (pitch N+2 when out==W, N otherwise).
-int mesh_inv_yx(cube_handle* h, Mesh& m, void* out, int smode, uint32_t* f2max) {
-  int64_t opitch = (out == m.W) ? m.N + 2 : m.N;
-  return m.prec == 4 ? inv_yx_t<float>(h, m, out, opitch, smode, f2max)
-                     : inv_yx_t<double>(h, m, out, opitch, smode, f2max);
+int mesh_inv_yx(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0,
+                const void* F1, int64_t p1, uint32_t* f2max) {
+  if (buf != m.R && buf != m.W) return cube_fail(h, CUBE_ERR_ARG, "inverse transform source must be R or W");
+  if (out == buf && opitch != m.N + 2) return cube_fail(h, CUBE_ERR_ARG, "in-place C2R needs pitch N+2");
+  if (f2max && (!F0 || !F1)) return cube_fail(h, CUBE_ERR_ARG, "force maximum needs both other components");
+  return m.prec == 4 ? inv_yx_t<float>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max)
+                     : inv_yx_t<double>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max);
 }
 
 // -------------------------------------------------------------- allocation
@@ -446,7 +453,7 @@ int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S, bool w
   int64_t H = N / 2 + 1;
   HIP_TRY(h, hipMalloc(&m.R, m.bytes_R()));
   HIP_TRY(h, hipMalloc(&m.W, m.bytes_R()));
-  if (with_S) HIP_TRY(h, hipMalloc(&m.S, sizeof(float) * N * N * N));
+  if (with_S) HIP_TRY(h, hipMalloc(&m.S, (size_t)prec * N * N * N));
   if (with_K) HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * H * H * prec));
   HIP_TRY(h, hipMalloc(&m.tw, (size_t)N * 2 * prec));
   std::vector<double> tw(2 * N);
@@ -665,7 +672,7 @@ static int pass_t(cube_handle* h, Mesh& m, void* buf, int axis, bool inv) {
 int mesh_pass(cube_handle* h, Mesh& m, void* buf, int axis, bool inverse) {
   return m.prec == 4 ? pass_t<float>(h, m, buf, axis, inverse) : pass_t<double>(h, m, buf, axis, inverse);
 }
-// C2R along x of a complex buffer into real rows of pitch opitch (no S epilogue).
+// C2R along x of a complex buffer into real rows of pitch opitch (no force maximum).
 template <typename T>
 static int c2r_t(cube_handle* h, Mesh& m, const void* src, void* dst, int64_t opitch) {
   int logn = ilog2(m.N);
@@ -673,7 +680,7 @@ static int c2r_t(cube_handle* h, Mesh& m, const void* src, void* dst, int64_t op
   LOGN_SWITCH(logn, {
     constexpr int WX = pick_w<T>(L - 1);
     k_fft_x_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
-        (const cpx<T>*)src, (T*)dst, opitch, (const cpx<T>*)m.tw, nrows, 0, nullptr, nullptr);
+        (const cpx<T>*)src, (T*)dst, opitch, (const cpx<T>*)m.tw, nrows, nullptr, 0, nullptr, 0, nullptr);
   });
   HIP_TRY(h, hipGetLastError());
   return 0;

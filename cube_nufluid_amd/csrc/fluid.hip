@@ -274,12 +274,12 @@ __global__ void k_fl_coarse_at_centres(const T* Fc, int ncc, int64_t nf, float a
 }
 
 // Gather of J_d and Q over one fluid's cells from an acceleration field on
-// fine-cell centres.  FINE: A is the fine force component (pitch nf+2) at
+// fine-cell centres (row pitch ap).  FINE: A is the fine force component at
 // nodes, the cell n taking the node n+1 on its right edge (pm.f90:163-165),
 // scaled by a_mid*dt/6/pi in float; else A already holds the acceleration.
 // accr = a/dx^3 (hydrogr.f90:142).  acc: [4][nc^3] doubles (J0, J1, J2, Q).
 template <typename T, bool FINE>
-__global__ void k_fl_grav_gather(const T* A, int64_t nf, int nc, float dx, int d, float a_mid, float dt, double* acc) {
+__global__ void k_fl_grav_gather(const T* A, int64_t ap, int64_t nf, int nc, float dx, int d, float a_mid, float dt, double* acc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n3 = (int64_t)nc * nc * nc;
   if (i >= n3) return;
@@ -323,7 +323,7 @@ __global__ void k_fl_grav_gather(const T* A, int64_t nf, int nc, float dx, int d
         const float dV = wx * wy * wz;
         const int s = FINE ? 1 : 0;  // fine force: node on the cell's right edge
         const int64_t ax = ((int64_t)mx + s + nf) % nf, ay = ((int64_t)my + s + nf) % nf, az = ((int64_t)mz + s + nf) % nf;
-        const float fv = (float)A[(az * nf + ay) * (nf + 2) + ax];
+        const float fv = (float)A[(az * nf + ay) * ap + ax];
         const double a = (double)(FINE ? fv * kf : fv) * inv3;
         J += (double)dV * a;
         Q += (double)dV * (1.0 - (double)dV) * a * a;
@@ -524,16 +524,16 @@ int fl_density_fine(cube_handle* h, float a_mid) {
 }
 
 // Fine-force component d (in fine.W) into the accumulators of every active fluid.
-int fl_gravity_fine(cube_handle* h, int d, float a_mid, float dt) {
+int fl_gravity_fine(cube_handle* h, int d, const void* F, int64_t pitch, float a_mid, float dt) {
   NuState& s = h->nu;
   for (int n = 0; n < 3; n++) {
     if (!(a_mid >= s.ai[n])) continue;
     const int64_t n3 = (int64_t)s.nc[n] * s.nc[n] * s.nc[n];
     if (h->prec == 4)
-      k_fl_grav_gather<float, true><<<nblk(n3), 256, 0, h->stream>>>((const float*)h->fine.W, h->nf, s.nc[n], s.dx[n], d, a_mid, dt,
-                                                                    s.acc[n]);
+      k_fl_grav_gather<float, true><<<nblk(n3), 256, 0, h->stream>>>((const float*)F, pitch, h->nf, s.nc[n], s.dx[n], d, a_mid,
+                                                                    dt, s.acc[n]);
     else
-      k_fl_grav_gather<double, true><<<nblk(n3), 256, 0, h->stream>>>((const double*)h->fine.W, h->nf, s.nc[n], s.dx[n], d, a_mid,
+      k_fl_grav_gather<double, true><<<nblk(n3), 256, 0, h->stream>>>((const double*)F, pitch, h->nf, s.nc[n], s.dx[n], d, a_mid,
                                                                      dt, s.acc[n]);
     HIP_TRY(h, hipGetLastError());
   }
@@ -572,7 +572,7 @@ int fl_gravity_finish(cube_handle* h, float a_mid, float dt) {
     for (int n = 0; n < 3; n++) {
       if (!(a_mid >= s.ai[n])) continue;
       const int64_t n3 = (int64_t)s.nc[n] * s.nc[n] * s.nc[n];
-      k_fl_grav_gather<float, false><<<nblk(n3), 256, 0, h->stream>>>(A, nf, s.nc[n], s.dx[n], d, a_mid, dt, s.acc[n]);
+      k_fl_grav_gather<float, false><<<nblk(n3), 256, 0, h->stream>>>(A, nf + 2, nf, s.nc[n], s.dx[n], d, a_mid, dt, s.acc[n]);
       HIP_TRY(h, hipGetLastError());
     }
   }

@@ -214,26 +214,32 @@ int cic_coarse(cube_handle* h, double* mass) {
 }
 
 // ------------------------------------------------------------------ kick
-// Component d of the fine kick (decode with sigma_old, encode with
-// sigma_new; pm.f90:145-154) followed by component d of the coarse kick
-// (decode/encode with sigma_new; pm.f90:353-363).  Components are
-// independent in both kicks, so running them per component reproduces the
-// reference's sequence.  Each tap adds (double)(F*k*w1*w2*w3) with
+// The fine kick (decode with sigma_old, encode with sigma_new;
+// pm.f90:145-154) followed by the coarse kick (decode/encode with
+// sigma_new; pm.f90:353-363), all three components per particle.  The
+// cell, the in-cell offsets and the 8+8 CIC weights are formed once per
+// particle; each component then gathers its own fine mesh Ff[d] (row pitch
+// pf[d]) and coarse mesh Fc[d].  Each tap adds (double)(F*k*w1*w2*w3) with
 // k = a_mid*dt/6/pi formed once in the mesh precision (the reference forms
 // F*a_mid*dt/6/pi per tap; the two agree to rounding).  With probe arrays
 // set, the kick is replaced by writing the interpolated forces.  Mesh
 // sizes are powers of two, so tap indices wrap with masks.
 template <typename T>
-__global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
-                                                  int64_t n3c, const float* __restrict__ vfield,
-                                                  const T* __restrict__ Ff, int64_t Nf,
-                                                  const T* __restrict__ Fc,
+struct KickMeshes {
+  const T* f[3];
+  int64_t pf[3];
+  const T* c;  // [3][nc^3]
+};
+
+template <typename T>
+__global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
+                                                  int64_t n3c, const float* __restrict__ vfield, KickMeshes<T> F, int64_t Nf,
                                                   float a_mid, float dt, double c_old, double c_new,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
                                                   const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
   __shared__ int64_t cs[KICK_CH + 1];
-  __shared__ float red[KICK_NT / 64];
+  __shared__ float red[3][KICK_NT / 64];
   const int64_t r0 = (int64_t)blockIdx.x * KICK_CH;
   for (int i = threadIdx.x; i <= KICK_CH; i += KICK_NT) {
     int64_t r = r0 + i;
@@ -242,9 +248,8 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __re
   __syncthreads();
   const int64_t p0 = cs[0], p1 = cs[KICK_CH];
   const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
-  const int64_t P = Nf + 2;  // fine rows are padded to Nf+2 reals
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
-  float vmax = 0.f;
+  float vmax[3] = {0.f, 0.f, 0.f};
   for (int64_t p = p0 + threadIdx.x; p < p1; p += KICK_NT) {
     int lo = 0, hi = KICK_CH;
     while (hi - lo > 1) {
@@ -276,67 +281,93 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, int d, const void* __re
     }
     // the 8 taps: bit 0 picks x, bit 1 y, bit 2 z (order of pm.f90:146-153)
     const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
-    T Fv[8], Fw[8], Cv[8], Cw[8];
+    uint32_t rf[8], xf[8], oc[8];
+    T Fw[8], Cw[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-      const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
-      Fv[q] = Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])];
+      rf[q] = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
+      xf[q] = ox ? f2[0] : f1[0];
       Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
-      const uint32_t rowc = (oz ? k2[2] : k1[2]) * Nc + (oy ? k2[1] : k1[1]);
-      Cv[q] = Fc[(int64_t)rowc * Nc + (ox ? k2[0] : k1[0])];
+      oc[q] = ((oz ? k2[2] : k1[2]) * Nc + (oy ? k2[1] : k1[1])) * Nc + (ox ? k2[0] : k1[0]);
       Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
-    if (probe_f) {
-      double sf = 0, sc = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T* Ff = F.f[d];
+      const int64_t P = F.pf[d];
+      const T* Fc = F.c + (size_t)d * n3c;
+      T Fv[8], Cv[8];
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        sf += (double)(Fv[q] * Fw[q]);
-        sc += (double)(Cv[q] * Cw[q]);
+        Fv[q] = Ff[(int64_t)rf[q] * P + xf[q]];
+        Cv[q] = Fc[oc[q]];
       }
-      probe_f[3 * p + d] = sf;
-      probe_c[3 * p + d] = sc;
-      continue;
+      if (probe_f) {
+        double sf = 0, sc = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          sf += (double)(Fv[q] * Fw[q]);
+          sc += (double)(Cv[q] * Cw[q]);
+        }
+        probe_f[3 * p + d] = sf;
+        probe_c[3 * p + d] = sc;
+        continue;
+      }
+      double v = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
+#pragma unroll
+      for (int q = 0; q < 8; q++) v = v + (double)(Fv[q] * kf * Fw[q]);
+      const int32_t qn = vencode(venc, kmax, g.izipv, c_new, v);
+      v = vdecode(vdt_new, qn, g.izipv);
+#pragma unroll
+      for (int q = 0; q < 8; q++) v = v + (double)(Cv[q] * kf * Cw[q]);
+      float av = (float)fabs(v + (double)vfield[r * 3 + d]);
+      vmax[d] = fmaxf(vmax[d], av);
+      store_v(vp, 3 * p + d, g.izipv, vencode(venc, kmax, g.izipv, c_new, v));
     }
-    double v = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
-#pragma unroll
-    for (int q = 0; q < 8; q++) v = v + (double)(Fv[q] * kf * Fw[q]);
-    const int32_t qn = vencode(venc, kmax, g.izipv, c_new, v);
-    v = vdecode(vdt_new, qn, g.izipv);
-#pragma unroll
-    for (int q = 0; q < 8; q++) v = v + (double)(Cv[q] * kf * Cw[q]);
-    float av = (float)fabs(v + (double)vfield[r * 3 + d]);
-    vmax = fmaxf(vmax, av);
-    store_v(vp, 3 * p + d, g.izipv, vencode(venc, kmax, g.izipv, c_new, v));
   }
   if (!probe_f) {
-    for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      float m = vmax[d];
+      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+      if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = m;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      float m = red[0];
-      for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[i]);
-      atomicMax(vmax_bits, __float_as_uint(m));
+    if (threadIdx.x < 3) {
+      const int d = threadIdx.x;
+      float m = red[d][0];
+      for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[d][i]);
+      atomicMax(vmax_bits + d, __float_as_uint(m));
     }
   }
 }
 
-int kick_component(cube_handle* h, int d, float a_mid, float dt, double c_old, double c_new,
-                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
+template <typename T>
+static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
+                        double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, double* probe_f,
+                        double* probe_c) {
   Geo g = geo_of(h);
   int64_t nb = (h->n3c + KICK_CH - 1) / KICK_CH;
-  const char* Fc = (const char*)h->fc + (size_t)d * h->n3c * h->prec;
+  KickMeshes<T> F;
+  for (int d = 0; d < 3; d++) {
+    F.f[d] = (const T*)Ff[d];
+    F.pf[d] = pf[d];
+  }
+  F.c = (const T*)h->fc;
+  k_kick<T><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N, a_mid, dt,
+                                                      c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+}
+
+int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
+             double c_new, uint32_t* vmax_bits, double* probe_f, double* probe_c) {
   const double* vo = vtab_for(h, c_old);
   const double* vn = vtab_for(h, c_new);
   int pr = prof_begin(h, PS_KICK);
   if (h->prec == 4)
-    k_kick<float><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
-                                                            (const float*)h->fine.W, h->fine.N, (const float*)Fc, a_mid, dt, c_old,
-                                                            c_new, vo, vn, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+    kick_launch<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
   else
-    k_kick<double><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, d, h->xp, h->vp, h->cstart, h->n3c, h->vfield,
-                                                             (const double*)h->fine.W, h->fine.N, (const double*)Fc, a_mid, dt,
-                                                             c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+    kick_launch<double>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   return 0;

@@ -110,9 +110,9 @@ extern "C" int cube_step(cube_handle* h, cube_clock* c) {
 extern "C" int64_t cube_footprint(const cube_params* p, int32_t ic_np_nc) {
   if (!p) return -1;
   const int64_t npm = p->np_max, nc = p->nc, n3c = nc * nc * nc, nf = 4 * nc, prec = p->precision;
-  auto mesh = [&](int64_t N) {
+  auto mesh = [&](int64_t N, bool with_S) {
     const int64_t H = N / 2 + 1;
-    return 2 * N * N * (N + 2) * prec + 4 * N * N * N + 3 * H * H * H * prec + 2 * N * prec;
+    return 2 * N * N * (N + 2) * prec + (with_S ? N * N * N * prec : 0) + 3 * H * H * H * prec + 2 * N * prec;
   };
   const int64_t nb2 = (nc / 2) * (nc / 2) * (nc / 2);
   int64_t b = 0;
@@ -121,7 +121,7 @@ extern "C" int64_t cube_footprint(const cube_params* p, int32_t ic_np_nc) {
   b += 8 * (2 * nb2 + npm / 256 + nb2 + 1) + 8 * nb2 + 20480;  // PP lists, totals
   b += 2 * (4 * n3c + 12 * n3c + 8 * (n3c + 1)) + 3 * n3c * prec + 8 * (std::max<int64_t>(n3c, 65536) + 16);
   b += 4 * 65536 + 8 * 131072 + 16 * 65536;  // codec tables
-  b += mesh(nf) + mesh(nc);
+  b += mesh(nf, true) + mesh(nc, false);
   const int64_t fine_R = nf * nf * (nf + 2) * prec;
   if (fine_R < 24 * npm) b += 24 * npm;  // drift scratch outside the fine work mesh
   if (ic_np_nc > 0) {  // Zel'dovich generator's temporary mesh

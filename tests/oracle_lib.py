@@ -169,6 +169,26 @@ class Oracle:
         self.lib.orc_kernels(self.o, p(kf), p(kc))
         return kf, kc
 
+    # neutrino fluid
+    def nu_setup(self, f_neu, z_i_nu, gamma, weights, dxs, ais, ncs, ncbs, fluids):
+        fp = np.array([f_neu, z_i_nu, gamma, *weights, *dxs, *ais], dtype=np.float32)
+        ip = np.array([*ncs, *ncbs], dtype=np.int32)
+        self.lib.orc_nu_setup(self.o, p(fp), p(ip))
+        self.nu_ncs = list(ncs)
+        for n, f in enumerate(fluids):
+            self.lib.orc_nu_set(self.o, n, p(np.ascontiguousarray(f, dtype=np.float32)))
+
+    def nu_get(self, n):
+        nc = self.nu_ncs[n]
+        f = np.empty((5, nc, nc, nc), dtype=np.float32)
+        self.lib.orc_nu_get(self.o, n, p(f))
+        return f
+
+    def nu_evolve(self, dt, direction, a, dt_neu=1000.0):
+        d = np.array([dt_neu], dtype=np.float32)
+        self._chk(self.lib.orc_nu_evolve(self.o, dt, direction, a, p(d)))
+        return float(d[0])
+
 
 def expansion(a, dt, om, ol, wde=-1.0):
     lib = load()
@@ -194,25 +214,6 @@ def fft_r2c(r):
     return re + 1j * im.astype(np.float64)
 
 
-    # neutrino fluid
-    def nu_setup(self, f_neu, z_i_nu, gamma, weights, dxs, ais, ncs, ncbs, fluids):
-        fp = np.array([f_neu, z_i_nu, gamma, *weights, *dxs, *ais], dtype=np.float32)
-        ip = np.array([*ncs, *ncbs], dtype=np.int32)
-        self.lib.orc_nu_setup(self.o, p(fp), p(ip))
-        self.nu_ncs = list(ncs)
-        for n, f in enumerate(fluids):
-            self.lib.orc_nu_set(self.o, n, p(np.ascontiguousarray(f, dtype=np.float32)))
-
-    def nu_get(self, n):
-        nc = self.nu_ncs[n]
-        f = np.empty((5, nc, nc, nc), dtype=np.float32)
-        self.lib.orc_nu_get(self.o, n, p(f))
-        return f
-
-    def nu_evolve(self, dt, direction, a, dt_neu=1000.0):
-        d = np.array([dt_neu], dtype=np.float32)
-        self._chk(self.lib.orc_nu_evolve(self.o, dt, direction, a, p(d)))
-        return float(d[0])
 
 
 def fluid_evolve(fld, ncb, gamma, dt, dx, direction, dtg=1000.0):
