@@ -6,9 +6,14 @@ region holds exactly --steps steps between device synchronisations; inputs
 are resident in HBM before it starts.  Prints one JSON line.
 
 Extra fields:
-  roofline     -- the kernel with the largest share of step time: algorithmic
-                  bytes per launch / mean launch time (HIP events on the
-                  library's stream), against MI355X's 8 TB/s;
+  roofline     -- the HBM-bound kernel class with the largest share of step
+                  time: algorithmic bytes per launch / mean launch time (HIP
+                  events on the library's stream), against MI355X's 8 TB/s;
+                  `traffic` is the measured HBM bytes per launch from the
+                  rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/ when
+                  that summary covers the same kernel and workload;
+  pp           -- the PP force (FP32 VALU bound): pair interactions per
+                  second and the FP32 rate at 30 flops per pair;
   cpu_baseline -- the CPU oracle (a C restatement of CUBE's step, OpenMP on
                   the host cores) timed on a bounded sample of the same
                   workload: the first tiles of the same box.
@@ -53,11 +58,47 @@ def algorithmic_bytes(slot, cfg, npart):
         "fft_y_fwd": 8 * mf,
         "fft_zconv": 8.5 * mf,
         "fft_y_inv": 8 * mf,
-        "fft_x_c2r": 8 * mf + 16.0 / 3 * mf,
-        "kick": n * (bx + 2 * cfg.izipv) + 4 * mf + 20 * mc,
+        # the last of the three inverses also reads the other two components
+        "fft_x_c2r": 8 * mf + 8.0 / 3 * mf,
+        # xp, vp read + write; three fine and three coarse force meshes,
+        # vfield and cell offsets
+        "kick": n * (bx + 2 * bv) + 12 * mf + 32 * mc,
         "fft_coarse": None,
     }
     return table.get(slot)
+
+
+# Slots bound by FP32 VALU work rather than HBM (reported separately).
+VALU_SLOTS = ("pp",)
+
+
+def read_traffic(kernel_slot, workload):
+    """HBM bytes per launch for `kernel_slot` from profiles/traffic.txt
+    (lines 'slot workload fetch_bytes write_bytes', already scaled by the
+    copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] == kernel_slot and f[1] == workload:
+            return float(f[2]) + float(f[3])
+    return None
+
+
+def copy_calibration():
+    """Three plain 4 GiB device copies (torch copy kernel): under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE they give the counters' reading
+    for a known byte count."""
+    import torch
+    src = torch.zeros(1 << 30, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy 4 GiB: %.3f ms" % ((time.perf_counter() - t) * 1e3))
 
 
 def make_state(cfg, seed, z):
@@ -138,6 +179,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
     ap.add_argument("--ens-requests", type=int, default=16)
     ap.add_argument("--ens-rate", type=float, default=6.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--copy-calib", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--cpu-tiles", type=int, default=0,
                     help="tiles in the CPU sample (0: one tile when tiles hold >4M particles, else one per thread)")
     args = ap.parse_args()
@@ -146,6 +188,9 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calib:
+        copy_calibration()
+        return
     import torch
     from cube_nufluid_amd.params import CONFIGS
     from cube_nufluid_amd.driver import Cube
@@ -186,14 +231,24 @@ def main():
     if cfg.pp_on:
         lp = cube.last_pp
         log("pp: npairs %d (%.1f per particle), dense blocks %d" % (lp.npairs, lp.npairs / npart, lp.dense_blocks))
-    cands = [s for s in range(ns) if cnt[s] and algorithmic_bytes(names[s], cfg, npart) is not None]
+    cands = [s for s in range(ns)
+             if cnt[s] and names[s] not in VALU_SLOTS and algorithmic_bytes(names[s], cfg, npart) is not None]
     dom = max(cands, key=lambda s: ms[s])
     avg_ms = ms[dom] / cnt[dom]
     abytes = algorithmic_bytes(names[dom], cfg, npart)
     achieved = abytes / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(names[dom], args.config),
                 "algorithmic_bytes_per_launch": abytes, "mean_launch_ms": round(avg_ms, 4)}
+    pp_info = None
+    if cfg.pp_on and "pp" in names and cnt[names.index("pp")]:
+        s_pp = names.index("pp")
+        pp_s = ms[s_pp] / cnt[s_pp] * 1e-3
+        pairs = float(cube.last_pp.npairs)
+        pp_info = {"bound": "fp32 valu", "pairs_per_step": pairs, "mean_launch_ms": round(pp_s * 1e3, 4),
+                   "pairs_per_s": pairs / pp_s, "achieved_tflops": round(30 * pairs / pp_s / 1e12, 3),
+                   "peak_tflops": FP32_PEAK_TFLOPS, "flops_per_pair": 30}
     value = npart * args.steps / wall
     out = {
         "metric": "particle-steps/s (PM+PP, %d^3 on %d^3 mesh)" % (round(npart ** (1 / 3)), cfg.nf),
@@ -212,6 +267,7 @@ def main():
                    "box_mpc_h": cfg.box, "z_start": z0, "izipx": cfg.izipx, "izipv": cfg.izipv,
                    "pp": bool(cfg.pp_on), "pp_range": cfg.pp_range, "parallelism": "single GPU"},
         "roofline": roofline,
+        "pp": pp_info,
         "per_kernel_ms": per_kernel,
     }
     st_cpu = None

@@ -134,6 +134,15 @@ struct Geo {  // box geometry passed by value to kernels
   int32_t lnt, lnnt;  // log2 of nt and nnt when they are powers of two, else -1
 };
 
+// Workgroups go to the 8 XCDs round-robin by linear id and each XCD has its
+// own L2.  xcd_remap gives hardware block L of T a logical id such that
+// every XCD runs one contiguous range of logical ids: blocks that share
+// cache lines then run on the same XCD at about the same time.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t L, uint32_t T) {
+  const uint32_t q = T >> 3, r = T & 7, x = L & 7, sl = L >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + sl;
+}
+
 // Cell indices stay below 2^27 (nc <= 512), so index arithmetic is 32-bit;
 // nc is a power of two and so, in the usual layouts, are nt and nnt.
 __host__ __device__ __forceinline__ int32_t wrapc(int32_t a, int32_t n) { return a & (n - 1); }

@@ -122,14 +122,6 @@ __device__ __forceinline__ void lds_fft(cpx<T>* s, const cpx<T>* tw, int tws) {
   FftPasses<T, N, W, LD, INV, 1>::run(s, tw, tws);
 }
 
-// Workgroups go to the 8 XCDs round-robin by linear id and each XCD has its
-// own L2.  Logical block ids are assigned so that every XCD runs one
-// contiguous range: blocks that share cache lines (neighbouring columns of a
-// strided pass) then run on the same XCD at about the same time.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t L, uint32_t T) {
-  const uint32_t q = T >> 3, r = T & 7, x = L & 7, sl = L >> 3;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + sl;
-}
 __device__ __forceinline__ void grid_block_xy(int& bx, int& by) {
   const uint32_t L = blockIdx.x + gridDim.x * blockIdx.y, T = gridDim.x * gridDim.y;
   const uint32_t lg = xcd_remap(L, T);
@@ -191,18 +183,30 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_r2c(cpx<T>* data, const cpx<T>
 template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst, int64_t opitch,
                                                        const cpx<T>* __restrict__ tw, int64_t nrows,
-                                                       const T* F0, int64_t p0, const T* F1, int64_t p1,
-                                                       uint32_t* f2max) {
+                                                       const T* __restrict__ F0, int64_t p0,
+                                                       const T* __restrict__ F1, int64_t p1, uint32_t* f2max) {
   constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
   __shared__ cpx<T> s[H * LD];
   __shared__ cpx<T> xnyq[W];
   __shared__ float red[NT_FFT / 64];
   const int64_t r0 = (int64_t)blockIdx.x * W;
-  for (int idx = threadIdx.x; idx < (H + 1) * W; idx += NT_FFT) {
-    int w = idx / (H + 1), k = idx % (H + 1);
-    int64_t row = r0 + w;
-    cpx<T> v = row < nrows ? src[row * (H + 1) + k] : cpx<T>{0, 0};
-    if (k < H) s[k * LD + w] = v; else xnyq[w] = v;
+  {
+    // all loads of a thread are issued before its LDS stores
+    constexpr int PER = ((H + 1) * W + NT_FFT - 1) / NT_FFT;
+    cpx<T> r[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = threadIdx.x + q * NT_FFT, w = idx / (H + 1), k = idx % (H + 1);
+      const int64_t row = r0 + w;
+      r[q] = (idx < (H + 1) * W && row < nrows) ? src[row * (H + 1) + k] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = threadIdx.x + q * NT_FFT, w = idx / (H + 1), k = idx % (H + 1);
+      if (idx < (H + 1) * W) {
+        if (k < H) s[k * LD + w] = r[q]; else xnyq[w] = r[q];
+      }
+    }
   }
   __syncthreads();
   // Z[k] = (X[k] + conj X[H-k]) + i exp(+2 pi i k/N) (X[k] - conj X[H-k]),
@@ -231,21 +235,39 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst,
   __syncthreads();
   lds_fft<T, H, W, true, LD>(s, tw, 2);
   float mymax = 0.f;
-  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
-    int w = idx / H, m = idx % H;
-    int64_t row = r0 + w;
-    if (row >= nrows) continue;
-    cpx<T> z = s[m * LD + w];
-    *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
-    if (f2max) {
-      const cpx<T> a = *reinterpret_cast<const cpx<T>*>(F0 + row * p0 + 2 * m);
-      const cpx<T> b = *reinterpret_cast<const cpx<T>*>(F1 + row * p1 + 2 * m);
-      float s0 = (float)a.x * (float)a.x, s1 = (float)a.y * (float)a.y;
-      s0 = fmaf((float)b.x, (float)b.x, s0);
-      s1 = fmaf((float)b.y, (float)b.y, s1);
-      s0 = fmaf((float)z.x, (float)z.x, s0);
-      s1 = fmaf((float)z.y, (float)z.y, s1);
-      mymax = fmaxf(mymax, fmaxf(s0, s1));
+  if (!f2max) {
+    for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
+      int w = idx / H, m = idx % H;
+      int64_t row = r0 + w;
+      if (row < nrows) *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = s[m * LD + w];
+    }
+  } else {
+    // groups of FG nodes: their F0, F1 loads are issued together
+    constexpr int PER = H * W / NT_FFT, FG = PER < 8 ? PER : 8;
+    static_assert(PER % FG == 0, "whole groups");
+    for (int q0 = 0; q0 < PER; q0 += FG) {
+      cpx<T> a[FG], b[FG];
+#pragma unroll
+      for (int q = 0; q < FG; q++) {
+        const int idx = threadIdx.x + (q0 + q) * NT_FFT, w = idx / H, m = idx % H;
+        const int64_t row = r0 + w < nrows ? r0 + w : nrows - 1;
+        a[q] = *reinterpret_cast<const cpx<T>*>(F0 + row * p0 + 2 * m);
+        b[q] = *reinterpret_cast<const cpx<T>*>(F1 + row * p1 + 2 * m);
+      }
+#pragma unroll
+      for (int q = 0; q < FG; q++) {
+        const int idx = threadIdx.x + (q0 + q) * NT_FFT, w = idx / H, m = idx % H;
+        const int64_t row = r0 + w;
+        if (row >= nrows) continue;
+        const cpx<T> z = s[m * LD + w];
+        *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
+        float s0 = (float)a[q].x * (float)a[q].x, s1 = (float)a[q].y * (float)a[q].y;
+        s0 = fmaf((float)b[q].x, (float)b[q].x, s0);
+        s1 = fmaf((float)b[q].y, (float)b[q].y, s1);
+        s0 = fmaf((float)z.x, (float)z.x, s0);
+        s1 = fmaf((float)z.y, (float)z.y, s1);
+        mymax = fmaxf(mymax, fmaxf(s0, s1));
+      }
     }
   }
   if (f2max) {
@@ -315,18 +337,27 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
   const int ky = by;
   const int64_t stride = (int64_t)N * KX;
   const int64_t base = (int64_t)ky * KX + kx0;
-  col_load<T, N, W>(s, src + base, stride, KX - kx0);
-  lds_fft<T, N, W, false>(s, tw, 1);
+  // the kernel values of this thread's elements are loaded before the
+  // forward transform, so their latency overlaps it
+  constexpr int PER = N * W / NT_FFT;
   const int kyp = ky <= N / 2 ? ky : N - ky;
   const T sy = (d == 1 && ky > N / 2) ? (T)-1 : (T)1;
-  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
-    int w = idx % W, n = idx / W;
-    int kx = kx0 + w;
-    if (kx >= KX) continue;
-    int kzp = n <= N / 2 ? n : N - n;
-    T sz = (d == 2 && n > N / 2) ? (T)-1 : (T)1;
-    T k = K[(((int64_t)d * H + kyp) * KX + kx) * H + kzp] * sy * sz;
-    cpx<T> a = s[idx];
+  T kv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int idx = threadIdx.x + q * NT_FFT, w = idx % W, n = idx / W;
+    const int kx = kx0 + w < KX ? kx0 + w : KX - 1;
+    const int kzp = n <= N / 2 ? n : N - n;
+    kv[q] = K[(((int64_t)d * H + kyp) * KX + kx) * H + kzp];
+  }
+  col_load<T, N, W>(s, src + base, stride, KX - kx0);
+  lds_fft<T, N, W, false>(s, tw, 1);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int idx = threadIdx.x + q * NT_FFT, n = idx / W;
+    const T sz = (d == 2 && n > N / 2) ? (T)-1 : (T)1;
+    const T k = kv[q] * sy * sz;
+    const cpx<T> a = s[idx];
     s[idx] = cpx<T>{-a.y * k, a.x * k};
   }
   __syncthreads();

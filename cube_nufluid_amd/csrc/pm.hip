@@ -11,7 +11,6 @@
 
 #define CIC_NT 256
 #define KICK_NT 256
-#define KICK_CH 256
 
 // --------------------------------------------------------------- helpers
 // Load cstart for the cells of a (BX+2*lo)^3 region around block origin c0
@@ -216,14 +215,22 @@ int cic_coarse(cube_handle* h, double* mass) {
 // ------------------------------------------------------------------ kick
 // The fine kick (decode with sigma_old, encode with sigma_new;
 // pm.f90:145-154) followed by the coarse kick (decode/encode with
-// sigma_new; pm.f90:353-363), all three components per particle.  The
-// cell, the in-cell offsets and the 8+8 CIC weights are formed once per
-// particle; each component then gathers its own fine mesh Ff[d] (row pitch
-// pf[d]) and coarse mesh Fc[d].  Each tap adds (double)(F*k*w1*w2*w3) with
-// k = a_mid*dt/6/pi formed once in the mesh precision (the reference forms
-// F*a_mid*dt/6/pi per tap; the two agree to rounding).  With probe arrays
-// set, the kick is replaced by writing the interpolated forces.  Mesh
-// sizes are powers of two, so tap indices wrap with masks.
+// sigma_new; pm.f90:353-363), all three components per particle.
+//
+// A workgroup takes BX consecutive coarse cells of one x-row of a tile.  It
+// first stages the force nodes those cells' particles can touch in LDS:
+// fine nodes 4*c0 + [0, 4*BX] x [0, 4] x [0, 4] of the three fine
+// components (rows loaded coalesced along x).  The
+// particles then gather their 8 fine taps per component from LDS and their
+// 8 coarse taps from the (L2-resident) coarse meshes.  A float
+// in-cell coordinate can round up onto the next cell's far node; those
+// rare particles read their fine taps from HBM instead.
+//
+// Each tap adds (double)(F*k*w1*w2*w3) with k = a_mid*dt/6/pi formed once
+// in the mesh precision (the reference forms F*a_mid*dt/6/pi per tap; the
+// two agree to rounding).  With probe arrays set, the kick is replaced by
+// writing the interpolated forces.  Mesh sizes are powers of two, so node
+// indices wrap with masks.
 template <typename T>
 struct KickMeshes {
   const T* f[3];
@@ -231,36 +238,72 @@ struct KickMeshes {
   const T* c;  // [3][nc^3]
 };
 
-template <typename T>
+template <typename T, int BX>
 __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                                   int64_t n3c, const float* __restrict__ vfield, KickMeshes<T> F, int64_t Nf,
                                                   float a_mid, float dt, double c_old, double c_new,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
                                                   const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
-  __shared__ int64_t cs[KICK_CH + 1];
+  constexpr int SX = 4 * BX + 1, FROW = 25 * SX, NFS = 3 * FROW;  // fine slab [3][5][5][SX]
+  __shared__ T sF[NFS];
+  __shared__ int64_t cs[BX + 1];
   __shared__ float red[3][KICK_NT / 64];
-  const int64_t r0 = (int64_t)blockIdx.x * KICK_CH;
-  for (int i = threadIdx.x; i <= KICK_CH; i += KICK_NT) {
-    int64_t r = r0 + i;
-    cs[i] = cstart[r < n3c ? r : n3c];
+  // Block order: BX-cell chunks along x fastest, then (y, z) in groups of
+  // 4 x 4 rows, and each XCD runs a contiguous range of that order, so the
+  // blocks that share fine rows (y and z neighbours) run close together on
+  // one L2.
+  const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t nxb = (uint32_t)g.nc / BX, M = L / nxb, grp = M >> 4, nq = (uint32_t)g.nc >> 2;
+  int32_t c0[3] = {(int32_t)((L % nxb) * BX), (int32_t)((grp % nq) * 4 + (M & 3)), (int32_t)((grp / nq) * 4 + ((M >> 2) & 3))};
+  const int64_t r0 = rank_of(g, c0[0], c0[1], c0[2]);
+  const int tid = threadIdx.x;
+  const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
+  for (int i = tid; i <= BX; i += KICK_NT) cs[i] = cstart[r0 + i];
+  // stage the fine slab: 75 rows of SX nodes, a wavefront per row and a
+  // lane per node, all of a lane's loads issued before its LDS stores
+  {
+    constexpr int NW = KICK_NT / 64, RPW = (75 + NW - 1) / NW, XC = (SX + 63) / 64;
+    const int lane = tid & 63, wv = tid >> 6;
+    T v[RPW][XC];
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+      const int row = wv + k * NW;  // row = (d*5 + z)*5 + y
+      const int y = row % 5, z = (row / 5) % 5, d = row / 25;
+      const T* f = d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]);
+      const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
+      const uint32_t gy = (uint32_t)(4 * c0[1] + y) & mf, gz = (uint32_t)(4 * c0[2] + z) & mf;
+      const T* rowp = f + (int64_t)(gz * (uint32_t)Nf + gy) * P;
+#pragma unroll
+      for (int j = 0; j < XC; j++) {
+        const int x = lane + 64 * j;
+        v[k][j] = (row < 75 && x < SX) ? rowp[(uint32_t)(4 * c0[0] + x) & mf] : (T)0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+      const int row = wv + k * NW;
+#pragma unroll
+      for (int j = 0; j < XC; j++) {
+        const int x = lane + 64 * j;
+        if (row < 75 && x < SX) sF[row * SX + x] = v[k][j];
+      }
+    }
   }
   __syncthreads();
-  const int64_t p0 = cs[0], p1 = cs[KICK_CH];
-  const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
+  const int64_t p0 = cs[0], p1 = cs[BX];
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax[3] = {0.f, 0.f, 0.f};
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += KICK_NT) {
-    int lo = 0, hi = KICK_CH;
+  for (int64_t p = p0 + tid; p < p1; p += KICK_NT) {
+    int lo = 0, hi = BX;
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
       if (cs[mid] <= p) lo = mid; else hi = mid;
     }
-    const int64_t r = r0 + lo;
-    int32_t c[3];
-    cell_of_rank(g, r, c);
+    const int32_t c[3] = {c0[0] + lo, c0[1], c0[2]};
     T dx1[3], dx2[3], cx1[3], cx2[3];
     uint32_t f1[3], f2[3], k1[3], k2[3];
+    int lf[3];
     for (int e = 0; e < 3; e++) {
       const int il = (g.lnt >= 0 ? (c[e] & (g.nt - 1)) : c[e] % g.nt) + 1;
       const int torg = c[e] - (il - 1);
@@ -272,46 +315,56 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       // a tile-local float coordinate may round up to 4*nt; the mask wraps it
       f1[e] = (uint32_t)(CUBE_NCELL * torg + a1 - 1) & mf;
       f2[e] = (f1[e] + 1) & mf;
+      lf[e] = CUBE_NCELL * (c[e] - c0[e]) + (a1 - 1 - CUBE_NCELL * (il - 1));  // 0..4*BX (x), 0..4 (y, z)
       const T tc = (T)(((double)(torg + il - 1) + xo) - 0.5);
-      const int b1 = (int)floor(tc) + 1;  // 1-based global node, 0..nc (0 and nc+1 are halo)
+      const int b1 = (int)floor(tc) + 1;  // 1-based global node: b1-1 is c-1 or c
       cx1[e] = (T)b1 - tc;
       cx2[e] = 1 - cx1[e];
       k1[e] = (uint32_t)(b1 - 1) & mc;
       k2[e] = (uint32_t)b1 & mc;
     }
+    const bool in_slab = lf[0] < 4 * BX && lf[1] < 4 && lf[2] < 4;
     // the 8 taps: bit 0 picks x, bit 1 y, bit 2 z (order of pm.f90:146-153)
     const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
-    uint32_t rf[8], xf[8], oc[8];
+    int sf[8];
+    uint32_t oc[8];
     T Fw[8], Cw[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-      rf[q] = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
-      xf[q] = ox ? f2[0] : f1[0];
-      Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
+      sf[q] = ((lf[2] + oz) * 5 + lf[1] + oy) * SX + lf[0] + ox;
       oc[q] = ((oz ? k2[2] : k1[2]) * Nc + (oy ? k2[1] : k1[1])) * Nc + (ox ? k2[0] : k1[0]);
+      Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
       Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      const T* Ff = F.f[d];
-      const int64_t P = F.pf[d];
-      const T* Fc = F.c + (size_t)d * n3c;
       T Fv[8], Cv[8];
+      if (in_slab) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        Fv[q] = Ff[(int64_t)rf[q] * P + xf[q]];
-        Cv[q] = Fc[oc[q]];
-      }
-      if (probe_f) {
-        double sf = 0, sc = 0;
+        for (int q = 0; q < 8; q++) Fv[q] = sF[d * FROW + sf[q]];
+      } else {
+        const T* Ff = F.f[d];
+        const int64_t P = F.pf[d];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-          sf += (double)(Fv[q] * Fw[q]);
-          sc += (double)(Cv[q] * Cw[q]);
+          const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+          const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
+          Fv[q] = Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])];
         }
-        probe_f[3 * p + d] = sf;
-        probe_c[3 * p + d] = sc;
+      }
+      const T* Fc = F.c + (size_t)d * n3c;
+#pragma unroll
+      for (int q = 0; q < 8; q++) Cv[q] = Fc[oc[q]];
+      if (probe_f) {
+        double sfv = 0, scv = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          sfv += (double)(Fv[q] * Fw[q]);
+          scv += (double)(Cv[q] * Cw[q]);
+        }
+        probe_f[3 * p + d] = sfv;
+        probe_c[3 * p + d] = scv;
         continue;
       }
       double v = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
@@ -321,7 +374,7 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       v = vdecode(vdt_new, qn, g.izipv);
 #pragma unroll
       for (int q = 0; q < 8; q++) v = v + (double)(Cv[q] * kf * Cw[q]);
-      float av = (float)fabs(v + (double)vfield[r * 3 + d]);
+      float av = (float)fabs(v + (double)vfield[(r0 + lo) * 3 + d]);
       vmax[d] = fmaxf(vmax[d], av);
       store_v(vp, 3 * p + d, g.izipv, vencode(venc, kmax, g.izipv, c_new, v));
     }
@@ -331,32 +384,46 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     for (int d = 0; d < 3; d++) {
       float m = vmax[d];
       for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-      if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = m;
+      if ((tid & 63) == 0) red[d][tid >> 6] = m;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
-      const int d = threadIdx.x;
-      float m = red[d][0];
-      for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[d][i]);
-      atomicMax(vmax_bits + d, __float_as_uint(m));
+    if (tid < 3) {
+      float m = red[tid][0];
+      for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[tid][i]);
+      atomicMax(vmax_bits + tid, __float_as_uint(m));
     }
   }
 }
 
-template <typename T>
+template <typename T, int BX>
 static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
                         double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, double* probe_f,
                         double* probe_c) {
   Geo g = geo_of(h);
-  int64_t nb = (h->n3c + KICK_CH - 1) / KICK_CH;
   KickMeshes<T> F;
   for (int d = 0; d < 3; d++) {
     F.f[d] = (const T*)Ff[d];
     F.pf[d] = pf[d];
   }
   F.c = (const T*)h->fc;
-  k_kick<T><<<(unsigned)nb, KICK_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N, a_mid, dt,
-                                                      c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits, probe_f, probe_c);
+  k_kick<T, BX><<<(unsigned)(h->n3c / BX), KICK_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
+                                                                    a_mid, dt, c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits,
+                                                                    probe_f, probe_c);
+}
+
+template <typename T>
+static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
+                     double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, double* probe_f,
+                     double* probe_c) {
+  if (h->nt % 32 == 0)
+    kick_launch<T, 32>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+  else if (h->nt % 16 == 0)
+    kick_launch<T, 16>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+  else if (h->nt % 4 == 0)
+    kick_launch<T, 4>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+  else
+    return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
+  return 0;
 }
 
 int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
@@ -364,11 +431,10 @@ int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float
   const double* vo = vtab_for(h, c_old);
   const double* vn = vtab_for(h, c_new);
   int pr = prof_begin(h, PS_KICK);
-  if (h->prec == 4)
-    kick_launch<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
-  else
-    kick_launch<double>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+  int r = h->prec == 4 ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c)
+                       : kick_pick<double>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
   prof_end(h, pr);
+  if (r) return r;
   HIP_TRY(h, hipGetLastError());
   return 0;
 }

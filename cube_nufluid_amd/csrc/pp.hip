@@ -467,6 +467,8 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
   __shared__ float redf[NT / 64];
   __shared__ unsigned long long redn[NT / 64];
   __shared__ uint8_t owner[PF_OWN];
+  __shared__ int32_t inoff[PB * PB * PB + 1];
+  __shared__ uint8_t ia[PB * PB * PB];
   int c0x, c0y, c0z;
   pf_origin<PB>(g, blockIdx.x, c0x, c0y, c0z);
   const int pr = c.pr, W = G::FB + 2 * pr, W3 = W * W * W;
@@ -537,13 +539,39 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
     }
   }
   __syncthreads();
+  // targets: the particles of the workgroup's own PB^3 coarse cells, dealt
+  // out densely to the threads (inner cell ib <-> region cell ia[ib])
+  if (threadIdx.x < 64) {
+    const int ib = threadIdx.x;
+    int32_t cnt = 0;
+    if (ib < PB * PB * PB) {
+      const int a = ((ib / (PB * PB)) + 1) * G::SPAN * G::SPAN + ((ib / PB) % PB + 1) * G::SPAN + (ib % PB) + 1;
+      ia[ib] = (uint8_t)a;
+      cnt = off[a + 1] - off[a];
+    }
+    int32_t incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(incl, o);
+      if (ib >= o) incl += v;
+    }
+    if (ib < PB * PB * PB) inoff[ib + 1] = incl;
+    if (ib == 0) inoff[0] = 0;
+  }
+  __syncthreads();
+  const int32_t ntarget = inoff[PB * PB * PB];
   float f2loc = 0.f;
   unsigned long long npl = 0;
   int32_t ninner = 0;
-  for (int32_t t = threadIdx.x; t < ncand; t += NT) {
-    const PPent ti = ent[t];
-    const int fx = (int)floorf(ti.x), fy = (int)floorf(ti.y), fz = (int)floorf(ti.z);
-    if ((unsigned)fx >= (unsigned)G::FB || (unsigned)fy >= (unsigned)G::FB || (unsigned)fz >= (unsigned)G::FB) continue;
+  for (int32_t k = threadIdx.x; k < ntarget; k += NT) {
+    int lo = 0, hi = PB * PB * PB;
+    while (hi - lo > 1) {
+      const int m = (lo + hi) >> 1;
+      if (inoff[m] <= k) lo = m; else hi = m;
+    }
+    const int a = ia[lo];
+    PPent ti;
+    int wi_unused;
+    (void)pf_cand<PB>(g, xp, beg, off, off[a] + (k - inoff[lo]), pr, W, wi_unused, ti, own);
     ninner++;
     int32_t vc[3] = {0, 0, 0};
     if (!probe)

@@ -200,23 +200,33 @@ def test_gpu_pm_with_fluid_matches_oracle():
     cube.particle_mesh()
     orc.pm(float(a_mid), float(dt))
     # The two sides' forces agree to FP32 FFT rounding (about 1e-5 relative
-    # RMS), so the gravity increments agree to that order; an index, pitch
-    # or sign error in the coupling shows up at order 1.
+    # RMS), so the momentum increments agree to that order; an index, pitch
+    # or sign error in the coupling shows up at order 1.  The oracle adds the
+    # energy impulses one by one in float, as the reference does (up to
+    # 4 (2 dx)^3 of them per cell: three fine components and the coarse
+    # pass); the GPU sums them in closed form in double.  Energies therefore
+    # agree to that many float roundings of E.
     for n in range(3):
         g, o = cube.nu_get(n), orc.nu_get(n)
         scale = np.abs(o).reshape(5, -1).max(1)
         err = (np.abs(g - o).reshape(5, -1).max(1) / scale)
         dg, do = (g - fl[n]).reshape(5, -1), (o - fl[n]).reshape(5, -1)
-        inc = np.sqrt(((dg[1:] - do[1:]) ** 2).mean(1) / (do[1:] ** 2).mean(1))
-        print("fluid", n, "max rel err per field", err, "increment rel rms", inc, flush=True)
+        inc = np.sqrt(((dg[1:4] - do[1:4]) ** 2).mean(1) / (do[1:4] ** 2).mean(1))
+        e_bound = 4 * (2 * infos[n]["dx"]) ** 3 * 2.0 ** -24
+        print("fluid", n, "max rel err per field", err, "momentum increment rel rms", inc, "E bound", e_bound,
+              flush=True)
         assert np.array_equal(g[0], o[0])
-        assert np.all(err < 1e-4), (n, err)
-        assert np.all(inc < 1e-4), (n, inc)
+        assert np.all(err[1:4] < 1e-5), (n, err)
+        assert np.all(inc < 1e-5), (n, inc)
+        assert err[4] < e_bound, (n, err[4], e_bound)
         assert not np.array_equal(g, fl[n])
     a, b = cube.download(), orc.get_state()
     diff = np.abs(a.vp.astype(np.int64) - b.vp.astype(np.int64))
+    # codes straddling a quantisation boundary under the two sides' force
+    # rounding; this kick (dt 0.3 at a 0.25) is several times the parity
+    # tests' kicks, so proportionally more codes straddle
     print("vp mismatch frac", float(np.mean(diff > 0)), flush=True)
-    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    assert diff.max() <= 1 and np.mean(diff > 0) < 5e-3
     cube.close()
 
 
