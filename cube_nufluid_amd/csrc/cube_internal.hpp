@@ -217,9 +217,12 @@ __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int 
   int32_t n = (int32_t)rintf(y);
   if (n > kmax) n = kmax;
   if (n < -kmax) n = -kmax;
+  // both bounds of the guessed code are loaded together
   for (int it = 0; it < 4; it++) {
-    if (n > -kmax && x < thr[n + kmax - 1]) { n--; continue; }
-    if (n < kmax && x >= thr[n + kmax]) { n++; continue; }
+    const double lo = n > -kmax ? thr[n + kmax - 1] : -HUGE_VAL;
+    const double hi = n < kmax ? thr[n + kmax] : HUGE_VAL;
+    if (x < lo) { n--; continue; }
+    if (x >= hi) { n++; continue; }
     break;
   }
   return n;

@@ -238,12 +238,57 @@ __global__ __launch_bounds__(DR_NT) void k_drift_sort_cells(const int64_t* __res
   __shared__ int32_t soff[DR_NT / 64];
   __shared__ int32_t med[DR_NT];
   __shared__ int32_t nmed;
+  __shared__ int32_t loff[DR_NT + 1];
   const int64_t r = (int64_t)blockIdx.x * DR_NT + threadIdx.x;
   int64_t base = 0;
   int n = 0;
   if (r < n3c) {
     base = cs_new[r];
     n = (int)(cs_new[r + 1] - base);
+  }
+  // Usual case: the workgroup's contiguous key range fits in LDS.  It is
+  // loaded coalesced, and each key is written to its cell's start plus its
+  // rank in the cell (the keys are distinct), one thread per key.  Keys of
+  // cells above DR_MED are left for k_drift_sort_big.
+  {
+    const int64_t rb = (int64_t)blockIdx.x * DR_NT;
+    const int ncell = (int)min((int64_t)DR_NT, n3c - rb);
+    const int64_t k0 = cs_new[rb], tot = cs_new[rb + ncell] - k0;
+    if (tot <= DR_NT * DR_SMALL) {
+      if ((int)threadIdx.x < ncell) loff[threadIdx.x] = (int32_t)(base - k0);
+      if (threadIdx.x == 0) loff[ncell] = (int32_t)tot;
+      constexpr int G = 8;
+      for (int64_t i0 = 0; i0 < tot; i0 += G * DR_NT) {
+        uint64_t v[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+          const int64_t i = i0 + q * DR_NT + threadIdx.x;
+          v[q] = i < tot ? keys[k0 + i] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+          const int64_t i = i0 + q * DR_NT + threadIdx.x;
+          if (i < tot) sk[i] = v[q];
+        }
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < (int)tot; i += DR_NT) {
+        int lo = 0, hi = ncell;  // last cell starting at or before i
+        while (hi - lo > 1) {
+          const int m = (lo + hi) >> 1;
+          if (loff[m] <= i) lo = m; else hi = m;
+        }
+        const int cb = loff[lo], ce = loff[lo + 1];
+        const uint64_t v = sk[i];
+        int rank = i - cb;
+        if (ce - cb <= DR_MED) {
+          rank = 0;
+          for (int j = cb; j < ce; j++) rank += sk[j] < v ? 1 : 0;
+        }
+        keys[k0 + cb + rank] = v;
+      }
+      return;
+    }
   }
   if (threadIdx.x == 0) nmed = 0;
   const int m = n <= DR_SMALL ? n : 0;  // keys this thread stages
@@ -388,8 +433,20 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
     const int64_t b = scs[threadIdx.x], n = scs[threadIdx.x + 1] - b;
     float vf[3];
     for (int e = 0; e < 3; e++) vf[e] = (float)((double)vfield[r * 3 + e] * wv);
-    for (int64_t k = 0; k < n; k++)
-      for (int e = 0; e < 3; e++) vf[e] = (float)((double)vf[e] + vr[3 * (b + k) + e]);
+    // in arrival order; the loads of U arrivals are issued together
+    constexpr int U = 4;
+    for (int64_t k0 = 0; k0 < n; k0 += U) {
+      double a[U][3];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) a[u][e] = k0 + u < n ? vr[3 * (b + k0 + u) + e] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (k0 + u < n)
+#pragma unroll
+          for (int e = 0; e < 3; e++) vf[e] = (float)((double)vf[e] + a[u][e]);
+    }
     float vfn[3];
     for (int e = 0; e < 3; e++) {
       vfn[e] = (float)((double)vf[e] / ((double)n + wv));

@@ -13,24 +13,47 @@
 #define KICK_NT 256
 
 // --------------------------------------------------------------- helpers
-// Load cstart for the cells of a (BX+2*lo)^3 region around block origin c0
-// into LDS as per-cell begin/offset arrays (one segment per cell).
+// Load cstart for the cells of a SPAN^3 region around block origin c0 (lo
+// cells below it) into LDS as per-cell begin/offset arrays (one segment per
+// cell).  The offsets are an exclusive scan over the whole workgroup:
+// each thread takes PER consecutive cells.
 template <int SPAN>
 __device__ void load_region(const Geo& g, const int64_t* cstart, int c0x, int c0y, int c0z, int lo,
                             int64_t* beg, int32_t* off) {
-  constexpr int NCELL = SPAN * SPAN * SPAN;
-  for (int i = threadIdx.x; i < NCELL; i += blockDim.x) {
-    int rx = i % SPAN - lo, ry = (i / SPAN) % SPAN - lo, rz = i / (SPAN * SPAN) - lo;
-    int64_t r = rank_of(g, wrapc(c0x + rx, g.nc), wrapc(c0y + ry, g.nc), wrapc(c0z + rz, g.nc));
-    int64_t b = cstart[r];
-    beg[i] = b;
-    off[i + 1] = (int32_t)(cstart[r + 1] - b);
+  constexpr int NCELL = SPAN * SPAN * SPAN, PER = (NCELL + CIC_NT - 1) / CIC_NT;
+  __shared__ int32_t wsum[CIC_NT / 64];
+  int32_t cnt[PER];
+  int32_t tsum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = threadIdx.x * PER + k;
+    cnt[k] = 0;
+    if (i < NCELL) {
+      const int rx = i % SPAN - lo, ry = (i / SPAN) % SPAN - lo, rz = i / (SPAN * SPAN) - lo;
+      const int64_t r = rank_of(g, wrapc(c0x + rx, g.nc), wrapc(c0y + ry, g.nc), wrapc(c0z + rz, g.nc));
+      const int64_t b = cstart[r];
+      beg[i] = b;
+      cnt[k] = (int32_t)(cstart[r + 1] - b);
+    }
+    tsum += cnt[k];
   }
+  const int lane = threadIdx.x & 63;
+  int32_t incl = tsum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[threadIdx.x >> 6] = incl;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    off[0] = 0;
-    for (int i = 0; i < NCELL; i++) off[i + 1] += off[i];
+  int32_t run = incl - tsum;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); w++) run += wsum[w];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = threadIdx.x * PER + k;
+    if (i < NCELL) off[i] = run;
+    run += cnt[k];
   }
+  if (threadIdx.x == CIC_NT - 1) off[NCELL] = run;
   __syncthreads();
 }
 __device__ __forceinline__ int seg_find(const int32_t* off, int n, int32_t t) {
@@ -220,9 +243,9 @@ int cic_coarse(cube_handle* h, double* mass) {
 // A workgroup takes BX consecutive coarse cells of one x-row of a tile.  It
 // first stages the force nodes those cells' particles can touch in LDS:
 // fine nodes 4*c0 + [0, 4*BX] x [0, 4] x [0, 4] of the three fine
-// components (rows loaded coalesced along x).  The
-// particles then gather their 8 fine taps per component from LDS and their
-// 8 coarse taps from the (L2-resident) coarse meshes.  A float
+// components (rows loaded coalesced along x) and coarse nodes
+// c0 - 1 + [0, BX+1] x [0, 2] x [0, 2] of the three coarse ones.  The
+// particles then gather their 8+8 taps per component from LDS.  A float
 // in-cell coordinate can round up onto the next cell's far node; those
 // rare particles read their fine taps from HBM instead.
 //
@@ -246,17 +269,15 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
                                                   const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* vmax_bits, double* probe_f, double* probe_c) {
   constexpr int SX = 4 * BX + 1, FROW = 25 * SX, NFS = 3 * FROW;  // fine slab [3][5][5][SX]
+  constexpr int CX = BX + 2, CROW = 9 * CX, NCS = 3 * CROW;       // coarse slab [3][3][3][CX]
+  static_assert(CX <= 64, "a wavefront per coarse row");
   __shared__ T sF[NFS];
+  __shared__ T sC[NCS];
   __shared__ int64_t cs[BX + 1];
   __shared__ float red[3][KICK_NT / 64];
-  // Block order: BX-cell chunks along x fastest, then (y, z) in groups of
-  // 4 x 4 rows, and each XCD runs a contiguous range of that order, so the
-  // blocks that share fine rows (y and z neighbours) run close together on
-  // one L2.
-  const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
-  const uint32_t nxb = (uint32_t)g.nc / BX, M = L / nxb, grp = M >> 4, nq = (uint32_t)g.nc >> 2;
-  int32_t c0[3] = {(int32_t)((L % nxb) * BX), (int32_t)((grp % nq) * 4 + (M & 3)), (int32_t)((grp / nq) * 4 + ((M >> 2) & 3))};
-  const int64_t r0 = rank_of(g, c0[0], c0[1], c0[2]);
+  const int64_t r0 = (int64_t)blockIdx.x * BX;
+  int32_t c0[3];
+  cell_of_rank(g, r0, c0);
   const int tid = threadIdx.x;
   const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
   for (int i = tid; i <= BX; i += KICK_NT) cs[i] = cstart[r0 + i];
@@ -290,6 +311,25 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       }
     }
   }
+  // coarse slab: 27 rows of CX nodes, a wavefront per row
+  {
+    constexpr int NW = KICK_NT / 64, RPW = (27 + NW - 1) / NW;
+    const int lane = tid & 63, wv = tid >> 6;
+    T v[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+      const int row = wv + k * NW;  // row = (d*3 + z)*3 + y
+      const int y = row % 3, z = (row / 3) % 3, d = row / 9;
+      const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
+                     gz = (uint32_t)(c0[2] - 1 + z) & mc;
+      v[k] = (row < 27 && lane < CX) ? F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+      const int row = wv + k * NW;
+      if (row < 27 && lane < CX) sC[row * CX + lane] = v[k];
+    }
+  }
   __syncthreads();
   const int64_t p0 = cs[0], p1 = cs[BX];
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
@@ -302,8 +342,8 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     }
     const int32_t c[3] = {c0[0] + lo, c0[1], c0[2]};
     T dx1[3], dx2[3], cx1[3], cx2[3];
-    uint32_t f1[3], f2[3], k1[3], k2[3];
-    int lf[3];
+    uint32_t f1[3], f2[3];
+    int lf[3], lc[3];
     for (int e = 0; e < 3; e++) {
       const int il = (g.lnt >= 0 ? (c[e] & (g.nt - 1)) : c[e] % g.nt) + 1;
       const int torg = c[e] - (il - 1);
@@ -320,20 +360,18 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       const int b1 = (int)floor(tc) + 1;  // 1-based global node: b1-1 is c-1 or c
       cx1[e] = (T)b1 - tc;
       cx2[e] = 1 - cx1[e];
-      k1[e] = (uint32_t)(b1 - 1) & mc;
-      k2[e] = (uint32_t)b1 & mc;
+      lc[e] = b1 - c0[e];  // local coarse node of the lower tap: 0..BX (x), 0..1 (y, z)
     }
     const bool in_slab = lf[0] < 4 * BX && lf[1] < 4 && lf[2] < 4;
     // the 8 taps: bit 0 picks x, bit 1 y, bit 2 z (order of pm.f90:146-153)
     const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
-    int sf[8];
-    uint32_t oc[8];
+    int sf[8], sc[8];
     T Fw[8], Cw[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
       sf[q] = ((lf[2] + oz) * 5 + lf[1] + oy) * SX + lf[0] + ox;
-      oc[q] = ((oz ? k2[2] : k1[2]) * Nc + (oy ? k2[1] : k1[1])) * Nc + (ox ? k2[0] : k1[0]);
+      sc[q] = ((lc[2] + oz) * 3 + lc[1] + oy) * CX + lc[0] + ox;
       Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
       Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
@@ -353,9 +391,8 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
           Fv[q] = Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])];
         }
       }
-      const T* Fc = F.c + (size_t)d * n3c;
 #pragma unroll
-      for (int q = 0; q < 8; q++) Cv[q] = Fc[oc[q]];
+      for (int q = 0; q < 8; q++) Cv[q] = sC[d * CROW + sc[q]];
       if (probe_f) {
         double sfv = 0, scv = 0;
 #pragma unroll
