@@ -63,10 +63,18 @@ __device__ __forceinline__ void dft_small(cpx<T>* v) {
   }
 }
 
+// Twiddle source: exp(-2 pi i m/L) for an L-point transform, read from a
+// table of the mesh length at stride s.
+template <typename T>
+struct TwG {
+  const cpx<T>* __restrict__ p;
+  int s;
+  __device__ __forceinline__ cpx<T> operator()(int m) const { return p[m * s]; }
+};
 // One Stockham pass over all W columns held in s[n*LD + w] (LD >= W; a
 // padded LD keeps transposed loads free of LDS bank conflicts).
-template <typename T, int N, int W, int LD, bool INV, int R, int NS>
-__device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ tw, int tws) {
+template <typename T, int N, int W, int LD, bool INV, int R, int NS, typename TW>
+__device__ __forceinline__ void fft_pass(cpx<T>* s, const TW& tw) {
   constexpr int B = N / R;
   constexpr int TOT = B * W;
   constexpr int PER = (TOT + NT_FFT - 1) / NT_FFT;
@@ -82,7 +90,7 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ t
       for (int r = 0; r < R; r++) {
         cpx<T> a = s[(j + r * B) * LD + w];
         if (NS > 1 && r > 0) {
-          cpx<T> t = tw[(k * r * (N / (NS * R))) * tws];
+          cpx<T> t = tw(k * r * (N / (NS * R)));
           if (INV) t.y = -t.y;
           a = cmul(a, t);
         }
@@ -107,19 +115,24 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const cpx<T>* __restrict__ t
 
 template <typename T, int N, int W, int LD, bool INV, int NS>
 struct FftPasses {
-  __device__ __forceinline__ static void run(cpx<T>* s, const cpx<T>* tw, int tws) {
+  template <typename TW>
+  __device__ __forceinline__ static void run(cpx<T>* s, const TW& tw) {
     if constexpr (NS < N) {
       constexpr int REM = N / NS;
       constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
-      fft_pass<T, N, W, LD, INV, R, NS>(s, tw, tws);
-      FftPasses<T, N, W, LD, INV, NS * R>::run(s, tw, tws);
+      fft_pass<T, N, W, LD, INV, R, NS>(s, tw);
+      FftPasses<T, N, W, LD, INV, NS * R>::run(s, tw);
     }
   }
 };
-// In-LDS FFT of W columns of length N (twiddles tw[m*tws] = exp(-2 pi i m/N)).
+// In-LDS FFT of W columns of length N; tw(m) = exp(-2 pi i m/N).
+template <typename T, int N, int W, bool INV, int LD = W, typename TW>
+__device__ __forceinline__ void lds_fft(cpx<T>* s, const TW& tw) {
+  FftPasses<T, N, W, LD, INV, 1>::run(s, tw);
+}
 template <typename T, int N, int W, bool INV, int LD = W>
 __device__ __forceinline__ void lds_fft(cpx<T>* s, const cpx<T>* tw, int tws) {
-  FftPasses<T, N, W, LD, INV, 1>::run(s, tw, tws);
+  FftPasses<T, N, W, LD, INV, 1>::run(s, TwG<T>{tw, tws});
 }
 
 __device__ __forceinline__ void grid_block_xy(int& bx, int& by) {

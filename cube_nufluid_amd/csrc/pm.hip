@@ -85,7 +85,10 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
   __shared__ int32_t off[NCELL + 1];
   __shared__ double red[CIC_NT / 64];
   const int nbx = g.nc / BC;
-  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
+  // XCD-contiguous block order: x neighbours, which write the two halves of
+  // shared 128-byte lines, run on one L2
+  const int lb = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = lb % nbx, by = (lb / nbx) % nbx, bz = lb / (nbx * nbx);
   const int c0x = bx * BC, c0y = by * BC, c0z = bz * BC;
   // block-local tile frame: 0-based offset of c0 inside its tile
   const int l0x = c0x % g.nt, l0y = c0y % g.nt, l0z = c0z % g.nt;
@@ -152,7 +155,10 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
   __shared__ int32_t off[NCELL + 1];
   __shared__ double red[CIC_NT / 64];
   const int nbx = g.nc / BC;
-  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
+  // XCD-contiguous block order: x neighbours, which write the two halves of
+  // shared 128-byte lines, run on one L2
+  const int lb = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = lb % nbx, by = (lb / nbx) % nbx, bz = lb / (nbx * nbx);
   const int c0x = bx * BC, c0y = by * BC, c0z = bz * BC;
   const int l0x = c0x % g.nt, l0y = c0y % g.nt, l0z = c0z % g.nt;
   for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) s[i] = 0ull;
