@@ -346,6 +346,19 @@ __device__ __forceinline__ bool pf_pair(float dx, float dy, float dz, float mass
   sc = mass * pc * (ri * ri * ri);
   return true;
 }
+// the same pair term with the r > rsoft test as a select; r2 > rsoft2 > 0
+// keeps the hardware reciprocal square root away from denormal inputs
+__device__ __forceinline__ float pf_pair_sel(float dx, float dy, float dz, float mass, float rsoft2) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const bool ok = r2 > rsoft2;
+  const float ri = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
+  const float r = r2 * ri;
+  const float rc = r * (1.0f / 16.0f);
+  const float rc3 = rc * rc * rc;
+  const float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
+  const float sc = mass * pc * (ri * ri * ri);
+  return ok ? sc : 0.0f;
+}
 __device__ __forceinline__ bool pf_pair(double dx, double dy, double dz, double mass, double rsoft2, double& sc) {
   const double r2 = dx * dx + dy * dy + dz * dz;
   if (!(r2 > rsoft2)) return false;
@@ -355,6 +368,10 @@ __device__ __forceinline__ bool pf_pair(double dx, double dy, double dz, double 
   const double pc = 1.0 - 1.75 * rc3 + 0.75 * rc3 * rc * rc;
   sc = mass * pc / (r2 * r);
   return true;
+}
+__device__ __forceinline__ double pf_pair_sel(double dx, double dy, double dz, double mass, double rsoft2) {
+  double sc;
+  return pf_pair(dx, dy, dz, mass, rsoft2, sc) ? sc : 0.0;
 }
 
 // force on one target from the candidate runs (start[] exclusive offsets);
@@ -376,12 +393,15 @@ __device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, 
         const PPent pj = ent[j];
         const RT dx = (RT)pj.x - tx, dy = (RT)pj.y - ty, dz = (RT)pj.z - tz;
         RT sc;
-        if (!pf_pair(dx, dy, dz, mass, rsoft2, sc)) continue;
         if (ROWSUM) {
+          // branch-free: an excluded pair adds sc = 0 (a signed zero, which
+          // leaves the running sum unchanged)
+          sc = pf_pair_sel(dx, dy, dz, mass, rsoft2);
           sx += sc * dx;
           sy += sc * dy;
           sz += sc * dz;
         } else {
+          if (!pf_pair(dx, dy, dz, mass, rsoft2, sc)) continue;
           acc.add(0, sc * dx);
           acc.add(1, sc * dy);
           acc.add(2, sc * dz);
