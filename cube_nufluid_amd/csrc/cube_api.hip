@@ -364,19 +364,20 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
                          d == 2 ? f2c : nullptr)))
       return r;
   }
-  // fine mesh: the three force components stay resident for one fused
-  // kick: F0 in S (pitch N), F1 in W and F2 in R (pitch N+2, in place).  R
-  // holds the density transform until the last z convolution reads it.
+  // fine mesh, kx-tiled spectrum: the density in R goes to a tiled
+  // spectrum in W; components 0 and 1 are convolved into X and inverted into
+  // S (F0) and R (F1), component 2 in place in W (the spectrum's last use)
+  // and inverted into X (F2).  All three stay resident for one fused kick.
   if ((r = cic_fine(h, &mass_f))) return r;
   if (nu && (r = fl_density_fine(h, a_mid))) return r;
-  if ((r = mesh_fwd_xy(h, h->fine))) return r;
+  if ((r = mesh_t_fwd(h, h->fine, h->fine.R, h->fine.W))) return r;
   const int64_t Nf = h->fine.N;
-  const void* Ff[3] = {h->fine.S, h->fine.W, h->fine.R};
+  const void* Ff[3] = {h->fine.S, h->fine.R, h->fine.X};
   const int64_t pf[3] = {Nf, Nf + 2, Nf + 2};
   for (int d = 0; d < 3; d++) {
-    void* buf = d == 2 ? h->fine.R : h->fine.W;
-    if ((r = mesh_zconv(h, h->fine, d, buf))) return r;
-    if ((r = mesh_inv_yx(h, h->fine, buf, (void*)Ff[d], pf[d], Ff[0], pf[0], Ff[1], pf[1], d == 2 ? f2f : nullptr)))
+    void* spec = d == 2 ? h->fine.W : h->fine.X;
+    if ((r = mesh_t_zconv(h, h->fine, d, h->fine.W, spec))) return r;
+    if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d], Ff[0], pf[0], Ff[1], pf[1], d == 2 ? f2f : nullptr)))
       return r;
   }
   const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);

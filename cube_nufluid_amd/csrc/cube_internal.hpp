@@ -30,9 +30,16 @@ struct Mesh {
   void* R = nullptr;     // real N*N*(N+2) padded rows; complex view N*N*(N/2+1)
   void* W = nullptr;     // complex work, same size as R
   void* S = nullptr;     // unpadded N^3 real mesh (fine: force component 0)
+  void* X = nullptr;     // (fine) second spectrum / real work mesh, bytes_big()
   void* K = nullptr;     // kernel [3][H][H][H], layout [d][ky'][kx][kz'], scaled 1/N^3
   void* tw = nullptr;    // N twiddles exp(-2 pi i m/N), type cpx<real>
   size_t bytes_R() const { return (size_t)N * N * (N + 2) * prec; }
+  // the larger of the padded real mesh and the kx-tiled spectrum
+  // (ceil((N/2+1)/16)*16 complex per (ky, kz))
+  size_t bytes_big() const {
+    const size_t t = (size_t)((N / 2 + 1 + 15) / 16) * 16 * N * N * 2 * prec;
+    return t > bytes_R() ? t : bytes_R();
+  }
 };
 
 // Per-kernel timing with HIP events on the handle's stream (diagnostic).
@@ -247,6 +254,13 @@ int mesh_zconv(cube_handle* h, Mesh& m, int d, void* dst);
 int mesh_inv_yx(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0,
                 const void* F1, int64_t p1, uint32_t* f2max);
 int mesh_fft3_r2c(cube_handle* h, Mesh& m);                  // full forward on R (for kernel build)
+// kx-tiled spectrum pipeline (fine mesh, per step): real rows src (pitch
+// N+2) -> tiled spectrum dst with the y transform; z convolution src ->
+// dst (may alias); y inverse in place then C2R into out (pitch opitch)
+int mesh_t_fwd(cube_handle* h, Mesh& m, const void* src, void* dst);
+int mesh_t_zconv(cube_handle* h, Mesh& m, int d, const void* src, void* dst);
+int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0, const void* F1,
+               int64_t p1, uint32_t* f2max);
 int fine_kernel_build(cube_handle* h, const float* table);
 int coarse_kernel_build(cube_handle* h, const float* table);
 // pm.hip

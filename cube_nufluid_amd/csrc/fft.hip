@@ -381,6 +381,229 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
   }
 }
 
+// ------------------------------------------------- kx-tiled complex layout
+// The per-step fine-mesh convolution keeps its spectrum in a kx-tiled
+// layout: element (kx, ky, kz) at ((kx/16 * N + kz) * N + ky) * 16 + kx%16,
+// KT = ceil((N/2+1)/16) tiles of 16 kx (slots past N/2 are padding).  Every
+// pass then moves whole cache lines: a y column strip of W kx is W*8 bytes
+// of each 128-byte group (neighbouring strips complete the group), a z
+// strip likewise at a stride of N*16 elements, and an x row block of W rows
+// is written and read as 16*W contiguous elements per kx tile.  The R2C and
+// C2R passes run out of place (real rows <-> tiled spectrum).
+#define TB 16
+template <typename T>
+constexpr int pick_wy(int logn) { return pick_w<T>(logn) < TB ? pick_w<T>(logn) : TB; }
+template <typename T>
+constexpr int pick_wz(int logn) { return pick_w<T>(logn) < TB ? pick_w<T>(logn) : TB; }
+__host__ __device__ constexpr int64_t t_tiles(int64_t N) { return (N / 2 + 1 + TB - 1) / TB; }
+__host__ __device__ __forceinline__ int64_t t_index(int64_t N, int64_t kx, int64_t ky, int64_t kz) {
+  return (((kx / TB) * N + kz) * N + ky) * TB + kx % TB;
+}
+
+// real rows src (pitch N+2) -> tiled spectrum dst
+template <typename T, int LOGN>
+__global__ __launch_bounds__(NT_FFT) void k_tx_r2c(const T* __restrict__ src, cpx<T>* __restrict__ dst,
+                                                   const cpx<T>* __restrict__ tw, int64_t nrows) {
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
+  constexpr int KT = (int)t_tiles(N);
+  __shared__ cpx<T> s[H * LD];
+  const int64_t r0 = (int64_t)blockIdx.x * W;
+  {
+    constexpr int PER = (H * W + NT_FFT - 1) / NT_FFT;
+    cpx<T> r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = threadIdx.x + k * NT_FFT, w = idx / H, m = idx % H;
+      const int64_t row = r0 + w;
+      r[k] = (idx < H * W && row < nrows) ? *reinterpret_cast<const cpx<T>*>(src + row * (N + 2) + 2 * m) : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = threadIdx.x + k * NT_FFT, w = idx / H, m = idx % H;
+      if (idx < H * W) s[m * LD + w] = r[k];
+    }
+  }
+  __syncthreads();
+  lds_fft<T, H, W, false, LD>(s, tw, 2);
+  // lanes run over (kx%16, row) inside one kx tile: 16*W contiguous outputs
+  for (int idx = threadIdx.x; idx < KT * TB * W; idx += NT_FFT) {
+    const int kl = idx % TB, w = (idx / TB) % W, kh = idx / (TB * W), k = kh * TB + kl;
+    const int64_t row = r0 + w;
+    if (row >= nrows) continue;
+    cpx<T> X{0, 0};
+    if (k <= H) {
+      cpx<T> zk = s[(k % H) * LD + w];
+      cpx<T> zc = s[((H - k) % H) * LD + w];
+      zc.y = -zc.y;
+      cpx<T> e{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
+      cpx<T> dd = csub(zk, zc);
+      cpx<T> o{dd.y * (T)0.5, -dd.x * (T)0.5};
+      X = cadd(e, cmul(tw[k], o));
+    }
+    dst[t_index(N, k, row % N, row / N)] = X;
+  }
+}
+
+// tiled spectrum src -> real rows dst (pitch opitch); optional |F|^2 max as
+// in k_fft_x_c2r
+template <typename T, int LOGN>
+__global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ src, T* dst, int64_t opitch,
+                                                   const cpx<T>* __restrict__ tw, int64_t nrows,
+                                                   const T* __restrict__ F0, int64_t p0, const T* __restrict__ F1, int64_t p1,
+                                                   uint32_t* f2max) {
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
+  __shared__ cpx<T> s[H * LD];
+  __shared__ cpx<T> xnyq[W];
+  __shared__ float red[NT_FFT / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * W;
+  {
+    constexpr int TOT = (H / TB + 1) * TB * W, PER = (TOT + NT_FFT - 1) / NT_FFT;
+    cpx<T> r[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = threadIdx.x + q * NT_FFT, kl = idx % TB, w = (idx / TB) % W, kh = idx / (TB * W), k = kh * TB + kl;
+      const int64_t row = r0 + w;
+      r[q] = (idx < TOT && k <= H && row < nrows) ? src[t_index(N, k, row % N, row / N)] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = threadIdx.x + q * NT_FFT, kl = idx % TB, w = (idx / TB) % W, kh = idx / (TB * W), k = kh * TB + kl;
+      if (idx < TOT && k <= H) {
+        if (k < H) s[k * LD + w] = r[q]; else xnyq[w] = r[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < (H / 2 + 1) * W; idx += NT_FFT) {
+    int w = idx / (H / 2 + 1), k = idx % (H / 2 + 1);
+    auto zof = [&](cpx<T> a, cpx<T> b, int kk) {
+      b.y = -b.y;
+      cpx<T> sum = cadd(a, b), dif = csub(a, b);
+      cpx<T> t = tw[kk];
+      t.y = -t.y;  // exp(+2 pi i kk/N)
+      cpx<T> td = cmul(t, dif);
+      return cpx<T>{sum.x - td.y, sum.y + td.x};  // sum + i*t*dif
+    };
+    cpx<T> a = s[k * LD + w];
+    if (k == 0) {
+      s[w] = zof(a, xnyq[w], 0);
+    } else if (k == H / 2) {
+      s[k * LD + w] = zof(a, a, k);
+    } else {
+      cpx<T> b = s[(H - k) * LD + w];
+      s[k * LD + w] = zof(a, b, k);
+      s[(H - k) * LD + w] = zof(b, a, H - k);
+    }
+  }
+  __syncthreads();
+  lds_fft<T, H, W, true, LD>(s, tw, 2);
+  float mymax = 0.f;
+  if (!f2max) {
+    for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
+      int w = idx / H, m = idx % H;
+      int64_t row = r0 + w;
+      if (row < nrows) *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = s[m * LD + w];
+    }
+  } else {
+    constexpr int PER = H * W / NT_FFT, FG = PER < 8 ? PER : 8;
+    static_assert(PER % FG == 0, "whole groups");
+    for (int q0 = 0; q0 < PER; q0 += FG) {
+      cpx<T> a[FG], b[FG];
+#pragma unroll
+      for (int q = 0; q < FG; q++) {
+        const int idx = threadIdx.x + (q0 + q) * NT_FFT, w = idx / H, m = idx % H;
+        const int64_t row = r0 + w < nrows ? r0 + w : nrows - 1;
+        a[q] = *reinterpret_cast<const cpx<T>*>(F0 + row * p0 + 2 * m);
+        b[q] = *reinterpret_cast<const cpx<T>*>(F1 + row * p1 + 2 * m);
+      }
+#pragma unroll
+      for (int q = 0; q < FG; q++) {
+        const int idx = threadIdx.x + (q0 + q) * NT_FFT, w = idx / H, m = idx % H;
+        const int64_t row = r0 + w;
+        if (row >= nrows) continue;
+        const cpx<T> z = s[m * LD + w];
+        *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
+        float s0 = (float)a[q].x * (float)a[q].x, s1 = (float)a[q].y * (float)a[q].y;
+        s0 = fmaf((float)b[q].x, (float)b[q].x, s0);
+        s1 = fmaf((float)b[q].y, (float)b[q].y, s1);
+        s0 = fmaf((float)z.x, (float)z.x, s0);
+        s1 = fmaf((float)z.y, (float)z.y, s1);
+        mymax = fmaxf(mymax, fmaxf(s0, s1));
+      }
+    }
+  }
+  if (f2max) {
+    for (int off = 32; off > 0; off >>= 1) mymax = fmaxf(mymax, __shfl_xor(mymax, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mymax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int i = 1; i < NT_FFT / 64; i++) m = fmaxf(m, red[i]);
+      atomicMax(f2max, __float_as_uint(m));
+    }
+  }
+}
+
+// tiled y pass, in place: tile (kh, part, kz) = W columns kx = kh*16 +
+// part*W + w along ky (stride 16)
+template <typename T, int LOGN, bool INV>
+__global__ __launch_bounds__(NT_FFT) void k_ty(cpx<T>* data, const cpx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, W = pick_wy<T>(LOGN), NP = TB / W;
+  __shared__ cpx<T> s[N * W];
+  int bx, by;
+  grid_block_xy(bx, by);
+  const int kh = bx / NP, part = bx % NP, kz = by;
+  const int64_t base = (((int64_t)kh * N + kz) * N) * TB + part * W;
+  col_load<T, N, W>(s, data + base, TB, W);
+  lds_fft<T, N, W, INV>(s, tw, 1);
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    const int n = idx / W, w = idx % W;
+    data[base + (int64_t)n * TB + w] = s[idx];
+  }
+}
+
+// tiled z convolution: dst = Zinv(i*K_d . Zfwd(src)), tile (kh, part, ky) =
+// W columns along kz (stride N*16); dst may be src
+template <typename T, int LOGN>
+__global__ __launch_bounds__(NT_FFT) void k_tzconv(const cpx<T>* src, cpx<T>* dst, const T* __restrict__ K, int d,
+                                                   const cpx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, KX = N / 2 + 1, H = N / 2 + 1, W = pick_wz<T>(LOGN), NP = TB / W;
+  constexpr int PER = N * W / NT_FFT;
+  static_assert(N * W % NT_FFT == 0, "whole loads per thread");
+  __shared__ cpx<T> s[N * W];
+  int bx, by;
+  grid_block_xy(bx, by);
+  const int kh = bx / NP, part = bx % NP, ky = by, kx0 = kh * TB + part * W;
+  const int64_t stride = (int64_t)N * TB;
+  const int64_t base = ((int64_t)kh * N * N) * TB + (int64_t)ky * TB + part * W;
+  const int kyp = ky <= N / 2 ? ky : N - ky;
+  const T sy = (d == 1 && ky > N / 2) ? (T)-1 : (T)1;
+  T kv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int idx = threadIdx.x + q * NT_FFT, w = idx % W, n = idx / W;
+    const int kx = kx0 + w < KX ? kx0 + w : KX - 1;
+    const int kzp = n <= N / 2 ? n : N - n;
+    kv[q] = K[(((int64_t)d * H + kyp) * KX + kx) * H + kzp];
+  }
+  col_load<T, N, W>(s, src + base, stride, W);
+  lds_fft<T, N, W, false>(s, tw, 1);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int idx = threadIdx.x + q * NT_FFT, n = idx / W;
+    const T sz = (d == 2 && n > N / 2) ? (T)-1 : (T)1;
+    const T k = kv[q] * sy * sz;
+    const cpx<T> a = s[idx];
+    s[idx] = cpx<T>{-a.y * k, a.x * k};
+  }
+  __syncthreads();
+  lds_fft<T, N, W, true>(s, tw, 1);
+  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+    const int n = idx / W, w = idx % W;
+    dst[base + (int64_t)n * stride + w] = s[idx];
+  }
+}
+
 // ------------------------------------------------------------ dispatchers
 #define LOGN_SWITCH(logn, ...)             \
   switch (logn) {                           \
@@ -490,14 +713,86 @@ int mesh_inv_yx(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, c
                      : inv_yx_t<double>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max);
 }
 
+// ---------------------------------------- tiled fine-mesh convolution host
+template <typename T>
+static int t_fwd_t(cube_handle* h, Mesh& m, const void* src, void* dst) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N, nrows = N * N;
+  const cpx<T>* tw = (const cpx<T>*)m.tw;
+  LOGN_SWITCH(logn, {
+    constexpr int WX = pick_w<T>(L - 1);
+    int p1 = prof_begin(h, PS_FFT_XF);
+    k_tx_r2c<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>((const T*)src, (cpx<T>*)dst, tw, nrows);
+    prof_end(h, p1);
+    constexpr int WY = pick_wy<T>(L);
+    int p2 = prof_begin(h, PS_FFT_YF);
+    k_ty<T, L, false><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), NT_FFT, 0, h->stream>>>((cpx<T>*)dst, tw);
+    prof_end(h, p2);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_t_fwd(cube_handle* h, Mesh& m, const void* src, void* dst) {
+  return m.prec == 4 ? t_fwd_t<float>(h, m, src, dst) : t_fwd_t<double>(h, m, src, dst);
+}
+
+template <typename T>
+static int t_zconv_t(cube_handle* h, Mesh& m, int d, const void* src, void* dst) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N;
+  int p1 = prof_begin(h, PS_FFT_ZCONV);
+  LOGN_SWITCH(logn, {
+    constexpr int WZ = pick_wz<T>(L);
+    k_tzconv<T, L><<<dim3((unsigned)(t_tiles(N) * (TB / WZ)), (unsigned)N), NT_FFT, 0, h->stream>>>(
+        (const cpx<T>*)src, (cpx<T>*)dst, (const T*)m.K, d, (const cpx<T>*)m.tw);
+  });
+  prof_end(h, p1);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_t_zconv(cube_handle* h, Mesh& m, int d, const void* src, void* dst) {
+  return m.prec == 4 ? t_zconv_t<float>(h, m, d, src, dst) : t_zconv_t<double>(h, m, d, src, dst);
+}
+
+template <typename T>
+static int t_inv_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0, const void* F1,
+                   int64_t p1, uint32_t* f2max) {
+  int logn = ilog2(m.N);
+  int64_t N = m.N, nrows = N * N;
+  const cpx<T>* tw = (const cpx<T>*)m.tw;
+  LOGN_SWITCH(logn, {
+    constexpr int WY = pick_wy<T>(L);
+    int pa = prof_begin(h, PS_FFT_YI);
+    k_ty<T, L, true><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), NT_FFT, 0, h->stream>>>((cpx<T>*)buf, tw);
+    prof_end(h, pa);
+    constexpr int WX = pick_w<T>(L - 1);
+    int pb = prof_begin(h, PS_FFT_XI);
+    k_tx_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
+        (const cpx<T>*)buf, (T*)out, opitch, tw, nrows, (const T*)F0, p0, (const T*)F1, p1, f2max);
+    prof_end(h, pb);
+  });
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0, const void* F1,
+               int64_t p1, uint32_t* f2max) {
+  if (out == buf) return cube_fail(h, CUBE_ERR_ARG, "tiled C2R runs out of place");
+  if (f2max && (!F0 || !F1)) return cube_fail(h, CUBE_ERR_ARG, "force maximum needs both other components");
+  return m.prec == 4 ? t_inv_t<float>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max)
+                     : t_inv_t<double>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max);
+}
+
 // -------------------------------------------------------------- allocation
 int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S, bool with_K) {
   m.N = N;
   m.prec = prec;
   int64_t H = N / 2 + 1;
-  HIP_TRY(h, hipMalloc(&m.R, m.bytes_R()));
-  HIP_TRY(h, hipMalloc(&m.W, m.bytes_R()));
-  if (with_S) HIP_TRY(h, hipMalloc(&m.S, (size_t)prec * N * N * N));
+  HIP_TRY(h, hipMalloc(&m.R, m.bytes_big()));
+  HIP_TRY(h, hipMalloc(&m.W, m.bytes_big()));
+  if (with_S) {
+    HIP_TRY(h, hipMalloc(&m.S, (size_t)prec * N * N * N));
+    HIP_TRY(h, hipMalloc(&m.X, m.bytes_big()));
+  }
   if (with_K) HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * H * H * prec));
   HIP_TRY(h, hipMalloc(&m.tw, (size_t)N * 2 * prec));
   std::vector<double> tw(2 * N);
@@ -519,6 +814,7 @@ void mesh_free(Mesh& m) {
   if (m.R) (void)hipFree(m.R);
   if (m.W) (void)hipFree(m.W);
   if (m.S) (void)hipFree(m.S);
+  if (m.X) (void)hipFree(m.X);
   if (m.K) (void)hipFree(m.K);
   if (m.tw) (void)hipFree(m.tw);
   m = Mesh();

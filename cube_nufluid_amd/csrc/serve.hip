@@ -112,7 +112,8 @@ extern "C" int64_t cube_footprint(const cube_params* p, int32_t ic_np_nc) {
   const int64_t npm = p->np_max, nc = p->nc, n3c = nc * nc * nc, nf = 4 * nc, prec = p->precision;
   auto mesh = [&](int64_t N, bool with_S) {
     const int64_t H = N / 2 + 1;
-    return 2 * N * N * (N + 2) * prec + (with_S ? N * N * N * prec : 0) + 3 * H * H * H * prec + 2 * N * prec;
+    const int64_t big = std::max(N * N * (N + 2) * prec, (H + 15) / 16 * 16 * N * N * 2 * prec);  // Mesh::bytes_big
+    return 2 * big + (with_S ? N * N * N * prec + big : 0) + 3 * H * H * H * prec + 2 * N * prec;
   };
   const int64_t nb2 = (nc / 2) * (nc / 2) * (nc / 2);
   int64_t b = 0;

@@ -224,9 +224,11 @@ def test_gpu_pm_with_fluid_matches_oracle():
     diff = np.abs(a.vp.astype(np.int64) - b.vp.astype(np.int64))
     # codes straddling a quantisation boundary under the two sides' force
     # rounding; this kick (dt 0.3 at a 0.25) is several times the parity
-    # tests' kicks, so proportionally more codes straddle
-    print("vp mismatch frac", float(np.mean(diff > 0)), flush=True)
-    assert diff.max() <= 1 and np.mean(diff > 0) < 5e-3
+    # tests' kicks, so proportionally more codes straddle.  A code is
+    # re-quantised twice (after the fine and after the coarse kick), so two
+    # straddles can add up to 2.
+    print("vp mismatch frac", float(np.mean(diff > 0)), "max", int(diff.max()), flush=True)
+    assert diff.max() <= 2 and np.mean(diff > 0) < 5e-3
     cube.close()
 
 
