@@ -71,6 +71,48 @@ struct TwG {
   int s;
   __device__ __forceinline__ cpx<T> operator()(int m) const { return p[m * s]; }
 };
+// exp(-2 pi i m/16) (conjugated for the inverse) for the products m = n2*k1
+// that occur inside a 16-point DFT
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> w16(int m) {
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+  cpx<T> w;
+  switch (m) {
+    case 1: w = {c1, -s1}; break;
+    case 2: w = {h, -h}; break;
+    case 3: w = {s1, -c1}; break;
+    case 4: w = {(T)0, (T)-1}; break;
+    case 6: w = {-h, -h}; break;
+    case 9: w = {-c1, s1}; break;
+    default: w = {(T)1, (T)0};
+  }
+  if (INV) w.y = -w.y;
+  return w;
+}
+// 16-point DFT as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(cpx<T>* v) {
+  cpx<T> a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) {
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++) a[n2][n1] = v[4 * n1 + n2];
+    dft_small<T, 4, INV>(a[n2]);
+#pragma unroll
+    for (int k1 = 1; k1 < 4; k1++)
+      if (n2 > 0) a[n2][k1] = cmul(a[n2][k1], w16<T, INV>(n2 * k1));
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    cpx<T> b[4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) b[n2] = a[n2][k1];
+    dft_small<T, 4, INV>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = b[k2];
+  }
+}
+
 // One Stockham pass over all W columns held in s[n*LD + w] (LD >= W; a
 // padded LD keeps transposed loads free of LDS bank conflicts).
 template <typename T, int N, int W, int LD, bool INV, int R, int NS, typename TW>
@@ -103,7 +145,7 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const TW& tw) {
   for (int q = 0; q < PER; q++) {
     int b = tid + q * NT_FFT;
     if (TOT % NT_FFT == 0 || b < TOT) {
-      dft_small<T, R, INV>(v[q]);
+      if constexpr (R == 16) dft16<T, INV>(v[q]); else dft_small<T, R, INV>(v[q]);
       int w = b % W, j = b / W;
       int idxD = (j / NS) * NS * R + (j % NS);
 #pragma unroll
@@ -119,7 +161,7 @@ struct FftPasses {
   __device__ __forceinline__ static void run(cpx<T>* s, const TW& tw) {
     if constexpr (NS < N) {
       constexpr int REM = N / NS;
-      constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
+      constexpr int R = (REM % 16 == 0) ? 16 : ((REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2));
       fft_pass<T, N, W, LD, INV, R, NS>(s, tw);
       FftPasses<T, N, W, LD, INV, NS * R>::run(s, tw);
     }
