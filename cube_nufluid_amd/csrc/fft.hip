@@ -115,17 +115,17 @@ __device__ __forceinline__ void dft16(cpx<T>* v) {
 
 // One Stockham pass over all W columns held in s[n*LD + w] (LD >= W; a
 // padded LD keeps transposed loads free of LDS bank conflicts).
-template <typename T, int N, int W, int LD, bool INV, int R, int NS, typename TW>
+template <typename T, int N, int W, int LD, bool INV, int R, int NS, int NTH, typename TW>
 __device__ __forceinline__ void fft_pass(cpx<T>* s, const TW& tw) {
   constexpr int B = N / R;
   constexpr int TOT = B * W;
-  constexpr int PER = (TOT + NT_FFT - 1) / NT_FFT;
+  constexpr int PER = (TOT + NTH - 1) / NTH;
   cpx<T> v[PER][R];
   const int tid = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    int b = tid + q * NT_FFT;
-    if (TOT % NT_FFT == 0 || b < TOT) {
+    int b = tid + q * NTH;
+    if (TOT % NTH == 0 || b < TOT) {
       int w = b % W, j = b / W;
       int k = j % NS;
 #pragma unroll
@@ -143,8 +143,8 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const TW& tw) {
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    int b = tid + q * NT_FFT;
-    if (TOT % NT_FFT == 0 || b < TOT) {
+    int b = tid + q * NTH;
+    if (TOT % NTH == 0 || b < TOT) {
       if constexpr (R == 16) dft16<T, INV>(v[q]); else dft_small<T, R, INV>(v[q]);
       int w = b % W, j = b / W;
       int idxD = (j / NS) * NS * R + (j % NS);
@@ -155,26 +155,26 @@ __device__ __forceinline__ void fft_pass(cpx<T>* s, const TW& tw) {
   __syncthreads();
 }
 
-template <typename T, int N, int W, int LD, bool INV, int NS>
+template <typename T, int N, int W, int LD, bool INV, int NS, int NTH>
 struct FftPasses {
   template <typename TW>
   __device__ __forceinline__ static void run(cpx<T>* s, const TW& tw) {
     if constexpr (NS < N) {
       constexpr int REM = N / NS;
       constexpr int R = (REM % 16 == 0) ? 16 : ((REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2));
-      fft_pass<T, N, W, LD, INV, R, NS>(s, tw);
-      FftPasses<T, N, W, LD, INV, NS * R>::run(s, tw);
+      fft_pass<T, N, W, LD, INV, R, NS, NTH>(s, tw);
+      FftPasses<T, N, W, LD, INV, NS * R, NTH>::run(s, tw);
     }
   }
 };
-// In-LDS FFT of W columns of length N; tw(m) = exp(-2 pi i m/N).
-template <typename T, int N, int W, bool INV, int LD = W, typename TW>
+// In-LDS FFT of W columns of length N by NTH threads; tw(m) = exp(-2 pi i m/N).
+template <typename T, int N, int W, bool INV, int LD = W, int NTH = NT_FFT, typename TW>
 __device__ __forceinline__ void lds_fft(cpx<T>* s, const TW& tw) {
-  FftPasses<T, N, W, LD, INV, 1>::run(s, tw);
+  FftPasses<T, N, W, LD, INV, 1, NTH>::run(s, tw);
 }
-template <typename T, int N, int W, bool INV, int LD = W>
+template <typename T, int N, int W, bool INV, int LD = W, int NTH = NT_FFT>
 __device__ __forceinline__ void lds_fft(cpx<T>* s, const cpx<T>* tw, int tws) {
-  FftPasses<T, N, W, LD, INV, 1>::run(s, TwG<T>{tw, tws});
+  FftPasses<T, N, W, LD, INV, 1, NTH>::run(s, TwG<T>{tw, tws});
 }
 
 __device__ __forceinline__ void grid_block_xy(int& bx, int& by) {
@@ -340,18 +340,18 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_x_c2r(const cpx<T>* src, T* dst,
 // Load W columns of N complex values at stride `stride` into s[n*W + w]
 // (columns w >= wvalid read as zero).  All loads of a thread are issued
 // before any LDS store so that they are in flight together.
-template <typename T, int N, int W>
+template <typename T, int N, int W, int NTH = NT_FFT>
 __device__ __forceinline__ void col_load(cpx<T>* s, const cpx<T>* __restrict__ src, int64_t stride, int wvalid) {
-  constexpr int TOT = N * W, PER = TOT / NT_FFT;
-  static_assert(TOT % NT_FFT == 0, "whole loads per thread");
+  constexpr int TOT = N * W, PER = TOT / NTH;
+  static_assert(TOT % NTH == 0, "whole loads per thread");
   cpx<T> r[PER];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const int idx = threadIdx.x + k * NT_FFT, n = idx / W, w = idx % W;
+    const int idx = threadIdx.x + k * NTH, n = idx / W, w = idx % W;
     r[k] = w < wvalid ? src[n * stride + w] : cpx<T>{0, 0};
   }
 #pragma unroll
-  for (int k = 0; k < PER; k++) s[threadIdx.x + k * NT_FFT] = r[k];
+  for (int k = 0; k < PER; k++) s[threadIdx.x + k * NTH] = r[k];
   __syncthreads();
 }
 
@@ -435,8 +435,12 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
 #define TB 16
 template <typename T>
 constexpr int pick_wy(int logn) { return pick_w<T>(logn) < TB ? pick_w<T>(logn) : TB; }
+// z convolution strips: twice the y strip width (whole 64-byte halves of
+// each group), with twice the threads when that doubles the LDS tile
 template <typename T>
-constexpr int pick_wz(int logn) { return pick_w<T>(logn) < TB ? pick_w<T>(logn) : TB; }
+constexpr int pick_wz(int logn) { return 2 * pick_w<T>(logn) < TB ? 2 * pick_w<T>(logn) : TB; }
+template <typename T>
+constexpr int pick_ntz(int logn) { return pick_wz<T>(logn) > pick_w<T>(logn) ? 2 * NT_FFT : NT_FFT; }
 __host__ __device__ constexpr int64_t t_tiles(int64_t N) { return (N / 2 + 1 + TB - 1) / TB; }
 __host__ __device__ __forceinline__ int64_t t_index(int64_t N, int64_t kx, int64_t ky, int64_t kz) {
   return (((kx / TB) * N + kz) * N + ky) * TB + kx % TB;
@@ -607,11 +611,11 @@ __global__ __launch_bounds__(NT_FFT) void k_ty(cpx<T>* data, const cpx<T>* __res
 // tiled z convolution: dst = Zinv(i*K_d . Zfwd(src)), tile (kh, part, ky) =
 // W columns along kz (stride N*16); dst may be src
 template <typename T, int LOGN>
-__global__ __launch_bounds__(NT_FFT) void k_tzconv(const cpx<T>* src, cpx<T>* dst, const T* __restrict__ K, int d,
+__global__ __launch_bounds__(pick_ntz<T>(LOGN)) void k_tzconv(const cpx<T>* src, cpx<T>* dst, const T* __restrict__ K, int d,
                                                    const cpx<T>* __restrict__ tw) {
-  constexpr int N = 1 << LOGN, KX = N / 2 + 1, H = N / 2 + 1, W = pick_wz<T>(LOGN), NP = TB / W;
-  constexpr int PER = N * W / NT_FFT;
-  static_assert(N * W % NT_FFT == 0, "whole loads per thread");
+  constexpr int N = 1 << LOGN, KX = N / 2 + 1, H = N / 2 + 1, W = pick_wz<T>(LOGN), NP = TB / W, NTH = pick_ntz<T>(LOGN);
+  constexpr int PER = N * W / NTH;
+  static_assert(N * W % NTH == 0, "whole loads per thread");
   __shared__ cpx<T> s[N * W];
   int bx, by;
   grid_block_xy(bx, by);
@@ -623,24 +627,24 @@ __global__ __launch_bounds__(NT_FFT) void k_tzconv(const cpx<T>* src, cpx<T>* ds
   T kv[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int idx = threadIdx.x + q * NT_FFT, w = idx % W, n = idx / W;
+    const int idx = threadIdx.x + q * NTH, w = idx % W, n = idx / W;
     const int kx = kx0 + w < KX ? kx0 + w : KX - 1;
     const int kzp = n <= N / 2 ? n : N - n;
     kv[q] = K[(((int64_t)d * H + kyp) * KX + kx) * H + kzp];
   }
-  col_load<T, N, W>(s, src + base, stride, W);
-  lds_fft<T, N, W, false>(s, tw, 1);
+  col_load<T, N, W, NTH>(s, src + base, stride, W);
+  lds_fft<T, N, W, false, W, NTH>(s, tw, 1);
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    const int idx = threadIdx.x + q * NT_FFT, n = idx / W;
+    const int idx = threadIdx.x + q * NTH, n = idx / W;
     const T sz = (d == 2 && n > N / 2) ? (T)-1 : (T)1;
     const T k = kv[q] * sy * sz;
     const cpx<T> a = s[idx];
     s[idx] = cpx<T>{-a.y * k, a.x * k};
   }
   __syncthreads();
-  lds_fft<T, N, W, true>(s, tw, 1);
-  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+  lds_fft<T, N, W, true, W, NTH>(s, tw, 1);
+  for (int idx = threadIdx.x; idx < N * W; idx += NTH) {
     const int n = idx / W, w = idx % W;
     dst[base + (int64_t)n * stride + w] = s[idx];
   }
@@ -785,7 +789,7 @@ static int t_zconv_t(cube_handle* h, Mesh& m, int d, const void* src, void* dst)
   int p1 = prof_begin(h, PS_FFT_ZCONV);
   LOGN_SWITCH(logn, {
     constexpr int WZ = pick_wz<T>(L);
-    k_tzconv<T, L><<<dim3((unsigned)(t_tiles(N) * (TB / WZ)), (unsigned)N), NT_FFT, 0, h->stream>>>(
+    k_tzconv<T, L><<<dim3((unsigned)(t_tiles(N) * (TB / WZ)), (unsigned)N), pick_ntz<T>(L), 0, h->stream>>>(
         (const cpx<T>*)src, (cpx<T>*)dst, (const T*)m.K, d, (const cpx<T>*)m.tw);
   });
   prof_end(h, p1);
