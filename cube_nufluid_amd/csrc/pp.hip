@@ -548,15 +548,40 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
     if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) ent[atomicAdd(&wc[1 + wi], 1)] = e;
   }
   __syncthreads();
-  // runs in particle-index order: the pair sums below then have a fixed order
-  for (int i = threadIdx.x; i < W3; i += NT) {
-    const int a = wc[i], b = wc[i + 1];
-    for (int m = a + 1; m < b; m++) {
-      const PPent v = ent[m];
-      int j = m - 1;
-      while (j >= a && ent[j].p > v.p) { ent[j + 1] = ent[j]; j--; }
-      ent[j + 1] = v;
+  // runs in particle-index order, so the pair sums below have a fixed
+  // order: each entry moves to its run's start plus its rank by particle
+  // index within the run (entries are held in registers across the barrier)
+  {
+    constexpr int PE = (CAP + NT - 1) / NT;
+    PPent me[PE];
+    int dst[PE];
+#pragma unroll
+    for (int k = 0; k < PE; k++) {
+      const int m = threadIdx.x + k * NT;
+      dst[k] = -1;
+      if (m < ncand) {
+        me[k] = ent[m];
+        const int wi = (((int)floorf(me[k].z) + pr) * W + ((int)floorf(me[k].y) + pr)) * W + (int)floorf(me[k].x) + pr;
+        const int a = wc[wi], b = wc[wi + 1];
+        int rank = 0;
+        if (b - a > 1)
+          for (int j = a; j < b; j++) rank += ent[j].p < me[k].p ? 1 : 0;
+        dst[k] = a + rank;
+      }
     }
+    __syncthreads();
+    // the owner table is dead now; its bytes hold tpos[t], the sorted
+    // position of region particle t < PF_OWN/2, for the target lookup below
+    uint16_t* tpos = reinterpret_cast<uint16_t*>(owner);
+#pragma unroll
+    for (int k = 0; k < PE; k++)
+      if (dst[k] >= 0) {
+        ent[dst[k]] = me[k];
+        const int rx = (int)floorf(me[k].x * 0.25f), ry = (int)floorf(me[k].y * 0.25f), rz = (int)floorf(me[k].z * 0.25f);
+        const int a = ((rz + 1) * G::SPAN + (ry + 1)) * G::SPAN + (rx + 1);
+        const int32_t t = off[a] + (int32_t)((int64_t)me[k].p - beg[a]);
+        if (t < PF_OWN / 2) tpos[t] = (uint16_t)dst[k];
+      }
   }
   __syncthreads();
   // targets: the particles of the workgroup's own PB^3 coarse cells, dealt
@@ -589,9 +614,14 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
       if (inoff[m] <= k) lo = m; else hi = m;
     }
     const int a = ia[lo];
+    const int32_t t = off[a] + (k - inoff[lo]);
     PPent ti;
-    int wi_unused;
-    (void)pf_cand<PB>(g, xp, beg, off, off[a] + (k - inoff[lo]), pr, W, wi_unused, ti, own);
+    if (t < PF_OWN / 2) {
+      ti = ent[reinterpret_cast<const uint16_t*>(owner)[t]];
+    } else {
+      int wi_unused;
+      (void)pf_cand<PB>(g, xp, beg, off, t, pr, W, wi_unused, ti);
+    }
     ninner++;
     int32_t vc[3] = {0, 0, 0};
     if (!probe)
