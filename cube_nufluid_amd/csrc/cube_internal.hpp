@@ -235,6 +235,36 @@ __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int 
   return n;
 }
 
+// vencode in two stages, so that callers can issue the threshold loads of
+// several values together: the FP32 atan guess, then the correction from
+// the guess's two thresholds (lo = thr[n+kmax-1], hi = thr[n+kmax], or
+// -inf/+inf at the ends) and, rarely, further thresholds.  The result equals
+// vencode's: the correction only ever moves in one direction.
+__device__ __forceinline__ int32_t venc_guess(int32_t kmax, int izipv, double x) {
+  const float nv1 = izipv == 1 ? 255.0f : 65535.0f;
+  const float y = nv1 * atanf((float)x) * (1.0f / PI_F);
+  int32_t n = (int32_t)rintf(y);
+  if (n > kmax) n = kmax;
+  if (n < -kmax) n = -kmax;
+  return n;
+}
+__device__ __forceinline__ double venc_lo(const double* thr, int32_t kmax, int32_t n) {
+  return n > -kmax ? thr[n + kmax - 1] : -HUGE_VAL;
+}
+__device__ __forceinline__ double venc_hi(const double* thr, int32_t kmax, int32_t n) {
+  return n < kmax ? thr[n + kmax] : HUGE_VAL;
+}
+__device__ __forceinline__ int32_t venc_finish(const double* thr, int32_t kmax, double x, int32_t n, double lo, double hi) {
+  if (x < lo) {
+    n--;
+    for (int it = 1; it < 4 && n > -kmax && x < thr[n + kmax - 1]; it++) n--;
+  } else if (x >= hi) {
+    n++;
+    for (int it = 1; it < 4 && n < kmax && x >= thr[n + kmax]; it++) n++;
+  }
+  return n;
+}
+
 // ------------------------------------------------------------- launchers
 // decoded-velocity table for coefficient c (one of two cached slots)
 const double* vtab_for(cube_handle* h, double c);

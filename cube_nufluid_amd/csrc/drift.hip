@@ -233,6 +233,7 @@ __device__ __forceinline__ void bitonic(uint64_t* s, int n, int npow2, int tid, 
 }
 
 #define DR_MED_LDS 1024
+#define DC_SL 4  // slots per thread and batch in k_drift_cells
 __global__ __launch_bounds__(DR_NT) void k_drift_sort_cells(const int64_t* __restrict__ cs_new, int64_t n3c, uint64_t* keys) {
   __shared__ uint64_t sk[DR_NT * DR_SMALL];  // small cells; then DR_NT/64 wavefront areas of DR_MED_LDS
   __shared__ int32_t soff[DR_NT / 64];
@@ -385,7 +386,11 @@ __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __
 // ---------------------------------------------------------------- gather
 // One thread per output slot q (checkpoint order after the drift): the
 // arrival's full velocity vreal (update_particle.f90:69), its new position
-// code (:117) and PID; vreal is kept for the per-cell pass.
+// code (:117) and PID; vreal is kept for the per-cell pass.  Each thread
+// takes DG_SL slots (DR_NT apart) and issues each stage's loads for all of
+// them together: keys, then cell offsets, then the particle's codes, then
+// the decoded velocities.
+#define DG_SL 4
 __global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* __restrict__ keys, int64_t np,
                                                         const int64_t* __restrict__ cstart, const void* __restrict__ xp,
                                                         const void* __restrict__ vp, const int32_t* __restrict__ pid,
@@ -393,19 +398,53 @@ __global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* _
                                                         const double* __restrict__ vdt, void* xp_o, int32_t* pid_o,
                                                         double* vr) {
   const double xscale = (g.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0) * (double)CUBE_NCELL;
-  for (int64_t q = (int64_t)blockIdx.x * DR_NT + threadIdx.x; q < np; q += (int64_t)gridDim.x * DR_NT) {
-    const uint64_t key = keys[q];
-    const int64_t rs = key_rank(key);
-    const int64_t p = cstart[rs] + key_l(key);
-    for (int e = 0; e < 3; e++) {
-      const double vreal = vdecode(vdt, load_v(vp, 3 * p + e, g.izipv), g.izipv) + (double)vfield[rs * 3 + e];
-      int64_t xn = (int64_t)load_xu(xp, 3 * p + e, g.izipx);
-      xn = g.izipx == 1 ? (int64_t)(int8_t)xn : (int64_t)(int16_t)xn;
-      xn += (int64_t)nint_d((double)dt_mid * vreal / xscale);
-      store_x(xp_o, 3 * q + e, g.izipx, xn);
-      vr[3 * q + e] = vreal;
+  for (int64_t q0 = (int64_t)blockIdx.x * DR_NT * DG_SL + threadIdx.x; q0 < np; q0 += (int64_t)gridDim.x * DR_NT * DG_SL) {
+    uint64_t key[DG_SL];
+    bool ok[DG_SL];
+#pragma unroll
+    for (int j = 0; j < DG_SL; j++) {
+      ok[j] = q0 + j * DR_NT < np;
+      key[j] = ok[j] ? keys[q0 + j * DR_NT] : 0;
     }
-    pid_o[q] = pid[p];
+    int64_t rs[DG_SL], p[DG_SL];
+#pragma unroll
+    for (int j = 0; j < DG_SL; j++) {
+      rs[j] = key_rank(key[j]);
+      p[j] = ok[j] ? cstart[rs[j]] + key_l(key[j]) : 0;
+    }
+    int32_t vc[DG_SL][3], pv[DG_SL];
+    uint32_t xu[DG_SL][3];
+    float vf[DG_SL][3];
+#pragma unroll
+    for (int j = 0; j < DG_SL; j++) {
+      pv[j] = ok[j] ? pid[p[j]] : 0;
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        vc[j][e] = ok[j] ? load_v(vp, 3 * p[j] + e, g.izipv) : 0;
+        xu[j][e] = ok[j] ? load_xu(xp, 3 * p[j] + e, g.izipx) : 0;
+        vf[j][e] = ok[j] ? vfield[rs[j] * 3 + e] : 0.f;
+      }
+    }
+    double vd[DG_SL][3];
+#pragma unroll
+    for (int j = 0; j < DG_SL; j++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) vd[j][e] = ok[j] ? vdecode(vdt, vc[j][e], g.izipv) : 0.0;
+#pragma unroll
+    for (int j = 0; j < DG_SL; j++) {
+      if (!ok[j]) continue;
+      const int64_t q = q0 + j * DR_NT;
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        const double vreal = vd[j][e] + (double)vf[j][e];
+        int64_t xn = (int64_t)xu[j][e];
+        xn = g.izipx == 1 ? (int64_t)(int8_t)xn : (int64_t)(int16_t)xn;
+        xn += (int64_t)nint_d((double)dt_mid * vreal / xscale);
+        store_x(xp_o, 3 * q + e, g.izipx, xn);
+        vr[3 * q + e] = vreal;
+      }
+      pid_o[q] = pv[j];
+    }
   }
 }
 
@@ -457,23 +496,54 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
     sum_c = (double)(vfn[0] * vfn[0] + vfn[1] * vfn[1] + vfn[2] * vfn[2]);
   }
   __syncthreads();
+  // slots in DC_SL batches per thread (DR_NT apart, visited in increasing
+  // order as before); each stage's loads for the batch are issued together
   const int64_t q1 = scs[ncell];
-  for (int64_t q = scs[0] + threadIdx.x; q < q1; q += DR_NT) {
-    int lo = 0, hi = ncell;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (scs[mid] <= q) lo = mid; else hi = mid;
+  for (int64_t q0 = scs[0] + threadIdx.x; q0 < q1; q0 += DR_NT * DC_SL) {
+    bool ok[DC_SL];
+    int cl[DC_SL];
+    double x[DC_SL][3];
+#pragma unroll
+    for (int j = 0; j < DC_SL; j++) {
+      const int64_t q = q0 + j * DR_NT;
+      ok[j] = q < q1;
+      int lo = 0, hi = ncell;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (scs[mid] <= q) lo = mid; else hi = mid;
+      }
+      cl[j] = lo;
+#pragma unroll
+      for (int e = 0; e < 3; e++) x[j][e] = ok[j] ? vr[3 * q + e] : 0.0;
     }
-    double v[3];
-    for (int e = 0; e < 3; e++) {
-      const float vfn = svf[lo * 3 + e];
-      const int32_t code = vencode(venc, kmax, g.izipv, c_old, vr[3 * q + e] - (double)vfn);
-      store_v(vp_o, 3 * q + e, g.izipv, code);
-      v[e] = vdecode(vdt, code, g.izipv);
+    int32_t n[DC_SL][3];
+    double tlo[DC_SL][3], thi[DC_SL][3];
+#pragma unroll
+    for (int j = 0; j < DC_SL; j++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        x[j][e] = c_old * (x[j][e] - (double)svf[cl[j] * 3 + e]);
+        n[j][e] = venc_guess(kmax, g.izipv, x[j][e]);
+        tlo[j][e] = venc_lo(venc, kmax, n[j][e]);
+        thi[j][e] = venc_hi(venc, kmax, n[j][e]);
+      }
+    double v[DC_SL][3];
+#pragma unroll
+    for (int j = 0; j < DC_SL; j++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        n[j][e] = venc_finish(venc, kmax, x[j][e], n[j][e], tlo[j][e], thi[j][e]);
+        if (ok[j]) store_v(vp_o, 3 * (q0 + j * DR_NT) + e, g.izipv, n[j][e]);
+        v[j][e] = vdecode(vdt, n[j][e], g.izipv);
+      }
+#pragma unroll
+    for (int j = 0; j < DC_SL; j++) {
+      if (!ok[j]) continue;
+      double w[3] = {v[j][0], v[j][1], v[j][2]};
+      sum_r += w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+      for (int e = 0; e < 3; e++) w[e] += (double)svf[cl[j] * 3 + e];
+      sum_s += w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
     }
-    sum_r += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-    for (int e = 0; e < 3; e++) v[e] += (double)svf[lo * 3 + e];
-    sum_s += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   }
   double v3[3] = {sum_c, sum_r, sum_s};
   for (int f = 0; f < 3; f++) {
@@ -532,7 +602,7 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   prof_end(h, pid4);
   HIP_TRY(h, hipGetLastError());
   int pid5 = prof_begin(h, PS_DRIFT_WRITE);
-  const int64_t ngather = std::min<int64_t>((h->np + DR_NT - 1) / DR_NT, 1 << 20);
+  const int64_t ngather = std::min<int64_t>((h->np + DR_NT * DG_SL - 1) / (DR_NT * DG_SL), 1 << 20);
   k_drift_gather<<<(unsigned)std::max<int64_t>(ngather, 1), DR_NT, 0, h->stream>>>(
       g, keys, h->np, h->cstart, h->xp, h->vp, h->pid, h->vfield, dt_mid, c_old, vdt, h->xp2, h->pid2,
       (double*)h->dscratch);

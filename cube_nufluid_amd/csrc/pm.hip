@@ -381,6 +381,7 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
       Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
+    T Fk[3][8], Ck[3][8];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       T Fv[8], Cv[8];
@@ -410,17 +411,46 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
         probe_c[3 * p + d] = scv;
         continue;
       }
-      double v = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
+      // the per-tap terms (F*k*w in the mesh precision), added below in tap order
 #pragma unroll
-      for (int q = 0; q < 8; q++) v = v + (double)(Fv[q] * kf * Fw[q]);
-      const int32_t qn = vencode(venc, kmax, g.izipv, c_new, v);
-      v = vdecode(vdt_new, qn, g.izipv);
-#pragma unroll
-      for (int q = 0; q < 8; q++) v = v + (double)(Cv[q] * kf * Cw[q]);
-      float av = (float)fabs(v + (double)vfield[(r0 + lo) * 3 + d]);
-      vmax[d] = fmaxf(vmax[d], av);
-      store_v(vp, 3 * p + d, g.izipv, vencode(venc, kmax, g.izipv, c_new, v));
+      for (int q = 0; q < 8; q++) {
+        Fk[d][q] = Fv[q] * kf * Fw[q];
+        Ck[d][q] = Cv[q] * kf * Cw[q];
+      }
     }
+    if (probe_f) continue;
+    // codec stages for the three components together: decode old; fine kick;
+    // encode (guess, thresholds, finish); decode new; coarse kick; encode
+    double v[3], x[3], tl[3], th[3];
+    int32_t n[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      v[d] = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[d] = v[d] + (double)Fk[d][q];
+      x[d] = c_new * v[d];
+      n[d] = venc_guess(kmax, g.izipv, x[d]);
+      tl[d] = venc_lo(venc, kmax, n[d]);
+      th[d] = venc_hi(venc, kmax, n[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      n[d] = venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]);
+      v[d] = vdecode(vdt_new, n[d], g.izipv);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[d] = v[d] + (double)Ck[d][q];
+      const float av = (float)fabs(v[d] + (double)vfield[(r0 + lo) * 3 + d]);
+      vmax[d] = fmaxf(vmax[d], av);
+      x[d] = c_new * v[d];
+      n[d] = venc_guess(kmax, g.izipv, x[d]);
+      tl[d] = venc_lo(venc, kmax, n[d]);
+      th[d] = venc_hi(venc, kmax, n[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]));
   }
   if (!probe_f) {
 #pragma unroll
