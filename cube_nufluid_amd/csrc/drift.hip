@@ -53,6 +53,7 @@ __device__ __forceinline__ WaveRun wave_run(int64_t key) {
   return w;
 }
 
+#define DP_SL 2
 __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __restrict__ xp, const void* __restrict__ vp,
                                                        const int64_t* __restrict__ cstart, int64_t n3c,
                                                        const float* __restrict__ vfield, float dt_mid, double c_old,
@@ -71,54 +72,78 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
   __syncthreads();
   const int lane = threadIdx.x & 63;
   int lmax[3] = {0, 0, 0}, bad = 0, conf = 0;
-  for (int64_t pb = cs[0]; pb < cs[DR_CH]; pb += DR_NT) {
-    const int64_t p = pb + threadIdx.x;
-    int64_t rd = -1 - lane;
-    if (p < cs[DR_CH]) {
+  // DP_SL particles (DR_NT apart) per thread and batch: their codes and
+  // decoded velocities are loaded together
+  for (int64_t pb = cs[0]; pb < cs[DR_CH]; pb += DR_NT * DP_SL) {
+    int64_t pp[DP_SL], rr[DP_SL];
+    bool ok[DP_SL];
+    double xo[DP_SL][3], vd[DP_SL][3];
+    float vf[DP_SL][3];
+#pragma unroll
+    for (int j = 0; j < DP_SL; j++) {
+      pp[j] = pb + j * DR_NT + threadIdx.x;
+      ok[j] = pp[j] < cs[DR_CH];
       int lo = 0, hi = DR_CH;
       while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
-        if (cs[mid] <= p) lo = mid; else hi = mid;
+        if (cs[mid] <= pp[j]) lo = mid; else hi = mid;
       }
-      if (cs[lo + 1] - cs[lo] > ((int64_t)1 << DR_LBITS)) bad |= 2;
-      const int64_t r = r0 + lo;
-      int32_t c[3];
-      cell_of_rank(g, r, c);
-      int del[3];
-      bool far = false;
+      if (ok[j] && cs[lo + 1] - cs[lo] > ((int64_t)1 << DR_LBITS)) bad |= 2;
+      rr[j] = r0 + lo;
+#pragma unroll
       for (int e = 0; e < 3; e++) {
-        int il = (c[e] & (g.nt - 1)) + 1;  // tile-local 1-based source index
-        if (g.lnt < 0) il = c[e] % g.nt + 1;
-        double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
-        double vreal = vdecode(vdt, load_v(vp, 3 * p + e, g.izipv), g.izipv) + (double)vfield[r * 3 + e];
-        double deltax = ((double)dt_mid * vreal) / (double)CUBE_NCELL;
-        int ig = (int)ceil(((double)il - 1.0) + xo + deltax);
-        int d = ig - il;
-        if (ig < 1 || ig > g.nt) {
-          // the destination lies in the neighbouring tile: that tile computes
-          // this particle from its own frame (source index il -+ nt)
-          int sl = ig < 1 ? il + g.nt : il - g.nt;
-          int ig2 = (int)ceil(((double)sl - 1.0) + xo + deltax);
-          int d2 = ig2 - sl;
-          if (d2 != d) {
-            conf++;
-            if (ig2 >= 1 && ig2 <= g.nt) d = d2;
-          }
-        }
-        del[e] = d;
-        int ad = d < 0 ? -d : d;
-        if (ad > dmax) far = true;
-        lmax[e] = max(lmax[e], ad);
-      }
-      if (far) {
-        bad |= 1;
-      } else {
-        disp[p] = pack_disp(del[0], del[1], del[2]);
-        rd = rank_of(g, wrapc(c[0] + del[0], g.nc), wrapc(c[1] + del[1], g.nc), wrapc(c[2] + del[2], g.nc));
+        xo[j][e] = ok[j] ? xoff(load_xu(xp, 3 * pp[j] + e, g.izipx), g.izipx) : 0.0;
+        vd[j][e] = ok[j] ? (double)load_v(vp, 3 * pp[j] + e, g.izipv) : 0.0;
+        vf[j][e] = ok[j] ? vfield[rr[j] * 3 + e] : 0.f;
       }
     }
-    const WaveRun w = wave_run(rd);
-    if (rd >= 0 && w.head == lane) atomicAdd(&cnt_new[rd], w.len);
+#pragma unroll
+    for (int j = 0; j < DP_SL; j++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) vd[j][e] = ok[j] ? vdecode(vdt, (int32_t)vd[j][e], g.izipv) : 0.0;
+#pragma unroll
+    for (int j = 0; j < DP_SL; j++) {
+      const int64_t p = pp[j];
+      int64_t rd = -1 - lane;
+      if (ok[j]) {
+        const int64_t r = rr[j];
+        int32_t c[3];
+        cell_of_rank(g, r, c);
+        int del[3];
+        bool far = false;
+        for (int e = 0; e < 3; e++) {
+          int il = (c[e] & (g.nt - 1)) + 1;  // tile-local 1-based source index
+          if (g.lnt < 0) il = c[e] % g.nt + 1;
+          double vreal = vd[j][e] + (double)vf[j][e];
+          double deltax = ((double)dt_mid * vreal) / (double)CUBE_NCELL;
+          int ig = (int)ceil(((double)il - 1.0) + xo[j][e] + deltax);
+          int d = ig - il;
+          if (ig < 1 || ig > g.nt) {
+            // the destination lies in the neighbouring tile: that tile computes
+            // this particle from its own frame (source index il -+ nt)
+            int sl = ig < 1 ? il + g.nt : il - g.nt;
+            int ig2 = (int)ceil(((double)sl - 1.0) + xo[j][e] + deltax);
+            int d2 = ig2 - sl;
+            if (d2 != d) {
+              conf++;
+              if (ig2 >= 1 && ig2 <= g.nt) d = d2;
+            }
+          }
+          del[e] = d;
+          int ad = d < 0 ? -d : d;
+          if (ad > dmax) far = true;
+          lmax[e] = max(lmax[e], ad);
+        }
+        if (far) {
+          bad |= 1;
+        } else {
+          disp[p] = pack_disp(del[0], del[1], del[2]);
+          rd = rank_of(g, wrapc(c[0] + del[0], g.nc), wrapc(c[1] + del[1], g.nc), wrapc(c[2] + del[2], g.nc));
+        }
+      }
+      const WaveRun w = wave_run(rd);
+      if (rd >= 0 && w.head == lane) atomicAdd(&cnt_new[rd], w.len);
+    }
   }
   for (int e = 0; e < 3; e++) atomicMax(&smax[e], lmax[e]);
   if (bad) atomicOr(&sbad, bad);
