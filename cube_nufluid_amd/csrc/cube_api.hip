@@ -377,11 +377,11 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   for (int d = 0; d < 3; d++) {
     void* spec = d == 2 ? h->fine.W : h->fine.X;
     if ((r = mesh_t_zconv(h, h->fine, d, h->fine.W, spec))) return r;
-    if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d], Ff[0], pf[0], Ff[1], pf[1], d == 2 ? f2f : nullptr)))
-      return r;
+    if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d], nullptr, 0, nullptr, 0, nullptr))) return r;
   }
   const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
-  if ((r = kick_all(h, Ff, pf, a_mid, dt, c_old, c_new, vmx, probe_f, probe_c))) return r;
+  // the kick also forms max |F|^2 (dt_fine) from the force slabs it stages
+  if ((r = kick_all(h, Ff, pf, a_mid, dt, c_old, c_new, vmx, f2f, probe_f, probe_c))) return r;
   const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
   for (int d = 0; nu_kick && d < 3; d++)
     if ((r = fl_gravity_fine(h, d, Ff[d], pf[d], a_mid, dt))) return r;

@@ -297,8 +297,9 @@ int coarse_kernel_build(cube_handle* h, const float* table);
 int cic_fine(cube_handle* h, double* mass);
 int cic_coarse(cube_handle* h, double* mass);
 // fine force components Ff[d] (row pitch pf[d]) and the coarse force fc
+// f2max_bits (optional): max |F|^2 over all fine nodes, from the staged slabs
 int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-             double c_new, uint32_t* vmax_bits, double* probe_f, double* probe_c);
+             double c_new, uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f, double* probe_c);
 // pp.hip
 int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe);
 // drift.hip

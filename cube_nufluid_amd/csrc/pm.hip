@@ -273,14 +273,15 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
                                                   float a_mid, float dt, double c_old, double c_new,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
                                                   const double* __restrict__ venc, int32_t kmax,
-                                                  uint32_t* vmax_bits, double* probe_f, double* probe_c) {
+                                                  uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f,
+                                                  double* probe_c) {
   constexpr int SX = 4 * BX + 1, FROW = 25 * SX, NFS = 3 * FROW;  // fine slab [3][5][5][SX]
   constexpr int CX = BX + 2, CROW = 9 * CX, NCS = 3 * CROW;       // coarse slab [3][3][3][CX]
   static_assert(CX <= 64, "a wavefront per coarse row");
   __shared__ T sF[NFS];
   __shared__ T sC[NCS];
   __shared__ int64_t cs[BX + 1];
-  __shared__ float red[3][KICK_NT / 64];
+  __shared__ float red[4][KICK_NT / 64];
   const int64_t r0 = (int64_t)blockIdx.x * BX;
   int32_t c0[3];
   cell_of_rank(g, r0, c0);
@@ -337,6 +338,19 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     }
   }
   __syncthreads();
+  // max |F|^2 over the fine nodes this workgroup owns (the slab without its
+  // far faces; the workgroups' owned nodes tile the mesh), summed in float
+  // in component order: the dt_fine input of pm.f90
+  float f2loc = 0.f;
+  for (int i = tid; i < 4 * BX * 16; i += KICK_NT) {
+    const int x = i % (4 * BX), y = (i / (4 * BX)) & 3, z = i / (16 * BX);
+    const int o = (z * 5 + y) * SX + x;
+    const float a = (float)sF[o], b = (float)sF[FROW + o], c = (float)sF[2 * FROW + o];
+    float s2 = a * a;
+    s2 = fmaf(b, b, s2);
+    s2 = fmaf(c, c, s2);
+    f2loc = fmaxf(f2loc, s2);
+  }
   const int64_t p0 = cs[0], p1 = cs[BX];
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax[3] = {0.f, 0.f, 0.f};
@@ -452,26 +466,28 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
 #pragma unroll
     for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]));
   }
-  if (!probe_f) {
+  {
+    const float mv[4] = {vmax[0], vmax[1], vmax[2], f2loc};
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      float m = vmax[d];
+    for (int d = 0; d < 4; d++) {
+      float m = mv[d];
       for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
       if ((tid & 63) == 0) red[d][tid >> 6] = m;
     }
     __syncthreads();
-    if (tid < 3) {
+    if (tid < 4) {
       float m = red[tid][0];
       for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[tid][i]);
-      atomicMax(vmax_bits + tid, __float_as_uint(m));
+      if (tid < 3 && !probe_f) atomicMax(vmax_bits + tid, __float_as_uint(m));
+      if (tid == 3 && f2max_bits) atomicMax(f2max_bits, __float_as_uint(m));
     }
   }
 }
 
 template <typename T, int BX>
 static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-                        double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, double* probe_f,
-                        double* probe_c) {
+                        double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
+                        double* probe_f, double* probe_c) {
   Geo g = geo_of(h);
   KickMeshes<T> F;
   for (int d = 0; d < 3; d++) {
@@ -481,31 +497,32 @@ static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t p
   F.c = (const T*)h->fc;
   k_kick<T, BX><<<(unsigned)(h->n3c / BX), KICK_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
                                                                     a_mid, dt, c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits,
-                                                                    probe_f, probe_c);
+                                                                    f2max_bits, probe_f, probe_c);
 }
 
 template <typename T>
 static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-                     double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, double* probe_f,
-                     double* probe_c) {
+                     double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
+                     double* probe_f, double* probe_c) {
   if (h->nt % 32 == 0)
-    kick_launch<T, 32>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+    kick_launch<T, 32>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   else if (h->nt % 16 == 0)
-    kick_launch<T, 16>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+    kick_launch<T, 16>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   else if (h->nt % 4 == 0)
-    kick_launch<T, 4>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+    kick_launch<T, 4>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   else
     return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
   return 0;
 }
 
 int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-             double c_new, uint32_t* vmax_bits, double* probe_f, double* probe_c) {
+             double c_new, uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f, double* probe_c) {
   const double* vo = vtab_for(h, c_old);
   const double* vn = vtab_for(h, c_new);
   int pr = prof_begin(h, PS_KICK);
-  int r = h->prec == 4 ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c)
-                       : kick_pick<double>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, probe_f, probe_c);
+  int r = h->prec == 4
+              ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c)
+              : kick_pick<double>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   prof_end(h, pr);
   if (r) return r;
   HIP_TRY(h, hipGetLastError());
