@@ -258,7 +258,6 @@ __device__ __forceinline__ void bitonic(uint64_t* s, int n, int npow2, int tid, 
 }
 
 #define DR_MED_LDS 1024
-#define DC_SL 4  // slots per thread and batch in k_drift_cells
 __global__ __launch_bounds__(DR_NT) void k_drift_sort_cells(const int64_t* __restrict__ cs_new, int64_t n3c, uint64_t* keys) {
   __shared__ uint64_t sk[DR_NT * DR_SMALL];  // small cells; then DR_NT/64 wavefront areas of DR_MED_LDS
   __shared__ int32_t soff[DR_NT / 64];
@@ -408,27 +407,42 @@ __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __
   }
 }
 
-// ---------------------------------------------------------------- gather
-// One thread per output slot q (checkpoint order after the drift): the
-// arrival's full velocity vreal (update_particle.f90:69), its new position
-// code (:117) and PID; vreal is kept for the per-cell pass.  Each thread
-// takes DG_SL slots (DR_NT apart) and issues each stage's loads for all of
-// them together: keys, then cell offsets, then the particle's codes, then
-// the decoded velocities.
+// ---------------------------------------------------------- gather + cells
+// One workgroup per DW_CELLS destination cells; their slots (checkpoint
+// order after the drift) are contiguous.
+//   gather (one thread per slot, DG_SL slots per batch with each stage's
+//     loads issued together): the arrival's full velocity vreal
+//     (update_particle.f90:69), its new position code (:117) and PID.
+//     vreal stays in LDS when the workgroup's slots fit (else in HBM).
+//   cells (one thread per cell): vfield_new summed in arrival order
+//     (update_particle.f90:70-83).
+//   slots (DC_SL per batch): the velocity residual re-quantised with the old
+//     sigma (:119) and the sigma_v sums (:174-200).
+// part: 3 doubles per workgroup.
+#define DW_CELLS 128
+#define DW_STAGE 1536
 #define DG_SL 4
-__global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* __restrict__ keys, int64_t np,
-                                                        const int64_t* __restrict__ cstart, const void* __restrict__ xp,
-                                                        const void* __restrict__ vp, const int32_t* __restrict__ pid,
-                                                        const float* __restrict__ vfield, float dt_mid, double c_old,
-                                                        const double* __restrict__ vdt, void* xp_o, int32_t* pid_o,
-                                                        double* vr) {
+#define DC_SL 4
+template <bool STAGED>
+__device__ __forceinline__ void drift_write_body(Geo g, const uint64_t* __restrict__ keys, int64_t r0, int ncell,
+                                                 const int64_t* __restrict__ cstart, const void* __restrict__ xp,
+                                                 const void* __restrict__ vp, const int32_t* __restrict__ pid,
+                                                 const float* __restrict__ vfield, float dt_mid, double c_old,
+                                                 const double* __restrict__ vdt, const double* __restrict__ venc,
+                                                 int32_t kmax, void* xp_o, int32_t* pid_o, double* vr, void* vp_o,
+                                                 float* vfield_o, int32_t* rhoc_o, double* part, const int64_t* scs,
+                                                 float* svf, double (*red)[DR_NT / 64], double* svr) {
+  const int64_t qa = scs[0], q1 = scs[ncell];
+  // vreal of slot q: LDS when the workgroup's slots fit, else the HBM scratch
+  auto vr_at = [&](int64_t q, int e) -> double& { return STAGED ? svr[3 * (q - qa) + e] : vr[3 * q + e]; };
+  // ---- gather
   const double xscale = (g.izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0) * (double)CUBE_NCELL;
-  for (int64_t q0 = (int64_t)blockIdx.x * DR_NT * DG_SL + threadIdx.x; q0 < np; q0 += (int64_t)gridDim.x * DR_NT * DG_SL) {
+  for (int64_t q0 = qa + threadIdx.x; q0 < q1; q0 += (int64_t)DR_NT * DG_SL) {
     uint64_t key[DG_SL];
     bool ok[DG_SL];
 #pragma unroll
     for (int j = 0; j < DG_SL; j++) {
-      ok[j] = q0 + j * DR_NT < np;
+      ok[j] = q0 + j * DR_NT < q1;
       key[j] = ok[j] ? keys[q0 + j * DR_NT] : 0;
     }
     int64_t rs[DG_SL], p[DG_SL];
@@ -466,33 +480,17 @@ __global__ __launch_bounds__(DR_NT) void k_drift_gather(Geo g, const uint64_t* _
         xn = g.izipx == 1 ? (int64_t)(int8_t)xn : (int64_t)(int16_t)xn;
         xn += (int64_t)nint_d((double)dt_mid * vreal / xscale);
         store_x(xp_o, 3 * q + e, g.izipx, xn);
-        vr[3 * q + e] = vreal;
+        vr_at(q, e) = vreal;
       }
       pid_o[q] = pv[j];
     }
   }
-}
-
-// ---------------------------------------------------------------- cells
-// One workgroup per DR_NT destination cells.  Phase 1, one thread per cell:
-// vfield_new summed in arrival order (update_particle.f90:70-83).  Phase 2,
-// one thread per slot: the velocity residual re-quantised with the old sigma
-// (:119) and the sigma_v sums (:174-200).  part: 3 doubles per workgroup.
-__global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __restrict__ cs_new, int64_t n3c,
-                                                       const float* __restrict__ vfield, const double* __restrict__ vr,
-                                                       double c_old, const double* __restrict__ vdt,
-                                                       const double* __restrict__ venc, int32_t kmax, void* vp_o,
-                                                       float* vfield_o, int32_t* rhoc_o, double* part) {
-  __shared__ int64_t scs[DR_NT + 1];
-  __shared__ float svf[DR_NT * 3];
-  __shared__ double red[3][DR_NT / 64];
-  const int64_t r0 = (int64_t)blockIdx.x * DR_NT;
-  const int ncell = (int)min((int64_t)DR_NT, n3c - r0);
-  for (int i = threadIdx.x; i <= DR_NT; i += DR_NT) scs[i] = cs_new[r0 + min(i, ncell)];
+  if (!STAGED) __threadfence_block();
   __syncthreads();
+  // ---- cells
   const double wv = (double)0.1f;  // weight_v=0.1 is a default-real literal
   double sum_c = 0, sum_r = 0, sum_s = 0;
-  if (threadIdx.x < ncell) {
+  if ((int)threadIdx.x < ncell) {
     const int64_t r = r0 + threadIdx.x;
     const int64_t b = scs[threadIdx.x], n = scs[threadIdx.x + 1] - b;
     float vf[3];
@@ -504,7 +502,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int e = 0; e < 3; e++) a[u][e] = k0 + u < n ? vr[3 * (b + k0 + u) + e] : 0.0;
+        for (int e = 0; e < 3; e++) a[u][e] = k0 + u < n ? vr_at(b + k0 + u, e) : 0.0;
 #pragma unroll
       for (int u = 0; u < U; u++)
         if (k0 + u < n)
@@ -521,10 +519,8 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
     sum_c = (double)(vfn[0] * vfn[0] + vfn[1] * vfn[1] + vfn[2] * vfn[2]);
   }
   __syncthreads();
-  // slots in DC_SL batches per thread (DR_NT apart, visited in increasing
-  // order as before); each stage's loads for the batch are issued together
-  const int64_t q1 = scs[ncell];
-  for (int64_t q0 = scs[0] + threadIdx.x; q0 < q1; q0 += DR_NT * DC_SL) {
+  // ---- slots: re-quantisation and sigma sums, DC_SL per batch
+  for (int64_t q0 = qa + threadIdx.x; q0 < q1; q0 += (int64_t)DR_NT * DC_SL) {
     bool ok[DC_SL];
     int cl[DC_SL];
     double x[DC_SL][3];
@@ -539,7 +535,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
       }
       cl[j] = lo;
 #pragma unroll
-      for (int e = 0; e < 3; e++) x[j][e] = ok[j] ? vr[3 * q + e] : 0.0;
+      for (int e = 0; e < 3; e++) x[j][e] = ok[j] ? vr_at(q, e) : 0.0;
     }
     int32_t n[DC_SL][3];
     double tlo[DC_SL][3], thi[DC_SL][3];
@@ -582,6 +578,29 @@ __global__ __launch_bounds__(DR_NT) void k_drift_cells(Geo g, const int64_t* __r
     for (int i = 0; i < DR_NT / 64; i++) t += red[threadIdx.x][i];
     part[(int64_t)blockIdx.x * 3 + threadIdx.x] = t;
   }
+}
+
+__global__ __launch_bounds__(DR_NT) void k_drift_write(Geo g, const uint64_t* __restrict__ keys, const int64_t* __restrict__ cs_new,
+                                                       int64_t n3c, const int64_t* __restrict__ cstart, const void* __restrict__ xp,
+                                                       const void* __restrict__ vp, const int32_t* __restrict__ pid,
+                                                       const float* __restrict__ vfield, float dt_mid, double c_old,
+                                                       const double* __restrict__ vdt, const double* __restrict__ venc,
+                                                       int32_t kmax, void* xp_o, int32_t* pid_o, double* vr, void* vp_o,
+                                                       float* vfield_o, int32_t* rhoc_o, double* part) {
+  __shared__ int64_t scs[DW_CELLS + 1];
+  __shared__ float svf[DW_CELLS * 3];
+  __shared__ double red[3][DR_NT / 64];
+  __shared__ double svr[3 * DW_STAGE];
+  const int64_t r0 = (int64_t)blockIdx.x * DW_CELLS;
+  const int ncell = (int)min((int64_t)DW_CELLS, n3c - r0);
+  for (int i = threadIdx.x; i <= DW_CELLS; i += DR_NT) scs[i] = cs_new[r0 + min(i, ncell)];
+  __syncthreads();
+  if (scs[ncell] - scs[0] <= DW_STAGE)
+    drift_write_body<true>(g, keys, r0, ncell, cstart, xp, vp, pid, vfield, dt_mid, c_old, vdt, venc, kmax, xp_o, pid_o, vr, vp_o,
+                           vfield_o, rhoc_o, part, scs, svf, red, svr);
+  else
+    drift_write_body<false>(g, keys, r0, ncell, cstart, xp, vp, pid, vfield, dt_mid, c_old, vdt, venc, kmax, xp_o, pid_o, vr,
+                            vp_o, vfield_o, rhoc_o, part, scs, svf, red, svr);
 }
 
 int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
@@ -627,16 +646,15 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   prof_end(h, pid4);
   HIP_TRY(h, hipGetLastError());
   int pid5 = prof_begin(h, PS_DRIFT_WRITE);
-  const int64_t ngather = std::min<int64_t>((h->np + DR_NT * DG_SL - 1) / (DR_NT * DG_SL), 1 << 20);
-  k_drift_gather<<<(unsigned)std::max<int64_t>(ngather, 1), DR_NT, 0, h->stream>>>(
-      g, keys, h->np, h->cstart, h->xp, h->vp, h->pid, h->vfield, dt_mid, c_old, vdt, h->xp2, h->pid2,
-      (double*)h->dscratch);
-  k_drift_cells<<<(unsigned)nb3, DR_NT, 0, h->stream>>>(g, h->cstart2, h->n3c, h->vfield, (const double*)h->dscratch, c_old,
-                                                         vdt, h->venc, h->vkmax, h->vp2, h->vfield2, h->rhoc2, h->dred);
+  const int64_t nbw = (h->n3c + DW_CELLS - 1) / DW_CELLS;
+  if (nbw * 3 > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
+  k_drift_write<<<(unsigned)nbw, DR_NT, 0, h->stream>>>(g, keys, h->cstart2, h->n3c, h->cstart, h->xp, h->vp, h->pid, h->vfield,
+                                                         dt_mid, c_old, vdt, h->venc, h->vkmax, h->xp2, h->pid2,
+                                                         (double*)h->dscratch, h->vp2, h->vfield2, h->rhoc2, h->dred);
   prof_end(h, pid5);
   HIP_TRY(h, hipGetLastError());
   double sums[3];
-  if ((r = reduce_doubles(h, h->dred, nb3, 3, sums))) return r;
+  if ((r = reduce_doubles(h, h->dred, nbw, 3, sums))) return r;
   int64_t npnew;
   HIP_TRY(h, hipMemcpy(&npnew, h->cstart2 + h->n3c, sizeof(int64_t), hipMemcpyDeviceToHost));
   HIP_TRY(h, hipMemcpy(hf, flags, sizeof(hf), hipMemcpyDeviceToHost));
