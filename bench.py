@@ -58,8 +58,7 @@ def algorithmic_bytes(slot, cfg, npart):
         "fft_y_fwd": 8 * mf,
         "fft_zconv": 8.5 * mf,
         "fft_y_inv": 8 * mf,
-        # the last of the three inverses also reads the other two components
-        "fft_x_c2r": 8 * mf + 8.0 / 3 * mf,
+        "fft_x_c2r": 8 * mf,
         # xp, vp read + write; three fine and three coarse force meshes,
         # vfield and cell offsets
         "kick": n * (bx + 2 * bv) + 12 * mf + 32 * mc,
