@@ -288,12 +288,27 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
   const int tid = threadIdx.x;
   const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
   for (int i = tid; i <= BX; i += KICK_NT) cs[i] = cstart[r0 + i];
-  // stage the fine slab: 75 rows of SX nodes, a wavefront per row and a
-  // lane per node, all of a lane's loads issued before its LDS stores
+  const int64_t p0 = cstart[r0], p1 = cstart[r0 + BX];
+  // this thread's first particle: its position and velocity codes are
+  // requested before the slabs and its old velocities decoded while they
+  // arrive, so those latencies overlap the staging
+  const int64_t pa = p0 + tid;
+  uint32_t xa[3] = {0u, 0u, 0u};
+  int32_t va[3] = {0, 0, 0};
+  double vo_a[3] = {0.0, 0.0, 0.0};
+  if (pa < p1) {
+#pragma unroll
+    for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pa + e, g.izipx);
+#pragma unroll
+    for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
+  }
+  // stage the fine slab (75 rows of SX nodes, a wavefront per row and a
+  // lane per node) and the coarse slab (27 rows of CX nodes), all of a
+  // lane's loads issued before its LDS stores
   {
-    constexpr int NW = KICK_NT / 64, RPW = (75 + NW - 1) / NW, XC = (SX + 63) / 64;
+    constexpr int NW = KICK_NT / 64, RPW = (75 + NW - 1) / NW, XC = (SX + 63) / 64, RPC = (27 + NW - 1) / NW;
     const int lane = tid & 63, wv = tid >> 6;
-    T v[RPW][XC];
+    T v[RPW][XC], vc[RPC];
 #pragma unroll
     for (int k = 0; k < RPW; k++) {
       const int row = wv + k * NW;  // row = (d*5 + z)*5 + y
@@ -309,6 +324,18 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       }
     }
 #pragma unroll
+    for (int k = 0; k < RPC; k++) {
+      const int row = wv + k * NW;  // row = (d*3 + z)*3 + y
+      const int y = row % 3, z = (row / 3) % 3, d = row / 9;
+      const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
+                     gz = (uint32_t)(c0[2] - 1 + z) & mc;
+      vc[k] = (row < 27 && lane < CX) ? F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx] : (T)0;
+    }
+    if (pa < p1) {
+#pragma unroll
+      for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_old, va[e], g.izipv);
+    }
+#pragma unroll
     for (int k = 0; k < RPW; k++) {
       const int row = wv + k * NW;
 #pragma unroll
@@ -317,24 +344,10 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
         if (row < 75 && x < SX) sF[row * SX + x] = v[k][j];
       }
     }
-  }
-  // coarse slab: 27 rows of CX nodes, a wavefront per row
-  {
-    constexpr int NW = KICK_NT / 64, RPW = (27 + NW - 1) / NW;
-    const int lane = tid & 63, wv = tid >> 6;
-    T v[RPW];
 #pragma unroll
-    for (int k = 0; k < RPW; k++) {
-      const int row = wv + k * NW;  // row = (d*3 + z)*3 + y
-      const int y = row % 3, z = (row / 3) % 3, d = row / 9;
-      const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
-                     gz = (uint32_t)(c0[2] - 1 + z) & mc;
-      v[k] = (row < 27 && lane < CX) ? F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx] : (T)0;
-    }
-#pragma unroll
-    for (int k = 0; k < RPW; k++) {
+    for (int k = 0; k < RPC; k++) {
       const int row = wv + k * NW;
-      if (row < 27 && lane < CX) sC[row * CX + lane] = v[k];
+      if (row < 27 && lane < CX) sC[row * CX + lane] = vc[k];
     }
   }
   __syncthreads();
@@ -351,15 +364,33 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     s2 = fmaf(c, c, s2);
     f2loc = fmaxf(f2loc, s2);
   }
-  const int64_t p0 = cs[0], p1 = cs[BX];
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax[3] = {0.f, 0.f, 0.f};
-  for (int64_t p = p0 + tid; p < p1; p += KICK_NT) {
+  for (int64_t p = pa; p < p1; p += KICK_NT) {
+    // this particle's codes and old velocities; the next particle's codes
+    // are requested now and decoded at the end of this one
+    uint32_t xu[3];
+    double vo[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      xu[e] = xa[e];
+      vo[e] = vo_a[e];
+    }
+    const int64_t pn = p + KICK_NT;
+    if (pn < p1) {
+#pragma unroll
+      for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pn + e, g.izipx);
+#pragma unroll
+      for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pn + e, g.izipv);
+    }
     int lo = 0, hi = BX;
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
       if (cs[mid] <= p) lo = mid; else hi = mid;
     }
+    float vf[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) vf[e] = vfield[(r0 + lo) * 3 + e];
     const int32_t c[3] = {c0[0] + lo, c0[1], c0[2]};
     T dx1[3], dx2[3], cx1[3], cx2[3];
     uint32_t f1[3], f2[3];
@@ -367,7 +398,7 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     for (int e = 0; e < 3; e++) {
       const int il = (g.lnt >= 0 ? (c[e] & (g.nt - 1)) : c[e] % g.nt) + 1;
       const int torg = c[e] - (il - 1);
-      const double xo = xoff(load_xu(xp, 3 * p + e, g.izipx), g.izipx);
+      const double xo = xoff(xu[e], g.izipx);
       const T tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xo);
       const int a1 = (int)floor(tx) + 1;
       dx1[e] = (T)a1 - tx;
@@ -435,28 +466,30 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     if (probe_f) continue;
     // codec stages for the three components together: decode old; fine kick;
     // encode (guess, thresholds, finish); decode new; coarse kick; encode
-    double v[3], x[3], tl[3], th[3];
+    double v[3], x[3], tl[3], th[3], vn[3];
     int32_t n[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      v[d] = vdecode(vdt_old, load_v(vp, 3 * p + d, g.izipv), g.izipv);
+      v[d] = vo[d];
 #pragma unroll
       for (int q = 0; q < 8; q++) v[d] = v[d] + (double)Fk[d][q];
       x[d] = c_new * v[d];
       n[d] = venc_guess(kmax, g.izipv, x[d]);
       tl[d] = venc_lo(venc, kmax, n[d]);
       th[d] = venc_hi(venc, kmax, n[d]);
+      // the guess's decoded value, requested with its thresholds
+      vn[d] = vdecode(vdt_new, n[d], g.izipv);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      n[d] = venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]);
-      v[d] = vdecode(vdt_new, n[d], g.izipv);
+      const int32_t m = venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]);
+      v[d] = m == n[d] ? vn[d] : vdecode(vdt_new, m, g.izipv);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
       for (int q = 0; q < 8; q++) v[d] = v[d] + (double)Ck[d][q];
-      const float av = (float)fabs(v[d] + (double)vfield[(r0 + lo) * 3 + d]);
+      const float av = (float)fabs(v[d] + (double)vf[d]);
       vmax[d] = fmaxf(vmax[d], av);
       x[d] = c_new * v[d];
       n[d] = venc_guess(kmax, g.izipv, x[d]);
@@ -465,6 +498,10 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]));
+    if (pn < p1) {
+#pragma unroll
+      for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_old, va[e], g.izipv);
+    }
   }
   {
     const float mv[4] = {vmax[0], vmax[1], vmax[2], f2loc};
