@@ -29,6 +29,7 @@ SOURCES = {
     "pk.hip": [],
     "serve.hip": ["-ffp-contract=off"],
     "fluid.hip": ["-ffp-contract=off"],
+    "halo.hip": ["-ffp-contract=off"],
 }
 
 COMMON = [

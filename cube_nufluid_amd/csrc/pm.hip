@@ -75,11 +75,15 @@ __device__ __forceinline__ int seg_find(const int32_t* off, int n, int32_t t) {
 // of the order in which threads add them; the node value is the sum rounded
 // once to float.
 #define CIC_FIX 1099511627776.0  // 2^40
-template <int BC, typename T>
+// CENTRE: the halo finder's density (halofind.f90:159), whose nodes sit at
+// fine-cell centres ('-0.5'): a particle in cell c then touches nodes
+// 4c-1..4c+4 and the block gathers cells [c0-1, c0+BC].  R has row pitch
+// `pitch`.
+template <int BC, typename T, bool CENTRE = false>
 __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restrict__ xp, const int64_t* __restrict__ cstart,
-                                                     float mass_f, T* R, int64_t N, double* part) {
+                                                     float mass_f, T* R, int64_t N, int64_t pitch, double* part) {
   const T mass = (T)mass_f;
-  constexpr int F = 4 * BC, SPAN = BC + 1, NCELL = SPAN * SPAN * SPAN;
+  constexpr int F = 4 * BC, SPAN = BC + (CENTRE ? 2 : 1), NCELL = SPAN * SPAN * SPAN;
   __shared__ unsigned long long s[F * F * F];
   __shared__ int64_t beg[NCELL];
   __shared__ int32_t off[NCELL + 1];
@@ -105,7 +109,11 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
     for (int e = 0; e < 3; e++) {
       int il = l0[e] + rel[e] + 1;  // tile-local 1-based cell index (0 = left ghost)
       uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
-      T tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
+      T tx;
+      if constexpr (CENTRE)
+        tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx) - 0.5);
+      else
+        tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
       int a1 = (int)floor(tx) + 1;
       dx1[e] = (T)a1 - tx;
       dx2[e] = 1 - dx1[e];
@@ -129,7 +137,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
     int x = i % F, y = (i / F) % F, z = i / (F * F);
     int64_t gx = 4 * c0x + x, gy = 4 * c0y + y, gz = 4 * c0z + z;
     double vd = (double)(long long)s[i] * (1.0 / CIC_FIX);
-    R[(gz * N + gy) * (N + 2) + gx] = (T)vd;
+    R[(gz * N + gy) * pitch + gx] = (T)vd;
     ms += vd;
   }
   for (int off2 = 32; off2 > 0; off2 >>= 1) ms += __shfl_xor(ms, off2);
@@ -218,13 +226,25 @@ int cic_fine(cube_handle* h, double* mass) {
   if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   int pr = prof_begin(h, PS_CIC_F);
   if (h->prec == 4)
-    k_cic_fine<4, float><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N, h->dred);
+    k_cic_fine<4, float><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (float*)h->fine.R, h->fine.N,
+                                                                h->fine.N + 2, h->dred);
   else
     k_cic_fine<4, double><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (double*)h->fine.R, h->fine.N,
-                                                                 h->dred);
+                                                                 h->fine.N + 2, h->dred);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
+  return 0;
+}
+// halo-finder density at fine-cell centres into out (nf^3 float, pitch nf)
+int cic_halo_density(cube_handle* h, float* out) {
+  Geo g = geo_of(h);
+  if (h->nt % 4) return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
+  int64_t nb = (h->nc / 4) * (h->nc / 4) * (h->nc / 4);
+  if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
+  k_cic_fine<4, float, true><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, out, h->fine.N, h->fine.N,
+                                                                    h->dred);
+  HIP_TRY(h, hipGetLastError());
   return 0;
 }
 int cic_coarse(cube_handle* h, double* mass) {

@@ -297,6 +297,16 @@ class Cube:
         _lib.check(self.lib, self.h, self.lib.cube_probe_pp_forces(self.h, _lib.ptr(fp)))
         return fp
 
+    def halofind(self, hp=None, nq=None, nhid=0, max_halos=1 << 20, max_pid_words=None):
+        """Spherical-overdensity halo catalogue of the current state
+        (halofind.f90:18-598) as a HaloCatalog.  nq: pid lattice size per
+        dimension (the reference numbers the fine grid, nq = nf)."""
+        from .halo import HaloParams, find_halos
+        hp = (hp or HaloParams()).for_box(self.cfg, nq)
+        if max_pid_words is None:
+            max_pid_words = 2 * max_halos + int(self.sim.nplocal) + 16
+        return find_halos(self.lib, self.h, hp, max_halos=max_halos, max_pid_words=max_pid_words, nhid=nhid)
+
     def probe_density(self):
         nf = self.cfg.nf
         rho = np.empty((nf, nf, nf), dtype=np.float32)

@@ -166,6 +166,40 @@ int cube_probe_density(cube_handle* h, float* rho);
    Overwrites the fine mesh (recomputed by the next cube_pm). */
 int cube_power_spectrum(cube_handle* h, int nbin, double* modes, double* ksum, double* delta2sum);
 
+/* Runtime halo finder -- halofind.f90:18-598 (called from main.f90:88-90
+   at the z_halofind redshifts): spherical-overdensity halos of the current
+   state, tile by tile as the reference runs it.  Parameters are the
+   constants of halofind.f90:27-40 (den_odc = 200 as odc_vir is .false.),
+   h0 and box (parameters.f90) and the line-of-sight centre 'cen' (:58).
+   nq: pids are q-lattice indices + 1 on an nq^3 lattice of spacing nf/nq
+   fine cells (the reference's IC numbers the fine grid: nq = nf). */
+typedef struct cube_halo_params {
+  float den_odc, den_peak_cutoff;
+  int32_t min_halo_particles, nc_halo_max, newbox_max, search_ratio, refine_ratio, pad;
+  int64_t max_halo_np, nq;
+  float h0, box, los_centre[3], pad2;
+} cube_halo_params;
+/* Host buffers for the catalogue: halos[max_halos][42] receives the
+   type_halo_info records (halofind.f90:9-15, in the order the reference
+   writes them to the 'halo' file), pid_words the 'halo_pid' stream (per
+   halo: i_odc, its pids in radius order, 0), hid (optional, nhid int16
+   entries) the 'hid' array indexed by pid-1.  On return nhalo, npid_words,
+   npeaks (candidate peaks over all tiles) and nprocessed are set. */
+typedef struct cube_halo_catalog {
+  int64_t max_halos;
+  float* halos;
+  int64_t max_pid_words;
+  int32_t* pid_words;
+  int64_t nhid;
+  int16_t* hid;
+  int64_t nhalo, npid_words, npeaks, nprocessed;
+} cube_halo_catalog;
+/* Uses the fine mesh as scratch (recomputed by the next cube_pm).  Errors:
+   CUBE_ERR_CAPACITY when a catalogue buffer or a search list is too small,
+   CUBE_ERR_CHECK for the reference's own stops (search box beyond the tile
+   buffer, newbox > newbox_max). */
+int cube_halofind(cube_handle* h, const cube_halo_params* p, cube_halo_catalog* out);
+
 /* Seeded Zel'dovich initial conditions written straight into the resident
    state -- the role of the reference's IC tool (util/initial_conditions.f90:
    214-954), which writes CUBE's first checkpoint.  Noise: unit Gaussian per

@@ -56,6 +56,9 @@ def load():
         "orc_nu_get": (None, [_P, ctypes.c_int, _P]),
         "orc_nu_evolve": (ctypes.c_int, [_P, _f32, ctypes.c_int, _f32, _P]),
         "ofl_evolve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _f32, _f32, _f32, ctypes.c_int, _P]),
+        "ohf_halofind": (ctypes.c_int, [_P, _P, _i64, _P, _P, _P, _P, _P, _i64, _P, _i64, _P, _i64, _P, _P, ctypes.c_char_p]),
+        "ohf_density": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+        "ohf_offset_table": (_i64, [_i64, _P, _i64]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
@@ -225,3 +228,50 @@ def fluid_evolve(fld, ncb, gamma, dt, dx, direction, dtg=1000.0):
     if rc:
         raise RuntimeError("ofl_evolve failed")
     return f, float(d[0])
+
+
+def halo_arrays(cfg, st, hp):
+    """Parameter arrays of ohf_halofind for state st and HaloParams hp."""
+    ip = np.array([cfg.nc, cfg.nt, cfg.nnt, cfg.ncb, cfg.izipx, cfg.izipv, hp.nc_halo_max, hp.newbox_max,
+                   hp.max_halo_np, hp.search_ratio, hp.refine_ratio, hp.min_halo_particles, hp.nq], dtype=np.int64)
+    s = st.sim
+    fp = np.array([hp.den_odc, hp.den_peak_cutoff, s.mass_p_cdm, s.sigma_vi, s.a, s.vsim2phys, hp.h0, hp.box,
+                   *hp.los_centre], dtype=np.float32)
+    return ip, fp
+
+
+def halofind(cfg, st, hp, max_halos=100000, nhid=0):
+    """CPU restatement of halofind.f90 on state st: (halos [n,42] float32,
+    pid stream int32, hid int16 or None, counts)."""
+    lib = load()
+    ip, fp = halo_arrays(cfg, st, hp)
+    n = st.sim.nplocal
+    halos = np.zeros((max_halos, 42), dtype=np.float32)
+    pidw = np.zeros(2 * max_halos + n + 16, dtype=np.int32)
+    hid = np.zeros(nhid, dtype=np.int16) if nhid else None
+    counts = np.zeros(4, dtype=np.int64)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.ohf_halofind(p(ip), p(fp), n, p(np.ascontiguousarray(st.xp)), p(np.ascontiguousarray(st.vp)),
+                          p(np.ascontiguousarray(st.pid)), p(np.ascontiguousarray(st.rhoc)),
+                          p(np.ascontiguousarray(st.vfield)), max_halos, p(halos), len(pidw), p(pidw), nhid, p(hid),
+                          p(counts), err)
+    if rc:
+        raise RuntimeError("halo oracle: " + err.value.decode())
+    return halos[:counts[0]].copy(), pidw[:counts[1]].copy(), hid, counts.copy()
+
+
+def halo_density(cfg, st, hp):
+    lib = load()
+    ip, fp = halo_arrays(cfg, st, hp)
+    nf = cfg.nf
+    rho = np.zeros((nf, nf, nf), dtype=np.float32)
+    lib.ohf_density(p(ip), p(fp), p(np.ascontiguousarray(st.xp)), p(np.ascontiguousarray(st.rhoc)), p(rho))
+    return rho
+
+
+def halo_offset_table(R):
+    lib = load()
+    cap = (2 * R + 1) ** 3
+    out = np.zeros((cap, 4), dtype=np.int32)
+    n = lib.ohf_offset_table(R, p(out), cap)
+    return out[:n].copy()
