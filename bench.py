@@ -141,6 +141,72 @@ def cpu_baseline(cfg, st, steps_dt, sample_tiles, threads):
                                                                          t1 - t0, t2 - t1, t3 - t2, t_glob)}
 
 
+def halofind_c3(cube, name):
+    """One cube_halofind call (halofind.f90) on the state left by the timed
+    steps: wall time of the synchronous call, halos and peaks found."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    try:
+        cat = cube.halofind(max_halos=1 << 21)
+    except Exception as e:  # reported, not fatal to the step measurement
+        log("halofind failed: %s" % e)
+        return {"error": str(e)}
+    ms = (time.perf_counter() - t0) * 1e3
+    members = int(cat.field("mass_odc").sum()) if cat.nhalo else 0
+    log("halofind (%s): %d halos (%d particles) from %d candidate peaks in %.1f ms"
+        % (name, cat.nhalo, members, cat.npeaks, ms))
+    return {"workload": "%s state after the timed steps" % name, "ms": round(ms, 2), "nhalo": cat.nhalo,
+            "particles_in_halos": members, "candidate_peaks": cat.npeaks, "processed_peaks": cat.nprocessed}
+
+
+def halofind_evolved(config, seed, cpu):
+    """The halo finder where the reference runs it: a box evolved from z=49
+    to z=0 by the GPU step, halofind at the final step (z_halofind.txt =
+    1.0, 0.0 as shipped, whose first entry used is z=0).  GPU wall time of
+    the call; with cpu, the C restatement timed on the same state (one
+    thread) and its catalogue compared."""
+    from cube_nufluid_amd.params import CONFIGS
+    from cube_nufluid_amd.driver import Cube
+    cfg = CONFIGS[config]
+    t0 = time.perf_counter()
+    cube = Cube.from_ic(cfg, seed, 49.0, z_checkpoints=[49.0, 0.0], z_halofind=[1.0, 0.0])
+    try:
+        nsteps = cube.run()
+        t_run = time.perf_counter() - t0
+        if not cube.halo_catalogs:
+            return {"error": "no halofind step reached"}
+        z, cat = cube.halo_catalogs[-1]
+        ms = cube.timings["halofind"][-1] * 1e3
+        members = int(cat.field("mass_odc").sum()) if cat.nhalo else 0
+        log("halofind (%s evolved, %d steps in %.1f s): %d halos (%d particles) from %d candidate peaks in %.1f ms"
+            % (config, nsteps, t_run, cat.nhalo, members, cat.npeaks, ms))
+        out = {"workload": "%s: %d^3 particles from z=49 to z=0 (%d steps), halofind at z=0"
+                           % (config, round(cube.sim.nplocal ** (1 / 3)), nsteps),
+               "ms": round(ms, 2), "nhalo": cat.nhalo, "particles_in_halos": members,
+               "candidate_peaks": cat.npeaks, "processed_peaks": cat.nprocessed}
+        if cpu:
+            st = cube.download()
+            st.sim.a = cube.sim.a
+            st.sim.vsim2phys = cube.sim.vsim2phys
+    finally:
+        cube.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        from cube_nufluid_amd.halo import HaloParams
+        hp = HaloParams().for_box(cfg, None, st.sim)
+        t1 = time.perf_counter()
+        halos, words, _, counts = oracle_lib.halofind(cfg, st, hp, max_halos=1 << 20)
+        cpu_s = time.perf_counter() - t1
+        same = len(halos) == cat.nhalo and np.array_equal(words, cat.pid_words)
+        log("halofind cpu (restatement, 1 thread): %.1f s, %d halos, catalogue membership identical: %s"
+            % (cpu_s, len(halos), same))
+        out["cpu"] = {"seconds": round(cpu_s, 2), "cores": 1, "kind": "port", "nhalo": len(halos),
+                      "same_members_as_gpu": bool(same)}
+    return out
+
+
 def ensemble_bench(n, rate):
     """Ensemble serving: n seeded LCDM PM+PP boxes drawn uniformly from
     {32,64,128,256}^3 particles (mesh 2x particles per dimension), Zel'dovich
@@ -176,6 +242,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1003)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
+    ap.add_argument("--no-halofind", action="store_true", help="skip the halo-finder measurement")
     ap.add_argument("--ens-requests", type=int, default=16)
     ap.add_argument("--ens-rate", type=float, default=6.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--copy-calib", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
@@ -272,8 +339,12 @@ def main():
     st_cpu = None
     if not args.no_cpu_baseline:
         st_cpu = cube.download()
+    if not args.no_halofind:
+        out["halofind"] = {"state": halofind_c3(cube, args.config)}
     last_dt = float(cube.dt) or 0.05
     cube.close()
+    if not args.no_halofind:
+        out["halofind"]["evolved"] = halofind_evolved("C2", 1002, cpu=not args.no_cpu_baseline)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ens_requests, args.ens_rate)
     if st_cpu is not None:

@@ -38,7 +38,7 @@
 
 int cic_halo_density(cube_handle* h, float* out);
 
-static_assert(sizeof(cube_halo_params) == 72, "cube_halo_params layout");
+static_assert(sizeof(cube_halo_params) == 80, "cube_halo_params layout");
 static_assert(sizeof(cube_halo_catalog) == 80, "cube_halo_catalog layout");
 
 namespace {
@@ -832,6 +832,7 @@ extern "C" int cube_halofind(cube_handle* h, const cube_halo_params* hp, cube_ha
   if (hp->nc_halo_max < 1 || hp->nc_halo_max > 160) return cube_fail(h, CUBE_ERR_ARG, "nc_halo_max must lie in 1..160");
   if (h->nt < 4) return cube_fail(h, CUBE_ERR_ARG, "halo finder needs nt >= 4");
   if (!(hp->den_peak_cutoff >= 0)) return cube_fail(h, CUBE_ERR_ARG, "den_peak_cutoff must be >= 0");
+  if (!(hp->a > 0)) return cube_fail(h, CUBE_ERR_ARG, "halo finder needs the scale factor a > 0");
   if (hp->refine_ratio < 1 || hp->search_ratio < 1 || hp->nq < 1 || hp->max_halo_np < 2)
     return cube_fail(h, CUBE_ERR_ARG, "invalid halo finder parameters");
   if (!h->fine.S) return cube_fail(h, CUBE_ERR_ARG, "halo finder needs the fine mesh");
@@ -1048,11 +1049,10 @@ extern "C" int cube_halofind(cube_handle* h, const cube_halo_params* hp, cube_ha
       e = hipMalloc(&hidd, sizeof(int16_t) * out->nhid);
       if (e == hipSuccess) e = hipMemsetAsync(hidd, 0, sizeof(int16_t) * out->nhid, st);
     }
-    const cube_sim_header& sh = h->sim;
     if (e == hipSuccess) {
       const unsigned nb1 = (unsigned)std::min<int64_t>(npk, 65535);
       k_halo_cat1<<<nb1, HNT, 0, st>>>(g, c, npk, pk, res, is_halo, hidx, woff, mq, mrk, mbit, h->xp, h->vp, h->vfield, vdt,
-                                       h->pid, hp->box, hp->h0, sh.a, sh.vsim2phys, hp->los_centre[0], hp->los_centre[1],
+                                       h->pid, hp->box, hp->h0, hp->a, hp->vsim2phys, hp->los_centre[0], hp->los_centre[1],
                                        hp->los_centre[2], rec, sums, words, hidd, hidd ? out->nhid : 0);
       k_halo_seq1<<<1, 1, 0, st>>>(nh, sums, c.N, rec, qmean);
       k_halo_cat2<<<nb1, HNT, 0, st>>>(c, npk, res, is_halo, hidx, mq, h->pid, qmean, iq);

@@ -21,7 +21,7 @@ RA_MAX = f32(0.2)     # parameters.f90:110
 class Cube:
     """One box resident on the GPU, driven like CUBE's main program."""
 
-    def __init__(self, cfg, state, device=0, np_max=None, precision=4, z_checkpoints=None):
+    def __init__(self, cfg, state, device=0, np_max=None, precision=4, z_checkpoints=None, z_halofind=None):
         self.lib = _lib.load()
         self.cfg = cfg
         p = _lib.CubeParams()
@@ -40,10 +40,10 @@ class Cube:
         if state is None:
             raise ValueError("a state is required; use Cube.from_ic for a generated start")
         self.upload(state)
-        self._init_time(state.sim.a, z_checkpoints)
+        self._init_time(state.sim.a, z_checkpoints, z_halofind)
 
     @classmethod
-    def from_ic(cls, cfg, seed, z, device=0, z_checkpoints=None, np_max=None):
+    def from_ic(cls, cfg, seed, z, device=0, z_checkpoints=None, np_max=None, z_halofind=None):
         """Generate seeded Zel'dovich initial conditions directly in HBM
         (cube_ic_zeldovich) and return the resident box."""
         from .ic import power_table, velocity_factor
@@ -74,10 +74,10 @@ class Cube:
         sim.h0, sim.omega_m, sim.omega_l, sim.s8 = cfg.h0, cfg.omega_m, cfg.omega_l, cfg.sigma8
         sim.vsim2phys = (150.0 / a) * cfg.box * np.sqrt(cfg.omega_m) / cfg.nf
         self.sim = sim
-        self._init_time(np.float32(a), z_checkpoints or [z, 0.0])
+        self._init_time(np.float32(a), z_checkpoints or [z, 0.0], z_halofind)
         return self
 
-    def _init_time(self, a0, z_checkpoints):
+    def _init_time(self, a0, z_checkpoints, z_halofind=None):
         # initialize.f90:40-53 and particle_initialization.f90:136-138
         self.z_checkpoints = list(z_checkpoints) if z_checkpoints else [float(1.0 / a0 - 1.0), 0.0]
         self.cur_checkpoint = 1
@@ -92,6 +92,14 @@ class Cube:
         self.checkpoint_step = False
         self.final_step = False
         self.cur_checkpoint += 1  # main.f90:42
+        # HALOFIND (initialize.f90:62-71, main.f90:43): the list as read from
+        # z_halofind.txt; the main loop starts at entry cur_checkpoint+1, and
+        # entries past the list read as 0 (the unset array)
+        self.z_halofind = list(z_halofind) if z_halofind else None
+        self.cur_halofind = self.cur_checkpoint + 1
+        self.halofind_step = False
+        self.halo_params = None
+        self.halo_catalogs = []
         self.timings = {}
 
     # ------------------------------------------------------------- state
@@ -189,14 +197,23 @@ class Cube:
         d1, d2 = self.expansion(self.a, dt)
         da = f32(d1 + d2)
         self.checkpoint_step = False
+        self.halofind_step = False
         # initialize.f90:49 fills z_checkpoint with -0.5 beyond the list
         zc = self.z_checkpoints
-        z_next = f32(zc[self.cur_checkpoint - 1] if self.cur_checkpoint <= len(zc) else -0.5)
+        z_ck = f32(zc[self.cur_checkpoint - 1] if self.cur_checkpoint <= len(zc) else -0.5)
+        z_next = z_ck
+        if self.z_halofind is not None:  # timestep.f90:41-42
+            zh = self.z_halofind
+            z_h = f32(zh[self.cur_halofind - 1] if self.cur_halofind <= len(zh) else 0.0)
+            z_next = max(z_ck, z_h)
         a_next = f32(f32(1.0) / f32(1 + z_next))
         if da >= f32(a_next - self.a):
-            self.checkpoint_step = True
-            if self.cur_checkpoint == len(self.z_checkpoints):
-                self.final_step = True
+            if z_next == z_ck:
+                self.checkpoint_step = True
+                if self.cur_checkpoint == len(self.z_checkpoints):
+                    self.final_step = True
+            if self.z_halofind is not None and z_next == z_h:
+                self.halofind_step = True
             it = 0
             while abs(f32(f32(self.a + da) / a_next) - f32(1)) >= f32(1e-6) and it < 100:  # real(4) 1e-6
                 dt = f32(f32(dt * f32(a_next - self.a)) / da)  # dt*(a_next-a)/da, left to right
@@ -247,20 +264,40 @@ class Cube:
         self.particle_mesh()
         if nu:  # main.f90:66-70
             self.fluid_evolve(f32(self.dt / 2), -1, self.a_mid)
-        if self.checkpoint_step:
+        if self.checkpoint_step or self.halofind_step:  # main.f90:72-95
             self.dt_old = f32(0)
             self.update_x()
-            if opath is not None:
-                st = self.download()
-                st.sim.cur_checkpoint = self.cur_checkpoint
-                st.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
-                write_checkpoint(st, opath, self.z_checkpoints[self.cur_checkpoint - 1])
-            self.cur_checkpoint += 1
+            if self.checkpoint_step:
+                if opath is not None:
+                    st = self.download()
+                    st.sim.cur_checkpoint = self.cur_checkpoint
+                    st.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+                    write_checkpoint(st, opath, self.z_checkpoints[self.cur_checkpoint - 1])
+                self.cur_checkpoint += 1
+            if self.halofind_step:
+                self._halofind_step(opath)
+                self.cur_halofind += 1
             if self.final_step:
                 return True
             self.dt = f32(0)
         self.timings.setdefault("step", []).append(time.perf_counter() - t0)
         return False
+
+    def _halofind_step(self, opath):
+        """halofind.f90 at z_halofind(cur_halofind): the catalogue is kept in
+        halo_catalogs and, with an output path, written as the reference's
+        halo, halo_pid and hid files."""
+        from .halo import halo_prefix, write_halo_files
+        zh = self.z_halofind
+        z = float(zh[self.cur_halofind - 1]) if self.cur_halofind <= len(zh) else 0.0
+        self.sim.a = self.a
+        self.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+        t0 = time.perf_counter()
+        cat = self.halofind(self.halo_params, nhid=self.cfg.nf ** 3 if opath is not None else 0)
+        self.timings.setdefault("halofind", []).append(time.perf_counter() - t0)
+        self.halo_catalogs.append((z, cat))
+        if opath is not None:
+            write_halo_files(halo_prefix(opath, z), cat)
 
     def run(self, max_steps=2500, opath=None):
         n = 0
@@ -302,7 +339,7 @@ class Cube:
         (halofind.f90:18-598) as a HaloCatalog.  nq: pid lattice size per
         dimension (the reference numbers the fine grid, nq = nf)."""
         from .halo import HaloParams, find_halos
-        hp = (hp or HaloParams()).for_box(self.cfg, nq)
+        hp = (hp or HaloParams()).for_box(self.cfg, nq, self.sim)
         if max_pid_words is None:
             max_pid_words = 2 * max_halos + int(self.sim.nplocal) + 16
         return find_halos(self.lib, self.h, hp, max_halos=max_halos, max_pid_words=max_pid_words, nhid=nhid)

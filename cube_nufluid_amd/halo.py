@@ -45,13 +45,19 @@ class HaloParams:
     h0: float = 0.67
     box: float = 0.0
     los_centre: tuple = (30.0, 370.0, 370.0)   # halofind.f90:58 'cen'
+    a: float = 0.0               # sim%a and sim%vsim2phys of the current state
+    vsim2phys: float = 0.0
 
-    def for_box(self, cfg, nq=None):
-        """Copy with box and pid lattice filled in for box cfg.  The
-        reference's pids index the fine grid (nq = nf); the IC writer here
-        numbers its particle lattice (nq = particles per dimension)."""
+    def for_box(self, cfg, nq=None, sim=None):
+        """Copy with box, pid lattice and (from header sim) a and vsim2phys
+        filled in for box cfg.  The reference's pids index the fine grid
+        (nq = nf); the IC writer here numbers its particle lattice (nq =
+        particles per dimension)."""
         import dataclasses
-        return dataclasses.replace(self, box=self.box or cfg.box, nq=nq or self.nq or cfg.nf)
+        a, vs = self.a, self.vsim2phys
+        if sim is not None:
+            a, vs = a or float(sim.a), vs or float(sim.vsim2phys)
+        return dataclasses.replace(self, box=self.box or cfg.box, nq=nq or self.nq or cfg.nf, a=a, vsim2phys=vs)
 
 
 class cube_halo_params(ctypes.Structure):
@@ -61,7 +67,7 @@ class cube_halo_params(ctypes.Structure):
                 ("refine_ratio", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("max_halo_np", ctypes.c_int64), ("nq", ctypes.c_int64),
                 ("h0", ctypes.c_float), ("box", ctypes.c_float), ("los_centre", ctypes.c_float * 3),
-                ("pad2", ctypes.c_float)]
+                ("a", ctypes.c_float), ("vsim2phys", ctypes.c_float), ("pad2", ctypes.c_float)]
 
 
 class cube_halo_catalog(ctypes.Structure):
@@ -105,7 +111,7 @@ def params_struct(hp):
     s.min_halo_particles, s.nc_halo_max, s.newbox_max = hp.min_halo_particles, hp.nc_halo_max, hp.newbox_max
     s.search_ratio, s.refine_ratio = hp.search_ratio, hp.refine_ratio
     s.max_halo_np, s.nq = hp.max_halo_np, hp.nq
-    s.h0, s.box = hp.h0, hp.box
+    s.h0, s.box, s.a, s.vsim2phys = hp.h0, hp.box, hp.a, hp.vsim2phys
     for d in range(3):
         s.los_centre[d] = hp.los_centre[d]
     return s

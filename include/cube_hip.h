@@ -170,14 +170,16 @@ int cube_power_spectrum(cube_handle* h, int nbin, double* modes, double* ksum, d
    at the z_halofind redshifts): spherical-overdensity halos of the current
    state, tile by tile as the reference runs it.  Parameters are the
    constants of halofind.f90:27-40 (den_odc = 200 as odc_vir is .false.),
-   h0 and box (parameters.f90) and the line-of-sight centre 'cen' (:58).
-   nq: pids are q-lattice indices + 1 on an nq^3 lattice of spacing nf/nq
-   fine cells (the reference's IC numbers the fine grid: nq = nf). */
+   h0 and box (parameters.f90) and the line-of-sight centre 'cen' (:58);
+   a and vsim2phys are the header values sim%a and sim%vsim2phys that
+   halofind.f90:438 reads.  nq: pids are q-lattice indices + 1 on an nq^3
+   lattice of spacing nf/nq fine cells (the reference's IC numbers the fine
+   grid: nq = nf). */
 typedef struct cube_halo_params {
   float den_odc, den_peak_cutoff;
   int32_t min_halo_particles, nc_halo_max, newbox_max, search_ratio, refine_ratio, pad;
   int64_t max_halo_np, nq;
-  float h0, box, los_centre[3], pad2;
+  float h0, box, los_centre[3], a, vsim2phys, pad2;
 } cube_halo_params;
 /* Host buffers for the catalogue: halos[max_halos][42] receives the
    type_halo_info records (halofind.f90:9-15, in the order the reference

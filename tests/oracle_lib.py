@@ -235,8 +235,8 @@ def halo_arrays(cfg, st, hp):
     ip = np.array([cfg.nc, cfg.nt, cfg.nnt, cfg.ncb, cfg.izipx, cfg.izipv, hp.nc_halo_max, hp.newbox_max,
                    hp.max_halo_np, hp.search_ratio, hp.refine_ratio, hp.min_halo_particles, hp.nq], dtype=np.int64)
     s = st.sim
-    fp = np.array([hp.den_odc, hp.den_peak_cutoff, s.mass_p_cdm, s.sigma_vi, s.a, s.vsim2phys, hp.h0, hp.box,
-                   *hp.los_centre], dtype=np.float32)
+    fp = np.array([hp.den_odc, hp.den_peak_cutoff, s.mass_p_cdm, s.sigma_vi, hp.a or s.a, hp.vsim2phys or s.vsim2phys,
+                   hp.h0, hp.box, *hp.los_centre], dtype=np.float32)
     return ip, fp
 
 

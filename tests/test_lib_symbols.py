@@ -39,6 +39,6 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.EnsParams) == 16
     assert ctypes.sizeof(_lib.EnsResult) == 192
     from cube_nufluid_amd import halo
-    assert ctypes.sizeof(halo.cube_halo_params) == 72
+    assert ctypes.sizeof(halo.cube_halo_params) == 80
     assert ctypes.sizeof(halo.cube_halo_catalog) == 80
     assert ctypes.sizeof(_lib.NuParams) == 32
