@@ -291,7 +291,9 @@ class Cube:
         zh = self.z_halofind
         z = float(zh[self.cur_halofind - 1]) if self.cur_halofind <= len(zh) else 0.0
         self.sim.a = self.a
-        self.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+        # checkpoint.f90:30 in real(4)
+        self.sim.vsim2phys = f32(f32(f32(f32(f32(1.5) / self.a) * f32(self.cfg.box)) * f32(100.0))
+                                 * np.sqrt(f32(self.cfg.omega_m))) / f32(self.cfg.nf)
         t0 = time.perf_counter()
         cat = self.halofind(self.halo_params, nhid=self.cfg.nf ** 3 if opath is not None else 0)
         self.timings.setdefault("halofind", []).append(time.perf_counter() - t0)

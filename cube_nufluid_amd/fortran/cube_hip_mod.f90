@@ -50,6 +50,25 @@ module cube_hip
     real(c_double) :: mass_f, mass_c
   end type
 
+  !! halofind.f90:27-40 constants and the header values it reads (80 bytes)
+  type, bind(C) :: cube_halo_params
+    real(c_float) :: den_odc, den_peak_cutoff
+    integer(c_int32_t) :: min_halo_particles, nc_halo_max, newbox_max, search_ratio, refine_ratio, pad
+    integer(c_int64_t) :: max_halo_np, nq
+    real(c_float) :: h0, box, los_centre(3), a, vsim2phys, pad2
+  end type
+
+  !! host buffers of the halo, halo_pid and hid outputs (80 bytes)
+  type, bind(C) :: cube_halo_catalog
+    integer(c_int64_t) :: max_halos
+    type(c_ptr) :: halos
+    integer(c_int64_t) :: max_pid_words
+    type(c_ptr) :: pid_words
+    integer(c_int64_t) :: nhid
+    type(c_ptr) :: hid
+    integer(c_int64_t) :: nhalo, npid_words, npeaks, nprocessed
+  end type
+
   interface
     integer(c_int) function cube_init(p, kf, kc, h) bind(C, name="cube_init")
       import :: c_int, c_float, c_ptr, cube_params
@@ -101,6 +120,12 @@ module cube_hip
       type(c_ptr), value :: h
       character(kind=c_char), intent(out) :: buf(*)
       integer(c_size_t), value :: n
+    end function
+    integer(c_int) function cube_halofind(h, hp, cat) bind(C, name="cube_halofind")
+      import :: c_int, c_ptr, cube_halo_params, cube_halo_catalog
+      type(c_ptr), value :: h
+      type(cube_halo_params), intent(in) :: hp
+      type(cube_halo_catalog), intent(inout) :: cat
     end function
     integer(c_int) function cube_finalize(h) bind(C, name="cube_finalize")
       import :: c_int, c_ptr

@@ -3,7 +3,8 @@
 !! Same program shape as CUBE's main (main/main.f90:19-100): initialise,
 !! read the checkpoint at z_checkpoint(1), then loop timestep -> update_x ->
 !! [ext_pp_force] -> particle_mesh, writing a checkpoint (after the half
-!! drift with dt_old=0) at each redshift of z_checkpoint.txt.  The step
+!! drift with dt_old=0) at each redshift of z_checkpoint.txt and running the
+!! halo finder at those of z_halofind.txt (HALOFIND).  The step
 !! routines are the library's entry points; the time stepping (timestep.f90)
 !! stays on the host in real(4) as in the reference.
 !!
@@ -21,8 +22,15 @@ program cube_main
   real :: rsoft = 0.3, omega_m = 0.31, omega_l = 0.69, wde = -1, box = 100, h0 = 0.68
   real, parameter :: ra_max = 0.2, dt_max = 1
   character(256) :: opath = 'output/', kernel_dir = 'kernels/', zfile = 'z_checkpoint.txt'
+  ! HALOFIND: hfile names z_halofind.txt ('' leaves the halo finder off);
+  ! halofind.f90:36-40 constants, nq the pid lattice size (0: nf)
+  character(256) :: hfile = ''
+  real :: den_peak_cutoff = 100
+  integer :: min_halo_particles = 100
+  integer(8) :: nq = 0
   namelist /cube/ nc, nnt, ncb, izipx, izipv, istep_max, Extended_pp_force, pp_range, rsoft, &
-                  omega_m, omega_l, wde, box, h0, opath, kernel_dir, zfile, device
+                  omega_m, omega_l, wde, box, h0, opath, kernel_dir, zfile, device, &
+                  hfile, den_peak_cutoff, min_halo_particles, nq
 
   type(cube_params) :: p
   type(cube_sim_header) :: sim
@@ -35,11 +43,11 @@ program cube_main
   integer(c_int8_t), allocatable, target :: xpb(:), vpb(:)
   integer(c_int32_t), allocatable :: pid(:), rhoc(:)
   real(c_float), allocatable :: vfield(:)
-  real :: z_checkpoint(1000)
-  integer :: n_checkpoint, cur_checkpoint, i, u
+  real :: z_checkpoint(1000), z_halofind(1000)
+  integer :: n_checkpoint, cur_checkpoint, n_halofind, cur_halofind, i, u
   integer(8) :: istep, np_max
   real :: a, a_mid, da, dt, dt_old, dt_neu, t, tau
-  logical :: checkpoint_step, final_step
+  logical :: checkpoint_step, final_step, halofind_step, halofind_on
   character(256) :: nml
 
   nml = 'cube.nml'
@@ -57,6 +65,19 @@ program cube_main
   enddo
 71 close(u)
   if (n_checkpoint == 0) error stop 'z_checkpoint.txt empty'
+  ! initialize.f90:62-71; entries past the list stay 0 as in the reference
+  z_halofind = 0
+  n_halofind = 0
+  halofind_on = len_trim(hfile) > 0
+  if (halofind_on) then
+    open(newunit=u, file=trim(hfile), status='old', action='read')
+    do i = 1, 999
+      read(u, end=81, fmt='(f8.4)') z_halofind(i)
+      n_halofind = i
+    enddo
+81  close(u)
+    if (n_halofind == 0) error stop 'z_halofind.txt empty'
+  endif
 
   ! kernel tables (kernel_f.f90:14-23, kernel_c.f90:31-39)
   allocate(kf(4096 * 3), kc(64 * 3))
@@ -95,7 +116,8 @@ program cube_main
   t = 0; dt = 0; dt_old = 0; da = 0; dt_neu = dt_max
   tau = -3 / sqrt(1. / (1 + z_checkpoint(1)))
   cur_checkpoint = 2
-  checkpoint_step = .false.; final_step = .false.
+  cur_halofind = cur_checkpoint + 1   ! main.f90:43
+  checkpoint_step = .false.; final_step = .false.; halofind_step = .false.
   print*, 'cube_main: nplocal =', sim%nplocal, ' z_i =', z_checkpoint(1)
 
   do istep = 1, istep_max
@@ -111,12 +133,18 @@ program cube_main
     call cube_ok(h, rc, 'cube_pm')
     sim%dt_fine = pmo%dt_fine; sim%dt_coarse = pmo%dt_coarse; sim%dt_vmax = pmo%dt_vmax
     print '(a,i5,a,f9.5,a,es12.5,a,es12.5)', ' step', istep, '  a=', a, '  dt=', dt, '  sigma_vi=', dro%sigma_vi_new
-    if (checkpoint_step) then
+    if (checkpoint_step .or. halofind_step) then   ! main.f90:72-95
       dt_old = 0
       rc = cube_drift(h, dt_old, dt, dro)
       call cube_ok(h, rc, 'cube_drift')
-      call checkpoint(z_checkpoint(cur_checkpoint))
-      cur_checkpoint = cur_checkpoint + 1
+      if (checkpoint_step) then
+        call checkpoint(z_checkpoint(cur_checkpoint))
+        cur_checkpoint = cur_checkpoint + 1
+      endif
+      if (halofind_step) then
+        call halofind(z_halofind(cur_halofind))
+        cur_halofind = cur_halofind + 1
+      endif
       if (final_step) exit
       dt = 0
     endif
@@ -150,11 +178,16 @@ contains
     irc = cube_expansion(a, dt, omega_m, omega_l, wde, da1, da2)
     da = da1 + da2
     checkpoint_step = .false.
+    halofind_step = .false.
     z_next = z_checkpoint(cur_checkpoint)
+    if (halofind_on) z_next = max(z_checkpoint(cur_checkpoint), z_halofind(cur_halofind))
     a_next = 1.0 / (1 + z_next)
     if (da >= a_next - a) then
-      checkpoint_step = .true.
-      if (cur_checkpoint == n_checkpoint) final_step = .true.
+      if (z_next == z_checkpoint(cur_checkpoint)) then
+        checkpoint_step = .true.
+        if (cur_checkpoint == n_checkpoint) final_step = .true.
+      endif
+      if (halofind_on .and. z_next == z_halofind(cur_halofind)) halofind_step = .true.
       do while (abs((a + da) / a_next - 1) >= 1e-6)
         dt = dt * (a_next - a) / da
         irc = cube_expansion(a, dt, omega_m, omega_l, wde, da1, da2)
@@ -188,6 +221,49 @@ contains
     open(newunit=v, file=output_name(z, 'id'), status='replace', access='stream')
     write(v) pid(1:sim%nplocal); close(v)
     print*, 'checkpoint written at z =', z, ' nplocal =', sim%nplocal
+  end subroutine
+
+  !! halofind.f90 through cube_halofind; writes the halo, halo_pid and hid
+  !! files of halofind.f90:117-125,568-578
+  subroutine halofind(z)
+    real, intent(in) :: z
+    type(cube_halo_params) :: hp
+    type(cube_halo_catalog) :: cat
+    real(c_float), allocatable, target :: halos(:, :)
+    integer(c_int32_t), allocatable, target :: words(:)
+    integer(c_int16_t), allocatable, target :: hid(:)
+    integer(8) :: maxh, maxw, nf
+    integer :: v, irc
+    nf = 4 * nc
+    maxh = 1048576
+    maxw = 2 * maxh + sim%nplocal + 16
+    allocate(halos(42, maxh), words(maxw), hid(nf**3))
+    hp%den_odc = 200; hp%den_peak_cutoff = den_peak_cutoff; hp%min_halo_particles = min_halo_particles
+    hp%nc_halo_max = 128; hp%newbox_max = 300; hp%search_ratio = 2; hp%refine_ratio = 3; hp%pad = 0
+    hp%max_halo_np = 5 * 129**3 / 3
+    hp%nq = merge(nq, nf, nq > 0)
+    hp%h0 = h0; hp%box = box; hp%los_centre = [30., 370., 370.]
+    hp%a = a
+    hp%vsim2phys = (1.5 / a) * box * 100. * sqrt(omega_m) / nf   ! checkpoint.f90:30
+    hp%pad2 = 0
+    cat%max_halos = maxh; cat%halos = c_loc(halos)
+    cat%max_pid_words = maxw; cat%pid_words = c_loc(words)
+    cat%nhid = nf**3; cat%hid = c_loc(hid)
+    irc = cube_halofind(h, hp, cat)
+    call cube_ok(h, irc, 'cube_halofind')
+    call execute_command_line('mkdir -p '//trim(opath)//'image1')
+    open(newunit=v, file=output_name(z, 'halo'), status='replace', access='stream')
+    write(v) int(cat%nhalo, c_int32_t), int(cat%nhalo, c_int32_t), hp%den_odc
+    write(v) halos(:, 1:cat%nhalo)
+    close(v)
+    open(newunit=v, file=output_name(z, 'halo_pid'), status='replace', access='stream')
+    write(v) words(1:cat%npid_words)
+    close(v)
+    open(newunit=v, file=output_name(z, 'hid'), status='replace', access='stream')
+    write(v) hid
+    close(v)
+    print*, 'halofind at z =', z, ': nhalo =', cat%nhalo
+    deallocate(halos, words, hid)
   end subroutine
 
   !! basic_functions.fh:252-303

@@ -32,7 +32,8 @@ HALO_RECORD_FLOATS = 42
 @dataclass
 class HaloParams:
     """halofind.f90:27-40 (den_odc = 200: odc_vir is .false.) and the module
-    parameters the finder reads (parameters.f90:86 h0, the box size)."""
+    parameters the finder reads (h0 and box of parameters.f90; h0 = 0 takes
+    the box's own)."""
     den_odc: float = 200.0
     den_peak_cutoff: float = 100.0
     min_halo_particles: int = 100
@@ -42,7 +43,7 @@ class HaloParams:
     search_ratio: int = 2
     refine_ratio: int = 3
     nq: int = 0                  # pid lattice size per dimension (pids are q-lattice indices + 1)
-    h0: float = 0.67
+    h0: float = 0.0
     box: float = 0.0
     los_centre: tuple = (30.0, 370.0, 370.0)   # halofind.f90:58 'cen'
     a: float = 0.0               # sim%a and sim%vsim2phys of the current state
@@ -57,7 +58,8 @@ class HaloParams:
         a, vs = self.a, self.vsim2phys
         if sim is not None:
             a, vs = a or float(sim.a), vs or float(sim.vsim2phys)
-        return dataclasses.replace(self, box=self.box or cfg.box, nq=nq or self.nq or cfg.nf, a=a, vsim2phys=vs)
+        return dataclasses.replace(self, box=self.box or cfg.box, h0=self.h0 or cfg.h0, nq=nq or self.nq or cfg.nf,
+                                   a=a, vsim2phys=vs)
 
 
 class cube_halo_params(ctypes.Structure):
