@@ -222,12 +222,21 @@ def ensemble_bench(n, rate):
     log("one at a time: %.3g particle-steps/s over %.2f s" % (plain["particle_steps_per_s"], plain["span"]))
     key = lambda r: (r["npd"], r["seed"])
     same = {key(r): r["hash"] for r in mix["results"]} == {key(r): r["hash"] for r in plain["results"]}
+    # one-at-a-time run time per box size: the share of the largest boxes
+    # bounds what overlapping can gain when one such box already fills the GPU
+    by_size = {}
+    for r in plain["results"]:
+        by_size[r["npd"]] = by_size.get(r["npd"], 0.0) + (r["latency"] - r["wait"])
+    big = max(by_size)
+    log("one at a time, seconds by size: %s" % {k: round(v, 2) for k, v in sorted(by_size.items())})
     return {"workload": "C5: %d seeded LCDM PM+PP boxes, {32,64,128,256}^3 particles, z=49 to 0, Poisson arrivals "
                         "%.1f/s" % (n, rate),
             "p50_s": round(mix["p50"], 4), "p99_s": round(mix["p99"], 4),
             "particle_steps_per_s": mix["particle_steps_per_s"],
             "plain_particle_steps_per_s": plain["particle_steps_per_s"],
             "speedup_vs_one_at_a_time": mix["particle_steps_per_s"] / plain["particle_steps_per_s"],
+            "plain_seconds_by_size": {str(k): round(v, 3) for k, v in sorted(by_size.items())},
+            "plain_time_over_largest_boxes_time": plain["span"] / by_size[big],
             "steps": sorted(set(r["steps"] for r in mix["results"])),
             "same_result_alone_and_in_mix": bool(same)}
 

@@ -239,9 +239,9 @@ def compare(cat, halos, words, hid, nf):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nnt", [1, 2])
+@pytest.mark.parametrize("nnt", [1, 2, 4])
 def test_gpu_halofind_matches_restatement(nnt):
-    cfg = BoxConfig(nc=16, nnt=nnt, ncb=6, box=100.0)
+    cfg = BoxConfig(nc=16 if nnt < 4 else 32, nnt=nnt, ncb=6, box=100.0)
     rng = np.random.default_rng(30 + nnt)
     specs = []
     for k in range(14):
