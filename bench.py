@@ -252,6 +252,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
     ap.add_argument("--no-halofind", action="store_true", help="skip the halo-finder measurement")
+    ap.add_argument("--halofind-only", action="store_true",
+                    help="only generate the state and run the halo finder on it once (for profiling)")
     ap.add_argument("--ens-requests", type=int, default=16)
     ap.add_argument("--ens-rate", type=float, default=6.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--copy-calib", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
@@ -280,6 +282,10 @@ def main():
     log("state %s: %d particles, Zel'dovich at z=%.2f generated in HBM (%.1f s incl. init)"
         % (args.config, npart, z0, time.perf_counter() - t0))
     lib = cube.lib
+    if args.halofind_only:
+        print(json.dumps({"halofind": halofind_c3(cube, args.config)}), flush=True)
+        cube.close()
+        return
     for i in range(args.warmup):
         cube.step()
         log("warmup step %d done" % (i + 1))
