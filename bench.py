@@ -207,6 +207,104 @@ def halofind_evolved(config, seed, cpu):
     return out
 
 
+def nfw_clumps_state(cfg, seed, frac_halo=0.5):
+    """Synthetic heavy-tailed PP occupancy: a fraction frac_halo of the box's
+    particles in NFW halos (concentration 4..12, virial radii from the
+    halo's mass at 200 times the mean density, halo masses from
+    dN/dM ~ M^-2 between 50 and 2e5 particles), the rest uniform."""
+    from cube_nufluid_amd.ic import encode_particles
+    rng = np.random.default_rng(seed)
+    n = cfg.np_total if hasattr(cfg, "np_total") else (cfg.nc * cfg.np_nc) ** 3
+    nh_target = int(frac_halo * n)
+    mmin, mmax = 50.0, 2e5
+    sizes = []
+    tot = 0
+    while tot < nh_target:
+        u = rng.uniform(size=4096)
+        m = mmin / (1 - u * (1 - mmin / mmax))      # dN/dM ~ M^-2 on [mmin, mmax]
+        m = m.astype(np.int64)
+        for x in m:
+            sizes.append(int(x))
+            tot += int(x)
+            if tot >= nh_target:
+                break
+    sizes = np.array(sizes)
+    nbar = n / float(cfg.nc) ** 3                       # particles per coarse cell
+    rvir = (3 * sizes / (4 * np.pi * 200 * nbar)) ** (1 / 3)   # coarse-cell units
+    cgrid = np.linspace(4, 12, 17)
+    conc = rng.integers(0, len(cgrid), size=len(sizes))
+    centres = rng.uniform(0, cfg.nc, size=(len(sizes), 3))
+    # inverse CDF of the NFW enclosed mass ln(1+x) - x/(1+x), x = r/r_s,
+    # tabulated per concentration
+    xs = np.linspace(0, 1, 4097)
+    pos = []
+    for ci, cc in enumerate(cgrid):
+        sel = np.nonzero(conc == ci)[0]
+        if len(sel) == 0:
+            continue
+        m = sizes[sel]
+        owner = np.repeat(np.arange(len(sel)), m)
+        xg = xs * cc
+        mu = np.log1p(xg) - xg / (1 + xg)
+        r = np.interp(rng.uniform(0, mu[-1], size=len(owner)), mu, xg) * (rvir[sel][owner] / cc)
+        d = rng.normal(size=(len(owner), 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        pos.append(centres[sel][owner] + r[:, None] * d)
+    pos.append(rng.uniform(0, cfg.nc, size=(n - int(sizes.sum()), 3)))
+    pos = np.concatenate(pos)[:n]
+    vel = rng.normal(0, 1.0, size=pos.shape)
+    st = encode_particles(cfg, pos, vel, np.arange(1, len(pos) + 1, dtype=np.int32), np.float32(1.0))
+    return st, len(sizes), int(sizes.max())
+
+
+def pp_unit_bench(config, seed, cpu, reps=3):
+    """PP alone on a seeded NFW-clump occupancy of the config's size: GPU
+    pairs/s and TFLOP/s (30 flop per pair) over `reps` passes; with cpu, the
+    oracle's PP on the first tile of the same state with all host threads."""
+    from cube_nufluid_amd.params import CONFIGS
+    from cube_nufluid_amd.driver import Cube
+    import torch
+    cfg = CONFIGS[config]
+    t0 = time.perf_counter()
+    st, nhalo, mmax = nfw_clumps_state(cfg, seed)
+    log("pp unit: %s NFW-clump state, %d particles, %d halos (largest %d) built in %.1f s"
+        % (config, st.sim.nplocal, nhalo, mmax, time.perf_counter() - t0))
+    cube = Cube(cfg, st)
+    try:
+        cube.a_mid, cube.dt = np.float32(0.9), np.float32(0.01)
+        cube.ext_pp_force()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            out = cube.ext_pp_force()
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t1) / reps
+    finally:
+        cube.close()
+    pairs = float(out.npairs)
+    res = {"workload": "%s size (%d particles), %d seeded NFW halos holding half the particles, dN/dM ~ M^-2, "
+                       "largest %d particles" % (config, st.sim.nplocal, nhalo, mmax),
+           "ms_per_pass": round(sec * 1e3, 3), "pairs": pairs, "pairs_per_s": pairs / sec,
+           "achieved_tflops": round(30 * pairs / sec / 1e12, 3), "peak_tflops": FP32_PEAK_TFLOPS,
+           "dense_blocks": int(out.dense_blocks)}
+    log("pp unit: %.2f ms per pass, %.3g pairs/s, %d dense blocks" % (sec * 1e3, pairs / sec, out.dense_blocks))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import Oracle
+        threads = min(16, os.cpu_count() or 1)
+        orc = Oracle(cfg, threads=threads)
+        orc.set_state(st)
+        orc.lib.orc_set_tile_limit(orc.o, 1)
+        t2 = time.perf_counter()
+        r = orc.pp(0.9, 0.01)
+        cs = time.perf_counter() - t2
+        orc.close()
+        res["cpu"] = {"pairs_per_s": r["npairs"] / cs, "cores": threads, "kind": "port",
+                      "sample": "oracle PP on the first of %d tiles (%d pairs, %.1f s)" % (cfg.nnt ** 3, r["npairs"], cs)}
+        log("pp unit cpu: %.3g pairs/s on one tile" % (r["npairs"] / cs))
+    return res
+
+
 def ensemble_bench(n, rate):
     """Ensemble serving: n seeded LCDM PM+PP boxes drawn uniformly from
     {32,64,128,256}^3 particles (mesh 2x particles per dimension), Zel'dovich
@@ -252,6 +350,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
     ap.add_argument("--no-halofind", action="store_true", help="skip the halo-finder measurement")
+    ap.add_argument("--no-pp-unit", action="store_true", help="skip the PP unit benchmark on NFW clumps")
     ap.add_argument("--halofind-only", action="store_true",
                     help="only generate the state and run the halo finder on it once (for profiling)")
     ap.add_argument("--ens-requests", type=int, default=16)
@@ -360,6 +459,8 @@ def main():
     cube.close()
     if not args.no_halofind:
         out["halofind"]["evolved"] = halofind_evolved("C2", 1002, cpu=not args.no_cpu_baseline)
+    if not args.no_pp_unit:
+        out["pp_unit"] = pp_unit_bench("C2", 1002, cpu=not args.no_cpu_baseline)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ens_requests, args.ens_rate)
     if st_cpu is not None:
