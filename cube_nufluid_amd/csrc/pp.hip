@@ -374,19 +374,64 @@ __device__ __forceinline__ double pf_pair_sel(double dx, double dy, double dz, d
   return pf_pair(dx, dy, dz, mass, rsoft2, sc) ? sc : 0.0;
 }
 
+// run-start tables: int32 entries (dense pool), or two 16-bit entries per
+// word in LDS (the binned kernel: values never exceed its candidate cap)
+struct RunI32 {
+  const int32_t* p;
+  __device__ __forceinline__ int get(int i) const { return p[i]; }
+};
+struct RunU16 {
+  uint32_t* w;
+  __device__ __forceinline__ int get(int i) const { return (int)((w[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu); }
+  // add one to entry i, returning its previous value
+  __device__ __forceinline__ int inc(int i) const {
+    return (int)((atomicAdd(&w[i >> 1], 1u << ((i & 1) * 16)) >> ((i & 1) * 16)) & 0xFFFFu);
+  }
+};
+__host__ __device__ constexpr int pf_wc16_bytes(int W3) { return ((W3 + 3) / 2 * 4 + 15) / 16 * 16; }
+
+// exclusive scan of the 16-bit counts at entries 1..n of t in place, entry
+// n+1 = total; each thread owns whole words, so no two threads write one
+template <int NT>
+__device__ void pf_scan16(RunU16 t, int n, int32_t* wsum) {
+  const int nw = (n + 3) / 2;  // words covering entries 0..n+1
+  const int per = (nw + NT - 1) / NT;
+  const int w0 = min(nw, (int)threadIdx.x * per), w1 = min(nw, w0 + per);
+  int32_t s = 0;
+  for (int w = w0; w < w1; w++)
+    for (int h = 0; h < 2; h++) {
+      const int e = 2 * w + h;
+      if (e >= 1 && e <= n) s += t.get(e);
+    }
+  int32_t run = block_incl_scan<NT>(s, wsum) - s;
+  for (int w = w0; w < w1; w++) {
+    uint32_t word = t.w[w];
+    for (int h = 0; h < 2; h++) {
+      const int e = 2 * w + h;
+      if (e >= 1 && e <= n + 1) {
+        const uint32_t c = (word >> (16 * h)) & 0xFFFFu;
+        word = (word & ~(0xFFFFu << (16 * h))) | ((uint32_t)run << (16 * h));
+        if (e <= n) run += (int32_t)c;
+      }
+    }
+    t.w[w] = word;
+  }
+  __syncthreads();
+}
+
 // force on one target from the candidate runs (start[] exclusive offsets);
 // candidate coordinates are exact in FP32, so RT only sets the pair
 // arithmetic.  ROWSUM: candidates are in a fixed order, so each x-run is
 // summed in RT and added to the accumulator once per run.
-template <typename RT, bool ROWSUM, typename Acc, typename EntT>
-__device__ __forceinline__ void pf_force(const EntT* ent, const int32_t* start, int W, int pr, const PPent& ti, float mass_f,
+template <typename RT, bool ROWSUM, typename Acc, typename EntT, typename RunT>
+__device__ __forceinline__ void pf_force(const EntT* ent, const RunT start, int W, int pr, const PPent& ti, float mass_f,
                                          float rsoft_f, Acc& acc, unsigned long long& npl) {
   const int wx = (int)floorf(ti.x) + pr, wy = (int)floorf(ti.y) + pr, wz = (int)floorf(ti.z) + pr;
   const RT mass = (RT)mass_f, rsoft2 = (RT)rsoft_f * (RT)rsoft_f, tx = (RT)ti.x, ty = (RT)ti.y, tz = (RT)ti.z;
   for (int kz = wz - pr; kz <= wz + pr; kz++)
     for (int ky = wy - pr; ky <= wy + pr; ky++) {
       const int row = (kz * W + ky) * W;
-      const int j0 = start[row + wx - pr], j1 = start[row + wx + pr + 1];
+      const int j0 = start.get(row + wx - pr), j1 = start.get(row + wx + pr + 1);
       npl += (unsigned long long)(j1 - j0);
       RT sx = 0, sy = 0, sz = 0;
       for (int j = j0; j < j1; j++) {
@@ -465,14 +510,14 @@ __device__ void pf_reduce(float f2loc, unsigned long long npl, int32_t ninner, f
 template <int PB>
 struct PFcfg;
 template <>
-struct PFcfg<4> { static constexpr int NT = 512, K = 4, CAP = 2560; };
+struct PFcfg<4> { static constexpr int NT = 512, K = 4, CAP = 2048; };
 template <>
 struct PFcfg<2> { static constexpr int NT = 128, K = 4, CAP = 1024; };
 
 __host__ __device__ constexpr int pf_wc_bytes(int W3) { return ((W3 + 2) * 4 + 15) / 16 * 16; }
 
-// LDS (dynamic): wc[W^3+2] int32 (wc[1+i]: count, then start, then end of
-// run i; wc[0] = 0), then CAP entries
+// LDS (dynamic): the run table, W^3+2 16-bit entries (entry 1+i: count,
+// then start, then end of run i; entry 0 = 0), then CAP entries
 template <int PB, typename RT>
 __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const void* __restrict__ xp, void* vp,
                                                           const int64_t* __restrict__ cstart, uint32_t* f2max_bits,
@@ -492,9 +537,9 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
   int c0x, c0y, c0z;
   pf_origin<PB>(g, blockIdx.x, c0x, c0y, c0z);
   const int pr = c.pr, W = G::FB + 2 * pr, W3 = W * W * W;
-  int32_t* wc = (int32_t*)pf_dyn;
-  PPent* ent = (PPent*)(pf_dyn + pf_wc_bytes(W3));
-  for (int i = threadIdx.x; i <= W3 + 1; i += NT) wc[i] = 0;
+  const RunU16 wc{(uint32_t*)pf_dyn};
+  PPent* ent = (PPent*)(pf_dyn + pf_wc16_bytes(W3));
+  for (int i = threadIdx.x; i < (W3 + 3) / 2; i += NT) wc.w[i] = 0u;
   if (threadIdx.x == 0) ncand_s = 0;
   pf_region<PB, NT>(g, cstart, c0x, c0y, c0z, beg, off, wsum);
   __syncthreads();
@@ -519,14 +564,14 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
 #pragma unroll
   for (int k = 0; k < K; k++)
     if (rk[k]) {
-      atomicAdd(&wc[1 + rw[k]], 1);
+      wc.inc(1 + rw[k]);
       mine++;
     }
   for (int32_t t = threadIdx.x + K * NT; t < ntot; t += NT) {
     int wi;
     PPent e;
     if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) {
-      atomicAdd(&wc[1 + wi], 1);
+      wc.inc(1 + wi);
       mine++;
     }
   }
@@ -537,15 +582,15 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
     if (threadIdx.x == 0) dense[atomicAdd(ndense, 1u)] = (int32_t)blockIdx.x;
     return;
   }
-  pf_scan<NT>(wc + 1, W3, wsum);
-  // scatter; afterwards wc[1+i] is the end of run i, so wc[i] is the start of run i
+  pf_scan16<NT>(wc, W3, wsum);
+  // scatter; afterwards entry 1+i is the end of run i, so entry i is the start of run i
 #pragma unroll
   for (int k = 0; k < K; k++)
-    if (rk[k]) ent[atomicAdd(&wc[1 + rw[k]], 1)] = re[k];
+    if (rk[k]) ent[wc.inc(1 + rw[k])] = re[k];
   for (int32_t t = threadIdx.x + K * NT; t < ntot; t += NT) {
     int wi;
     PPent e;
-    if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) ent[atomicAdd(&wc[1 + wi], 1)] = e;
+    if (pf_cand<PB>(g, xp, beg, off, t, pr, W, wi, e)) ent[wc.inc(1 + wi)] = e;
   }
   __syncthreads();
   // runs in particle-index order, so the pair sums below have a fixed
@@ -562,7 +607,7 @@ __global__ __launch_bounds__(PFcfg<PB>::NT) void k_pp_fast(Geo g, PPctx c, const
       if (m < ncand) {
         me[k] = ent[m];
         const int wi = (((int)floorf(me[k].z) + pr) * W + ((int)floorf(me[k].y) + pr)) * W + (int)floorf(me[k].x) + pr;
-        const int a = wc[wi], b = wc[wi + 1];
+        const int a = wc.get(wi), b = wc.get(wi + 1);
         int rank = 0;
         if (b - a > 1)
           for (int j = a; j < b; j++) rank += ent[j].p < me[k].p ? 1 : 0;
@@ -769,7 +814,7 @@ __global__ __launch_bounds__(PD_NT) void k_pp_dense_pairs(Geo g, PPctx c, void* 
       if (!probe)
         for (int q = 0; q < 3; q++) vc[q] = load_v(vp, 3 * (int64_t)ti.p + q, g.izipv);
       PPacc<RT> acc;
-      pf_force<RT, false>(ent, st, W, pr, ti, c.mass, c.rsoft, acc, npl);
+      pf_force<RT, false>(ent, RunI32{st}, W, pr, ti, c.mass, c.rsoft, acc, npl);
       f2loc = pf_apply(g, c, vp, probe, ti.p, vc, acc);
       ninner = 1;
     }
@@ -784,7 +829,7 @@ static int pp_launch(cube_handle* h, const Geo& g, const PPctx& c, uint32_t* f2,
   using G = PBgeo<PB>;
   const int W = G::FB + 2 * c.pr, W3 = W * W * W;
   const int64_t nbx = h->nc / PB, nb = nbx * nbx * nbx;
-  const size_t lds_fast = pf_wc_bytes(W3) + sizeof(PPent) * PFcfg<PB>::CAP;
+  const size_t lds_fast = pf_wc16_bytes(W3) + sizeof(PPent) * PFcfg<PB>::CAP;
   const size_t lds_tab = sizeof(int32_t) * (W3 + 1);
   const uint64_t pool_cap = h->fine.bytes_R() / sizeof(PPent);
   HIP_TRY(h, hipFuncSetAttribute((const void*)k_pp_fast<PB, RT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast));
