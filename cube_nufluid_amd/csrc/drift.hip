@@ -174,7 +174,7 @@ __device__ __forceinline__ uint64_t arrival_key(int ilayer, int dx, int dy, int 
 __device__ __forceinline__ int64_t key_rank(uint64_t k) { return (int64_t)(k & ((1ull << DR_RBITS) - 1)); }
 __device__ __forceinline__ int64_t key_l(uint64_t k) { return (int64_t)((k >> DR_RBITS) & ((1ull << DR_LBITS) - 1)); }
 
-#define DR_SMALL 32   // cells up to this many arrivals: one thread sorts
+#define DR_SMALL 16   // cells up to this many arrivals: one thread sorts
 #define DR_MED 1024   // up to this many: one wavefront; above: a 1024-thread workgroup
 
 __global__ __launch_bounds__(DR_NT) void k_drift_scatter(Geo g, const int64_t* __restrict__ cstart, int64_t n3c,
