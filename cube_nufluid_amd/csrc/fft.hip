@@ -435,6 +435,9 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
 #define TB 16
 template <typename T>
 constexpr int pick_wy(int logn) { return pick_w<T>(logn) < TB ? pick_w<T>(logn) : TB; }
+// tiled x passes: rows per workgroup, half the LDS tile of the plain x pass
+template <typename T>
+constexpr int pick_wtx(int logn) { return pick_w<T>(logn - 1) > 1 ? pick_w<T>(logn - 1) / 2 : 1; }
 // z convolution strips: twice the y strip width (whole 64-byte halves of
 // each group), with twice the threads when that doubles the LDS tile
 template <typename T>
@@ -450,7 +453,7 @@ __host__ __device__ __forceinline__ int64_t t_index(int64_t N, int64_t kx, int64
 template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_tx_r2c(const T* __restrict__ src, cpx<T>* __restrict__ dst,
                                                    const cpx<T>* __restrict__ tw, int64_t nrows) {
-  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_wtx<T>(LOGN), LD = W + 1;
   constexpr int KT = (int)t_tiles(N);
   __shared__ cpx<T> s[H * LD];
   const int64_t r0 = (int64_t)blockIdx.x * W;
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ sr
                                                    const cpx<T>* __restrict__ tw, int64_t nrows,
                                                    const T* __restrict__ F0, int64_t p0, const T* __restrict__ F1, int64_t p1,
                                                    uint32_t* f2max) {
-  constexpr int N = 1 << LOGN, H = N / 2, W = pick_w<T>(LOGN - 1), LD = W + 1;
+  constexpr int N = 1 << LOGN, H = N / 2, W = pick_wtx<T>(LOGN), LD = W + 1;
   __shared__ cpx<T> s[H * LD];
   __shared__ cpx<T> xnyq[W];
   __shared__ float red[NT_FFT / 64];
@@ -766,7 +769,7 @@ static int t_fwd_t(cube_handle* h, Mesh& m, const void* src, void* dst) {
   int64_t N = m.N, nrows = N * N;
   const cpx<T>* tw = (const cpx<T>*)m.tw;
   LOGN_SWITCH(logn, {
-    constexpr int WX = pick_w<T>(L - 1);
+    constexpr int WX = pick_wtx<T>(L);
     int p1 = prof_begin(h, PS_FFT_XF);
     k_tx_r2c<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>((const T*)src, (cpx<T>*)dst, tw, nrows);
     prof_end(h, p1);
@@ -811,7 +814,7 @@ static int t_inv_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch
     int pa = prof_begin(h, PS_FFT_YI);
     k_ty<T, L, true><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), NT_FFT, 0, h->stream>>>((cpx<T>*)buf, tw);
     prof_end(h, pa);
-    constexpr int WX = pick_w<T>(L - 1);
+    constexpr int WX = pick_wtx<T>(L);
     int pb = prof_begin(h, PS_FFT_XI);
     k_tx_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
         (const cpx<T>*)buf, (T*)out, opitch, tw, nrows, (const T*)F0, p0, (const T*)F1, p1, f2max);
