@@ -377,7 +377,7 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   for (int d = 0; d < 3; d++) {
     void* spec = d == 2 ? h->fine.W : h->fine.X;
     if ((r = mesh_t_zconv(h, h->fine, d, h->fine.W, spec))) return r;
-    if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d], nullptr, 0, nullptr, 0, nullptr))) return r;
+    if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d]))) return r;
   }
   const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
   // the kick also forms max |F|^2 (dt_fine) from the force slabs it stages

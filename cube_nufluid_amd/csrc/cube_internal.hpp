@@ -289,8 +289,7 @@ int mesh_fft3_r2c(cube_handle* h, Mesh& m);                  // full forward on 
 // dst (may alias); y inverse in place then C2R into out (pitch opitch)
 int mesh_t_fwd(cube_handle* h, Mesh& m, const void* src, void* dst);
 int mesh_t_zconv(cube_handle* h, Mesh& m, int d, const void* src, void* dst);
-int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0, const void* F1,
-               int64_t p1, uint32_t* f2max);
+int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch);
 int fine_kernel_build(cube_handle* h, const float* table);
 int coarse_kernel_build(cube_handle* h, const float* table);
 // pm.hip

@@ -497,13 +497,13 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_r2c(const T* __restrict__ src, cp
 // in k_fft_x_c2r
 template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ src, T* dst, int64_t opitch,
-                                                   const cpx<T>* __restrict__ tw, int64_t nrows,
-                                                   const T* __restrict__ F0, int64_t p0, const T* __restrict__ F1, int64_t p1,
-                                                   uint32_t* f2max) {
+                                                   const cpx<T>* __restrict__ tw, int64_t nrows) {
   constexpr int N = 1 << LOGN, H = N / 2, W = pick_wtx<T>(LOGN), LD = W + 1;
-  __shared__ cpx<T> s[H * LD];
-  __shared__ cpx<T> xnyq[W];
-  __shared__ float red[NT_FFT / 64];
+  // s[k*LD + w] for w < W; the padding column keeps row w's Nyquist term at
+  // s[w*LD + W] when W <= H, so the workgroup needs exactly H*LD entries
+  constexpr int XN = W <= H ? 0 : W;
+  __shared__ cpx<T> s[H * LD + XN];
+  auto nyq = [](int w) { return XN ? H * LD + w : w * LD + W; };
   const int64_t r0 = (int64_t)blockIdx.x * W;
   {
     constexpr int TOT = (H / TB + 1) * TB * W, PER = (TOT + NT_FFT - 1) / NT_FFT;
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ sr
     for (int q = 0; q < PER; q++) {
       const int idx = threadIdx.x + q * NT_FFT, kl = idx % TB, w = (idx / TB) % W, kh = idx / (TB * W), k = kh * TB + kl;
       if (idx < TOT && k <= H) {
-        if (k < H) s[k * LD + w] = r[q]; else xnyq[w] = r[q];
+        if (k < H) s[k * LD + w] = r[q]; else s[nyq(w)] = r[q];
       }
     }
   }
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ sr
     };
     cpx<T> a = s[k * LD + w];
     if (k == 0) {
-      s[w] = zof(a, xnyq[w], 0);
+      s[w] = zof(a, s[nyq(w)], 0);
     } else if (k == H / 2) {
       s[k * LD + w] = zof(a, a, k);
     } else {
@@ -546,50 +546,10 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ sr
   }
   __syncthreads();
   lds_fft<T, H, W, true, LD>(s, tw, 2);
-  float mymax = 0.f;
-  if (!f2max) {
-    for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
-      int w = idx / H, m = idx % H;
-      int64_t row = r0 + w;
-      if (row < nrows) *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = s[m * LD + w];
-    }
-  } else {
-    constexpr int PER = H * W / NT_FFT, FG = PER < 8 ? PER : 8;
-    static_assert(PER % FG == 0, "whole groups");
-    for (int q0 = 0; q0 < PER; q0 += FG) {
-      cpx<T> a[FG], b[FG];
-#pragma unroll
-      for (int q = 0; q < FG; q++) {
-        const int idx = threadIdx.x + (q0 + q) * NT_FFT, w = idx / H, m = idx % H;
-        const int64_t row = r0 + w < nrows ? r0 + w : nrows - 1;
-        a[q] = *reinterpret_cast<const cpx<T>*>(F0 + row * p0 + 2 * m);
-        b[q] = *reinterpret_cast<const cpx<T>*>(F1 + row * p1 + 2 * m);
-      }
-#pragma unroll
-      for (int q = 0; q < FG; q++) {
-        const int idx = threadIdx.x + (q0 + q) * NT_FFT, w = idx / H, m = idx % H;
-        const int64_t row = r0 + w;
-        if (row >= nrows) continue;
-        const cpx<T> z = s[m * LD + w];
-        *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = z;
-        float s0 = (float)a[q].x * (float)a[q].x, s1 = (float)a[q].y * (float)a[q].y;
-        s0 = fmaf((float)b[q].x, (float)b[q].x, s0);
-        s1 = fmaf((float)b[q].y, (float)b[q].y, s1);
-        s0 = fmaf((float)z.x, (float)z.x, s0);
-        s1 = fmaf((float)z.y, (float)z.y, s1);
-        mymax = fmaxf(mymax, fmaxf(s0, s1));
-      }
-    }
-  }
-  if (f2max) {
-    for (int off = 32; off > 0; off >>= 1) mymax = fmaxf(mymax, __shfl_xor(mymax, off));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mymax;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float m = red[0];
-      for (int i = 1; i < NT_FFT / 64; i++) m = fmaxf(m, red[i]);
-      atomicMax(f2max, __float_as_uint(m));
-    }
+  for (int idx = threadIdx.x; idx < H * W; idx += NT_FFT) {
+    int w = idx / H, m = idx % H;
+    int64_t row = r0 + w;
+    if (row < nrows) *reinterpret_cast<cpx<T>*>(dst + row * opitch + 2 * m) = s[m * LD + w];
   }
 }
 
@@ -804,8 +764,7 @@ int mesh_t_zconv(cube_handle* h, Mesh& m, int d, const void* src, void* dst) {
 }
 
 template <typename T>
-static int t_inv_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0, const void* F1,
-                   int64_t p1, uint32_t* f2max) {
+static int t_inv_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch) {
   int logn = ilog2(m.N);
   int64_t N = m.N, nrows = N * N;
   const cpx<T>* tw = (const cpx<T>*)m.tw;
@@ -816,19 +775,16 @@ static int t_inv_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch
     prof_end(h, pa);
     constexpr int WX = pick_wtx<T>(L);
     int pb = prof_begin(h, PS_FFT_XI);
-    k_tx_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>(
-        (const cpx<T>*)buf, (T*)out, opitch, tw, nrows, (const T*)F0, p0, (const T*)F1, p1, f2max);
+    k_tx_c2r<T, L><<<dim3((unsigned)((nrows + WX - 1) / WX)), NT_FFT, 0, h->stream>>>((const cpx<T>*)buf, (T*)out, opitch, tw,
+                                                                                     nrows);
     prof_end(h, pb);
   });
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
-int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch, const void* F0, int64_t p0, const void* F1,
-               int64_t p1, uint32_t* f2max) {
+int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch) {
   if (out == buf) return cube_fail(h, CUBE_ERR_ARG, "tiled C2R runs out of place");
-  if (f2max && (!F0 || !F1)) return cube_fail(h, CUBE_ERR_ARG, "force maximum needs both other components");
-  return m.prec == 4 ? t_inv_t<float>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max)
-                     : t_inv_t<double>(h, m, buf, out, opitch, F0, p0, F1, p1, f2max);
+  return m.prec == 4 ? t_inv_t<float>(h, m, buf, out, opitch) : t_inv_t<double>(h, m, buf, out, opitch);
 }
 
 // -------------------------------------------------------------- allocation
