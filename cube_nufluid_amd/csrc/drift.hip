@@ -418,7 +418,7 @@ __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __
 //     (update_particle.f90:70-83).
 //   slots (DC_SL per batch): the velocity residual re-quantised with the old
 //     sigma (:119) and the sigma_v sums (:174-200).
-// part: 3 doubles per workgroup.
+// part: 3 doubles per workgroup, at its first cell's index / DW_CELLS.
 #define DW_CELLS 128
 #define DW_STAGE 1536
 #define DG_SL 4
@@ -576,7 +576,7 @@ __device__ __forceinline__ void drift_write_body(Geo g, const uint64_t* __restri
   if (threadIdx.x < 3) {
     double t = 0;
     for (int i = 0; i < DR_NT / 64; i++) t += red[threadIdx.x][i];
-    part[(int64_t)blockIdx.x * 3 + threadIdx.x] = t;
+    part[r0 / DW_CELLS * 3 + threadIdx.x] = t;
   }
 }
 
@@ -591,7 +591,9 @@ __global__ __launch_bounds__(DR_NT) void k_drift_write(Geo g, const uint64_t* __
   __shared__ float svf[DW_CELLS * 3];
   __shared__ double red[3][DR_NT / 64];
   __shared__ double svr[3 * DW_STAGE];
-  const int64_t r0 = (int64_t)blockIdx.x * DW_CELLS;
+  // XCD-contiguous block order: neighbouring cell rows gather from the same
+  // source cells, so they run on the same XCD (the same L2)
+  const int64_t r0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * DW_CELLS;
   const int ncell = (int)min((int64_t)DW_CELLS, n3c - r0);
   for (int i = threadIdx.x; i <= DW_CELLS; i += DR_NT) scs[i] = cs_new[r0 + min(i, ncell)];
   __syncthreads();
