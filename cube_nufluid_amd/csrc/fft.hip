@@ -587,13 +587,16 @@ __global__ __launch_bounds__(pick_ntz<T>(LOGN)) void k_tzconv(const cpx<T>* src,
   const int64_t base = ((int64_t)kh * N * N) * TB + (int64_t)ky * TB + part * W;
   const int kyp = ky <= N / 2 ? ky : N - ky;
   const T sy = (d == 1 && ky > N / 2) ? (T)-1 : (T)1;
+  // K in strips of W kx: K[d][kyp][kx / W][kzp][kx % W], so the strip's
+  // kernel values are one contiguous run of H*W
+  constexpr int G = (int)(t_tiles(N) * TB / W);
+  const T* Ks = K + (((int64_t)d * H + kyp) * G + kx0 / W) * H * W;
   T kv[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int idx = threadIdx.x + q * NTH, w = idx % W, n = idx / W;
-    const int kx = kx0 + w < KX ? kx0 + w : KX - 1;
     const int kzp = n <= N / 2 ? n : N - n;
-    kv[q] = K[(((int64_t)d * H + kyp) * KX + kx) * H + kzp];
+    kv[q] = Ks[kzp * W + w];
   }
   col_load<T, N, W, NTH>(s, src + base, stride, W);
   lds_fft<T, N, W, false, W, NTH>(s, tw, 1);
@@ -798,7 +801,8 @@ int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S, bool w
     HIP_TRY(h, hipMalloc(&m.S, (size_t)prec * N * N * N));
     HIP_TRY(h, hipMalloc(&m.X, m.bytes_big()));
   }
-  if (with_K) HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * H * H * prec));
+  // kx padded to whole 16-wide tiles: the fine mesh's strip layout needs it
+  if (with_K) HIP_TRY(h, hipMalloc(&m.K, (size_t)3 * H * t_tiles(N) * TB * H * prec));
   HIP_TRY(h, hipMalloc(&m.tw, (size_t)N * 2 * prec));
   std::vector<double> tw(2 * N);
   for (int64_t j = 0; j < N; j++) {
@@ -866,13 +870,17 @@ __global__ void k_kf_stage2(const double* A, const double* cs, const double* sn,
   B[i] = s;
 }
 template <typename T>
-__global__ void k_kf_stage3(const double* B, const double* cs, const double* sn, int64_t N, int64_t H, double scale, T* K) {
-  // K[d][ky][kx][kz] = -scale * sum_z B[d][kx][ky][z] g_d2(kz,z)
+__global__ void k_kf_stage3(const double* B, const double* cs, const double* sn, int64_t N, int64_t H, int W, double scale,
+                            T* K) {
+  // K[d][ky][kx / W][kz][kx % W] = -scale * sum_z B[d][kx][ky][z] g_d2(kz,z)
+  // over kx padded to whole 16-wide tiles (the padding repeats kx = H-1)
+  const int64_t G = t_tiles(N) * TB / W;
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int64_t tot = 3 * H * H * H;
+  int64_t tot = 3 * H * G * W * H;
   if (i >= tot) return;
-  int64_t kz = i % H, kx = (i / H) % H, ky = (i / (H * H)) % H;
-  int d = (int)(i / (H * H * H));
+  const int64_t w = i % W, kz = (i / W) % H, g = (i / (W * H)) % G, ky = (i / (W * H * G)) % H;
+  const int64_t kx = g * W + w < H ? g * W + w : H - 1;
+  int d = (int)(i / (W * H * G * H));
   const double* b = B + ((d * H + kx) * H + ky) * 16;
   double s = 0;
   for (int z = 0; z < 16; z++) s += b[z] * g_fac(cs, sn, N, 2, d, (int)kz, z);
@@ -896,14 +904,16 @@ int fine_kernel_build(cube_handle* h, const float* table) {
   HIP_TRY(h, hipMemcpy(dtab, table, sizeof(float) * 4096 * 3, hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(dcs, cs.data(), sizeof(double) * N, hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(dsn, sn.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-  int64_t t1 = 3 * H * 256, t2 = 3 * H * H * 16, t3 = 3 * H * H * H;
+  // the strip width of the tiled z convolution (k_tzconv) for this mesh
+  const int W = m.prec == 4 ? pick_wz<float>(ilog2(N)) : pick_wz<double>(ilog2(N));
+  int64_t t1 = 3 * H * 256, t2 = 3 * H * H * 16, t3 = 3 * H * t_tiles(N) * TB * H;
   k_kf_stage1<<<(unsigned)((t1 + 255) / 256), 256, 0, h->stream>>>(dtab, h->p.pp_on, h->p.pp_range, dcs, dsn, N, H, A);
   k_kf_stage2<<<(unsigned)((t2 + 255) / 256), 256, 0, h->stream>>>(A, dcs, dsn, N, H, B);
   double scale = 1.0 / ((double)N * N * N);
   if (m.prec == 4)
-    k_kf_stage3<float><<<(unsigned)((t3 + 255) / 256), 256, 0, h->stream>>>(B, dcs, dsn, N, H, scale, (float*)m.K);
+    k_kf_stage3<float><<<(unsigned)((t3 + 255) / 256), 256, 0, h->stream>>>(B, dcs, dsn, N, H, W, scale, (float*)m.K);
   else
-    k_kf_stage3<double><<<(unsigned)((t3 + 255) / 256), 256, 0, h->stream>>>(B, dcs, dsn, N, H, scale, (double*)m.K);
+    k_kf_stage3<double><<<(unsigned)((t3 + 255) / 256), 256, 0, h->stream>>>(B, dcs, dsn, N, H, W, scale, (double*)m.K);
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   (void)hipFree(dtab); (void)hipFree(dcs); (void)hipFree(dsn); (void)hipFree(A); (void)hipFree(B);

@@ -113,7 +113,7 @@ extern "C" int64_t cube_footprint(const cube_params* p, int32_t ic_np_nc) {
   auto mesh = [&](int64_t N, bool with_S) {
     const int64_t H = N / 2 + 1;
     const int64_t big = std::max(N * N * (N + 2) * prec, (H + 15) / 16 * 16 * N * N * 2 * prec);  // Mesh::bytes_big
-    return 2 * big + (with_S ? N * N * N * prec + big : 0) + 3 * H * H * H * prec + 2 * N * prec;
+    return 2 * big + (with_S ? N * N * N * prec + big : 0) + 3 * H * ((H + 15) / 16 * 16) * H * prec + 2 * N * prec;
   };
   const int64_t nb2 = (nc / 2) * (nc / 2) * (nc / 2);
   int64_t b = 0;
