@@ -591,6 +591,11 @@ __global__ __launch_bounds__(pick_ntz<T>(LOGN)) void k_tzconv(const cpx<T>* src,
   // kernel values are one contiguous run of H*W
   constexpr int G = (int)(t_tiles(N) * TB / W);
   const T* Ks = K + (((int64_t)d * H + kyp) * G + kx0 / W) * H * W;
+  col_load<T, N, W, NTH>(s, src + base, stride, W);
+  lds_fft<T, N, W, false, W, NTH>(s, tw, 1);
+  // the kernel values are requested only now: held across the forward FFT
+  // they push the kernel past 256 VGPRs into scratch spills, which
+  // cost more than this exposed load
   T kv[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
@@ -598,8 +603,6 @@ __global__ __launch_bounds__(pick_ntz<T>(LOGN)) void k_tzconv(const cpx<T>* src,
     const int kzp = n <= N / 2 ? n : N - n;
     kv[q] = Ks[kzp * W + w];
   }
-  col_load<T, N, W, NTH>(s, src + base, stride, W);
-  lds_fft<T, N, W, false, W, NTH>(s, tw, 1);
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int idx = threadIdx.x + q * NTH, n = idx / W;
