@@ -422,7 +422,7 @@ __global__ __launch_bounds__(DR_SORT_NT) void k_drift_sort_big(const int64_t* __
 #define DW_CELLS 128
 #define DW_STAGE 1536
 #define DG_SL 4
-#define DC_SL 4
+#define DC_SL 2
 template <bool STAGED>
 __device__ __forceinline__ void drift_write_body(Geo g, const uint64_t* __restrict__ keys, int64_t r0, int ncell,
                                                  const int64_t* __restrict__ cstart, const void* __restrict__ xp,
