@@ -493,8 +493,7 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_r2c(const T* __restrict__ src, cp
   }
 }
 
-// tiled spectrum src -> real rows dst (pitch opitch); optional |F|^2 max as
-// in k_fft_x_c2r
+// tiled spectrum src -> real rows dst (pitch opitch)
 template <typename T, int LOGN>
 __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ src, T* dst, int64_t opitch,
                                                    const cpx<T>* __restrict__ tw, int64_t nrows) {
