@@ -178,7 +178,7 @@ def check(lib, h, rc):
 
 
 def expansion(a, dt, omega_m, omega_l, wde=-1.0):
-    """timestep.f90:270-314 through the library (float interface)."""
+    """timestep.f90:136-180 through the library (float interface)."""
     lib = load()
     d1, d2 = c_f32(), c_f32()
     check(lib, None, lib.cube_expansion(a, dt, omega_m, omega_l, wde, ctypes.byref(d1), ctypes.byref(d2)))

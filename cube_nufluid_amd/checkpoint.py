@@ -1,8 +1,8 @@
 """CUBE checkpoint files, byte-compatible with main/checkpoint.f90:12-55 and
-main/particle_initialization.f90:121-157.
+main/particle_initialization.f90:34-82.
 
 Files are stream binaries named '{z:f7.3}_{content}_{image}.bin'
-(basic_functions.fh:252-303) inside '{opath}/image{image}/':
+(basic_functions.fh:52-87) inside '{opath}/image{image}/':
   info : 224-byte sim_header
   xp   : 3*nplocal components of izipx bytes
   vp   : 3*nplocal components of izipv bytes
@@ -21,7 +21,7 @@ CONTENTS = ("info", "xp", "vp", "np", "vc", "id")
 
 
 def z2str(z):
-    """basic_functions.fh:252-258: write(str,'(f7.3)') z, then trim."""
+    """basic_functions.fh:36-42: write(str,'(f7.3)') z, then trim."""
     return ("%7.3f" % np.float32(z)).strip()
 
 
@@ -62,7 +62,7 @@ def write_checkpoint(state, opath, z, image=1):
 
 
 def read_checkpoint(opath, z, nc, image=1, izipx=None, izipv=None):
-    """particle_initialization.f90:133-157; a header whose izipx/izipv differ
+    """particle_initialization.f90:34-82; a header whose izipx/izipv differ
     from the requested ones is rejected as in :140-144."""
     with open(output_name(opath, z, "info", image), "rb") as f:
         sim = SimHeader.from_bytes(f.read())

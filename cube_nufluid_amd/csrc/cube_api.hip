@@ -423,20 +423,31 @@ extern "C" int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out) 
   return CUBE_OK;
 }
 
+// Each operation is evaluated in the precision the reference's declarations
+// give it: default-real literals and parameters (4.0/9.0, 1-omega_m-omega_l,
+// the wde factors), integer powers of the real(4) dt_x and the sum a0+da1
+// are float operations widened afterwards; the rest is double, left to
+// right; x**3 of a real(8) is (x*x)*x, x**(real) is pow().
 extern "C" int cube_expansion(float a0, float dt0, float omega_m, float omega_l, float wde, float* da1, float* da2) {
   if (!da1 || !da2) return CUBE_ERR_ARG;
-  float dt_x = dt0 / 2;
-  double a_x = a0, omHsq = 4.0 / 9.0;
+  const float dt_x = dt0 / 2;
+  const double omHsq = (double)(4.0f / 9.0f);
+  const double dt2 = (double)(dt_x * dt_x), dt3 = (double)((dt_x * dt_x) * dt_x);
+  const double ekm = (double)((1.0f - omega_m) - omega_l);
+  const double w1 = (double)(1.5f * (1.0f - wde));
+  const double w2 = (double)((1.5f * (2.0f - 3.0f * wde)) * (1.0f - wde));
+  const double ew = (double)(-(3.0f * wde));
+  double a_x = a0;
   for (int half = 0; half < 2; half++) {
-    double a3rlm = pow(a_x, -3 * (double)wde) * (double)omega_l / (double)omega_m;
-    double arkm = a_x * (1.0 - (double)omega_m - (double)omega_l) / (double)omega_m;
-    double adot = sqrt(omHsq * a_x * a_x * a_x * (1.0 + arkm + a3rlm));
-    double addot = a_x * a_x * omHsq * (1.5 + 2.0 * arkm + 1.5 * (1.0 - (double)wde) * a3rlm);
-    double atdot = a_x * adot * omHsq * (3.0 + 6.0 * arkm + 1.5 * (2.0 - 3.0 * (double)wde) * (1.0 - (double)wde) * a3rlm);
-    float da = (float)(adot * dt_x + (addot * dt_x * dt_x) / 2.0 + (atdot * dt_x * dt_x * dt_x) / 6.0);
+    double a3rlm = pow(a_x, ew) * (double)omega_l / (double)omega_m;
+    double arkm = a_x * ekm / (double)omega_m;
+    double adot = sqrt(omHsq * ((a_x * a_x) * a_x) * (1.0 + arkm + a3rlm));
+    double addot = (a_x * a_x) * omHsq * (1.5 + 2.0 * arkm + w1 * a3rlm);
+    double atdot = a_x * adot * omHsq * (3.0 + 6.0 * arkm + w2 * a3rlm);
+    float da = (float)(adot * (double)dt_x + (addot * dt2) / 2.0 + (atdot * dt3) / 6.0);
     if (half == 0) {
       *da1 = da;
-      a_x = (double)a0 + (double)da;
+      a_x = (double)(a0 + da);
     } else {
       *da2 = da;
     }

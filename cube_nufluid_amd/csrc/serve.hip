@@ -28,7 +28,7 @@ extern "C" int cube_clock_init(cube_clock* c, const float* z_checkpoint, int n) 
   memset(c, 0, sizeof *c);
   for (int i = 0; i < n; i++) c->z_checkpoint[i] = z_checkpoint[i];
   c->n_checkpoint = n;
-  // initialize.f90:40-53, particle_initialization.f90:136, main.f90:42
+  // initialize.f90:40-53, particle_initialization.f90:37-38, main.f90:42
   c->a = 1.0f / (1.0f + z_checkpoint[0]);
   c->tau = -3.0f / sqrtf(1.0f / (1.0f + z_checkpoint[0]));
   c->dt_neu = 1.0f;
@@ -284,7 +284,7 @@ extern "C" int cube_ens_submit(cube_ensemble* e, const cube_params* p, const flo
 }
 
 extern "C" int cube_ens_poll(cube_ensemble* e, cube_ens_result* out, int max_results, int* n_out) {
-  if (!e || !n_out || (max_results > 0 && !out)) return CUBE_ERR_ARG;
+  if (!e || !n_out || max_results < 0 || (max_results > 0 && !out)) return CUBE_ERR_ARG;
   std::lock_guard<std::mutex> lk(e->mu);
   const int n = std::min<int>(max_results, (int)e->done.size());
   for (int i = 0; i < n; i++) out[i] = e->done[i];

@@ -78,7 +78,7 @@ class Cube:
         return self
 
     def _init_time(self, a0, z_checkpoints, z_halofind=None):
-        # initialize.f90:40-53 and particle_initialization.f90:136-138
+        # initialize.f90:40-53 and particle_initialization.f90:37-39
         self.z_checkpoints = list(z_checkpoints) if z_checkpoints else [float(1.0 / a0 - 1.0), 0.0]
         self.cur_checkpoint = 1
         self.a = f32(1.0 / (1.0 + f32(self.z_checkpoints[0])))

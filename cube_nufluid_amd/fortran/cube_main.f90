@@ -84,7 +84,7 @@ program cube_main
   call read_table(trim(kernel_dir)//'wfxyzf.3.ascii', 4096, kf)
   call read_table(trim(kernel_dir)//'wfxyzc.2.ascii', 64, kc)
 
-  ! particle_initialization.f90:133-157
+  ! particle_initialization.f90:34-82
   open(newunit=u, file=output_name(z_checkpoint(1), 'info'), status='old', access='stream')
   read(u) sim
   close(u)
@@ -111,7 +111,7 @@ program cube_main
   rc = cube_upload_state(h, sim, c_loc(xpb), c_loc(vpb), pid, rhoc, vfield)
   call cube_ok(h, rc, 'cube_upload_state')
 
-  ! initialize.f90:40-53, particle_initialization.f90:136, main.f90:42
+  ! initialize.f90:40-53, particle_initialization.f90:37-38, main.f90:42
   a = 1. / (1 + z_checkpoint(1))
   t = 0; dt = 0; dt_old = 0; da = 0; dt_neu = dt_max
   tau = -3 / sqrt(1. / (1 + z_checkpoint(1)))
@@ -266,7 +266,7 @@ contains
     deallocate(halos, words, hid)
   end subroutine
 
-  !! basic_functions.fh:252-303
+  !! basic_functions.fh:52-87
   function output_name(z, what) result(fn)
     real, intent(in) :: z
     character(*), intent(in) :: what

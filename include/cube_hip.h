@@ -33,7 +33,7 @@ enum cube_status {
   CUBE_ERR_ARG = 1,       /* invalid parameter or array size */
   CUBE_ERR_CAPACITY = 2,  /* particle capacity / displacement bound exceeded
                              (update_particle.f90:90-96, buffer_grid.f90:131-137) */
-  CUBE_ERR_FORMAT = 3,    /* header/zip mismatch (particle_initialization.f90:140-144) */
+  CUBE_ERR_FORMAT = 3,    /* header/zip mismatch (particle_initialization.f90:41-45) */
   CUBE_ERR_CHECK = 4,     /* conservation check failed (ext_pp_force.f90:154-158) */
   CUBE_ERR_HIP = 5        /* HIP runtime error */
 };
@@ -110,7 +110,7 @@ typedef struct cube_pm_out {
 int cube_init(const cube_params* p, const float* kern_f_table, const float* kern_c_table,
               cube_handle** out);
 
-/* Replaces particle_initialization.f90:133-157 (the reads) plus the halo
+/* Replaces particle_initialization.f90:34-82 (the reads) plus the halo
    set-up of buffer_grid/buffer_x/buffer_v (main.f90:20-23).  xp/vp hold
    3*nplocal components of izipx/izipv bytes; rhoc nc^3 int32; vfield
    3*nc^3 float32. */
@@ -134,7 +134,7 @@ int cube_pp(cube_handle* h, float a_mid, float dt, cube_pp_out* out);
    fine and coarse PM forces, kicks, re-quantisation, dt constraints. */
 int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out);
 
-/* Replaces expansion (timestep.f90:270-314): third-order Taylor step of the
+/* Replaces expansion (timestep.f90:136-180): third-order Taylor step of the
    Friedmann equation, real(4) interface, real(8) internals. */
 int cube_expansion(float a, float dt, float omega_m, float omega_l, float wde, float* da1, float* da2);
 

@@ -23,7 +23,7 @@
  *   ext_pp_force.f90:9-160       PP linked lists, pair force, kick
  *   kernel_f.f90:1-43            fine kernel from wfxyzf.3.ascii
  *   kernel_c.f90:1-134           coarse kernel + LRCKCORR
- *   timestep.f90:270-314         expansion (Taylor-3 Friedmann step)
+ *   timestep.f90:136-180         expansion (Taylor-3 Friedmann step)
  *   parameters.f90:8-19,111-116  fixed-point codecs
  *
  * FFTs: FFTW3 single precision (legacy sfftw_* API, version unpinned in the
@@ -1104,24 +1104,30 @@ int orc_pm(orc_t* o, float a_mid, float dt, double* out, int threads) { return p
 int orc_forces(orc_t* o, double* probe, double* out, int threads) { return pm_impl(o, 1.0f, 1.0f, out, probe, threads); }
 
 /* -------------------------------------------------------------- timestep */
-/* timestep.f90:270-314 (the default, non-'new_expansion' branch):
-   real(4) interface, real(8) internals. */
+/* timestep.f90:136-180 (the default, non-'new_expansion' branch):
+   real(4) interface, real(8) internals.  Default-real literals and
+   parameters are real(4): omHsq = 4.0/9.0 is the float 0.44444445,
+   1.0-omega_m-omega_l and the wde factors are float expressions, dt_x**2
+   and dt_x**3 are float powers of the real(4) dt_x, and a_x = a0+da1 adds
+   two real(4) values before widening.  Integer powers of the real(8) a_x
+   are products ((a*a)*a); a_x**(-3*wde) is pow(). */
+static void expansion_half(double a_x, float dt_x, float om, float ol, float wde, float* da) {
+  double omHsq = (double)(4.0f / 9.0f);
+  double a3rlm = pow(a_x, (double)(-(3.0f * wde))) * (double)ol / (double)om;
+  double arkm = a_x * (double)(1.0f - om - ol) / (double)om;
+  double adot = sqrt(omHsq * (a_x * a_x * a_x) * (1.0 + arkm + a3rlm));
+  double addot = (a_x * a_x) * omHsq * (1.5 + 2.0 * arkm + (double)(1.5f * (1.0f - wde)) * a3rlm);
+  double atdot = a_x * adot * omHsq *
+                 (3.0 + 6.0 * arkm + (double)(1.5f * (2.0f - 3.0f * wde) * (1.0f - wde)) * a3rlm);
+  float p2 = dt_x * dt_x, p3 = dt_x * dt_x * dt_x;
+  *da = (float)(adot * (double)dt_x + (addot * (double)p2) / 2.0 + (atdot * (double)p3) / 6.0);
+}
+
 void orc_expansion(float a0, float dt0, float omega_m, float omega_l, float wde, float* da1, float* da2) {
   float dt_x = dt0 / 2;
-  double a_x = a0, omHsq = 4.0 / 9.0;
-  double a3rlm = pow(a_x, -3 * (double)wde) * (double)omega_l / (double)omega_m;
-  double arkm = a_x * (1.0 - (double)omega_m - (double)omega_l) / (double)omega_m;
-  double adot = sqrt(omHsq * a_x * a_x * a_x * (1.0 + arkm + a3rlm));
-  double addot = a_x * a_x * omHsq * (1.5 + 2.0 * arkm + 1.5 * (1.0 - (double)wde) * a3rlm);
-  double atdot = a_x * adot * omHsq * (3.0 + 6.0 * arkm + 1.5 * (2.0 - 3.0 * (double)wde) * (1.0 - (double)wde) * a3rlm);
-  *da1 = (float)(adot * dt_x + (addot * dt_x * dt_x) / 2.0 + (atdot * dt_x * dt_x * dt_x) / 6.0);
-  a_x = (double)a0 + (double)*da1;
-  a3rlm = pow(a_x, -3 * (double)wde) * (double)omega_l / (double)omega_m;
-  arkm = a_x * (1.0 - (double)omega_m - (double)omega_l) / (double)omega_m;
-  adot = sqrt(omHsq * a_x * a_x * a_x * (1.0 + arkm + a3rlm));
-  addot = a_x * a_x * omHsq * (1.5 + 2.0 * arkm + 1.5 * (1.0 - (double)wde) * a3rlm);
-  atdot = a_x * adot * omHsq * (3.0 + 6.0 * arkm + 1.5 * (2.0 - 3.0 * (double)wde) * (1.0 - (double)wde) * a3rlm);
-  *da2 = (float)(adot * dt_x + (addot * dt_x * dt_x) / 2.0 + (atdot * dt_x * dt_x * dt_x) / 6.0);
+  expansion_half((double)a0, dt_x, omega_m, omega_l, wde, da1);
+  float a1 = a0 + *da1;
+  expansion_half((double)a1, dt_x, omega_m, omega_l, wde, da2);
 }
 
 /* ------------------------------------------------------- test exports */
