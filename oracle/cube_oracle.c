@@ -80,6 +80,20 @@ typedef struct orc {
 
 static const float PI_F = 3.14159274101257324f; /* parameters.f90:67 pi=4*atan(1.) */
 
+/* Planted mistakes for tests that must show they can see them (never set
+   outside tests): kicks without the 1/(6 pi) of pm.f90:146, kicks with 1 in
+   place of a_mid, and a drift by dt in place of dt_mid
+   (update_particle.f90:34).  The kick factor macros evaluate to a_mid, 6
+   and pi when no perturbation is set, so the arithmetic is unchanged. */
+#define ORC_PERTURB_NO_6PI 1
+#define ORC_PERTURB_NO_AMID 2
+#define ORC_PERTURB_DT_NOT_MID 4
+static int g_perturb = 0;
+void orc_set_perturb(int flags) { g_perturb = flags; }
+#define KA ((g_perturb & ORC_PERTURB_NO_AMID) ? 1.0f : a_mid)
+#define K6 ((g_perturb & ORC_PERTURB_NO_6PI) ? 1.0f : 6.0f)
+#define KPI ((g_perturb & ORC_PERTURB_NO_6PI) ? 1.0f : PI_F)
+
 /* ---------------------------------------------------------------- codecs */
 static inline int64_t nvbin_of(int64_t izipv) { return (int64_t)1 << (8 * izipv); }
 
@@ -505,7 +519,8 @@ int orc_drift(orc_t* o, float dt_old, float dt, double* out, int threads) {
   const int64_t lo = 1 - 2 * ncb, E = nt + 4 * ncb;
   const double xres = o->izipx == 1 ? 1.0 / 256.0 : 1.0 / 65536.0;
   const double wv = (double)0.1f; /* real(8),parameter :: weight_v=0.1 (a default-real literal) */
-  const float dt_mid = (dt_old + dt) / 2;
+  /* update_particle.f90:34; the perturbation drifts by dt instead */
+  const float dt_mid = (g_perturb & ORC_PERTURB_DT_NOT_MID) ? dt : (dt_old + dt) / 2;
   const int nlayer = 2 * (int)ceilf(dt_mid * o->vz_max / (float)o->ncell) + 1;
   const float sig = o->sigma_vi;
   int32_t* rhoc_new = (int32_t*)calloc(n3, sizeof(int32_t));
@@ -743,7 +758,7 @@ int orc_pp(orc_t* o, float a_mid, float dt, double* out, float* fpp_out, int thr
                   for (int d = 0; d < 3; d++) fpp_out[ip1 * 3 + d] = (float)ftot[d];
                 for (int d = 0; d < 3; d++) {
                   double v = orc_vdecode(o->vp[ip1 * 3 + d], sig, o->izipv);
-                  v = v + ftot[d] * (double)a_mid * (double)dt / 6.0 / (double)PI_F;
+                  v = v + ftot[d] * (double)KA * (double)dt / (double)K6 / (double)KPI;
                   o->vp[ip1 * 3 + d] = (int16_t)orc_vencode(v, sig, o->izipv);
                 }
                 float f2 = (float)(ftot[0] * ftot[0] + ftot[1] * ftot[1] + ftot[2] * ftot[2]);
@@ -912,14 +927,14 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
                     continue;
                   }
                   double v = orc_vdecode(o->vp[ip * 3 + d], sig_old, o->izipv);
-                  v = v + (double)(FF(d, a1[0], a1[1], a1[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx1[1] * dx1[2]);
-                  v = v + (double)(FF(d, a2[0], a1[1], a1[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx1[1] * dx1[2]);
-                  v = v + (double)(FF(d, a1[0], a2[1], a1[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx2[1] * dx1[2]);
-                  v = v + (double)(FF(d, a1[0], a1[1], a2[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx1[1] * dx2[2]);
-                  v = v + (double)(FF(d, a1[0], a2[1], a2[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx2[1] * dx2[2]);
-                  v = v + (double)(FF(d, a2[0], a1[1], a2[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx1[1] * dx2[2]);
-                  v = v + (double)(FF(d, a2[0], a2[1], a1[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx2[1] * dx1[2]);
-                  v = v + (double)(FF(d, a2[0], a2[1], a2[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx2[1] * dx2[2]);
+                  v = v + (double)(FF(d, a1[0], a1[1], a1[2]) * KA * dt / K6 / KPI * dx1[0] * dx1[1] * dx1[2]);
+                  v = v + (double)(FF(d, a2[0], a1[1], a1[2]) * KA * dt / K6 / KPI * dx2[0] * dx1[1] * dx1[2]);
+                  v = v + (double)(FF(d, a1[0], a2[1], a1[2]) * KA * dt / K6 / KPI * dx1[0] * dx2[1] * dx1[2]);
+                  v = v + (double)(FF(d, a1[0], a1[1], a2[2]) * KA * dt / K6 / KPI * dx1[0] * dx1[1] * dx2[2]);
+                  v = v + (double)(FF(d, a1[0], a2[1], a2[2]) * KA * dt / K6 / KPI * dx1[0] * dx2[1] * dx2[2]);
+                  v = v + (double)(FF(d, a2[0], a1[1], a2[2]) * KA * dt / K6 / KPI * dx2[0] * dx1[1] * dx2[2]);
+                  v = v + (double)(FF(d, a2[0], a2[1], a1[2]) * KA * dt / K6 / KPI * dx2[0] * dx2[1] * dx1[2]);
+                  v = v + (double)(FF(d, a2[0], a2[1], a2[2]) * KA * dt / K6 / KPI * dx2[0] * dx2[1] * dx2[2]);
                   o->vp[ip * 3 + d] = (int16_t)orc_vencode(v, sig_new, o->izipv);
                 }
               }
@@ -1060,14 +1075,14 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
           continue;
         }
         double v = orc_vdecode(o->vp[ip * 3 + d], sig_c, o->izipv);
-        v = v + (double)(FC(d, a1[0], a1[1], a1[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx1[1] * dx1[2]);
-        v = v + (double)(FC(d, a2[0], a1[1], a1[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx1[1] * dx1[2]);
-        v = v + (double)(FC(d, a1[0], a2[1], a1[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx2[1] * dx1[2]);
-        v = v + (double)(FC(d, a1[0], a1[1], a2[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx1[1] * dx2[2]);
-        v = v + (double)(FC(d, a1[0], a2[1], a2[2]) * a_mid * dt / 6 / PI_F * dx1[0] * dx2[1] * dx2[2]);
-        v = v + (double)(FC(d, a2[0], a1[1], a2[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx1[1] * dx2[2]);
-        v = v + (double)(FC(d, a2[0], a2[1], a1[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx2[1] * dx1[2]);
-        v = v + (double)(FC(d, a2[0], a2[1], a2[2]) * a_mid * dt / 6 / PI_F * dx2[0] * dx2[1] * dx2[2]);
+        v = v + (double)(FC(d, a1[0], a1[1], a1[2]) * KA * dt / K6 / KPI * dx1[0] * dx1[1] * dx1[2]);
+        v = v + (double)(FC(d, a2[0], a1[1], a1[2]) * KA * dt / K6 / KPI * dx2[0] * dx1[1] * dx1[2]);
+        v = v + (double)(FC(d, a1[0], a2[1], a1[2]) * KA * dt / K6 / KPI * dx1[0] * dx2[1] * dx1[2]);
+        v = v + (double)(FC(d, a1[0], a1[1], a2[2]) * KA * dt / K6 / KPI * dx1[0] * dx1[1] * dx2[2]);
+        v = v + (double)(FC(d, a1[0], a2[1], a2[2]) * KA * dt / K6 / KPI * dx1[0] * dx2[1] * dx2[2]);
+        v = v + (double)(FC(d, a2[0], a1[1], a2[2]) * KA * dt / K6 / KPI * dx2[0] * dx1[1] * dx2[2]);
+        v = v + (double)(FC(d, a2[0], a2[1], a1[2]) * KA * dt / K6 / KPI * dx2[0] * dx2[1] * dx1[2]);
+        v = v + (double)(FC(d, a2[0], a2[1], a2[2]) * KA * dt / K6 / KPI * dx2[0] * dx2[1] * dx2[2]);
         float av = (float)fabs(v + (double)o->vfield[r * 3 + d]);
         if (d == 0 && av > vm0) vm0 = av;
         if (d == 1 && av > vm1) vm1 = av;

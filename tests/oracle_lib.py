@@ -45,6 +45,7 @@ def load():
         "orc_pm": (ctypes.c_int, [_P, _f32, _f32, _P, ctypes.c_int]),
         "orc_forces": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
         "orc_expansion": (None, [_f32, _f32, _f32, _f32, _f32, _P, _P]),
+        "orc_set_perturb": (None, [ctypes.c_int]),
         "orc_vdecode": (_f64, [_i32, _f32, _i64]),
         "orc_vencode": (_i32, [_f64, _f32, _i64]),
         "orc_fft_r2c": (None, [_P, _i64, _P, _P, ctypes.c_int]),
