@@ -112,6 +112,7 @@ SIGNATURES = {
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
     "cube_power_spectrum": (c_i32, [_P, c_i32, _P, _P, _P]),
+    "cube_power_spectrum_noalloc": (c_i32, [_P, c_i32, _P, _P, _P]),
     "cube_halofind": (c_i32, [_P, _P, _P]),
     "cube_nu_init": (c_i32, [_P, ctypes.POINTER(NuParams)]),
     "cube_nu_info": (c_i32, [_P, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_f32), ctypes.POINTER(c_f32),

@@ -137,6 +137,7 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
   h->nf = h->nc * CUBE_NCELL;
   h->n3c = h->nc * h->nc * h->nc;
   if (hipSetDevice(p->device) != hipSuccess) return fail(cube_fail(h, CUBE_ERR_HIP, "hipSetDevice(%d) failed", p->device));
+  if ((rc = pk_scratch_setup(h))) return fail(rc);
 #define TRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fail(cube_fail(h, CUBE_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_))); } while (0)
   TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   const int64_t npm = p->np_max;

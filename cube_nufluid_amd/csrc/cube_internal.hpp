@@ -294,6 +294,7 @@ int fine_kernel_build(cube_handle* h, const float* table);
 int coarse_kernel_build(cube_handle* h, const float* table);
 // pm.hip
 int cic_fine(cube_handle* h, double* mass);
+int pk_scratch_setup(cube_handle* h);  // device address of the P(k) scratch array on h's GPU
 int cic_coarse(cube_handle* h, double* mass);
 // fine force components Ff[d] (row pitch pf[d]) and the coarse force fc
 // f2max_bits (optional): max |F|^2 over all fine nodes, from the staged slabs

@@ -317,16 +317,17 @@ class Cube:
         _lib.check(self.lib, self.h, self.lib.cube_probe_pm_forces(self.h, _lib.ptr(ff), _lib.ptr(fc)))
         return ff, fc
 
-    def power_spectrum(self):
+    def power_spectrum(self, noalloc=False):
         """(k in h/Mpc, Delta^2, modes) per linear bin 1..nbin of the fine
-        mesh (cicpower.f90 / powerspectrum.f90 estimator)."""
+        mesh (cicpower.f90 / powerspectrum.f90 estimator).  noalloc selects
+        the entry point that allocates no device memory."""
         nf = self.cfg.nf
         nbin = int(np.rint(nf // 2 * np.sqrt(3.0)))
         modes = np.zeros(nbin)
         ksum = np.zeros(nbin)
         d2 = np.zeros(nbin)
-        _lib.check(self.lib, self.h, self.lib.cube_power_spectrum(self.h, nbin, _lib.ptr(modes), _lib.ptr(ksum),
-                                                                  _lib.ptr(d2)))
+        fn = self.lib.cube_power_spectrum_noalloc if noalloc else self.lib.cube_power_spectrum
+        _lib.check(self.lib, self.h, fn(self.h, nbin, _lib.ptr(modes), _lib.ptr(ksum), _lib.ptr(d2)))
         m = np.maximum(modes, 1)
         return ksum / m * 2 * np.pi / self.cfg.box, d2 / m, modes
 

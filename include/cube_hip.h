@@ -166,6 +166,16 @@ int cube_probe_density(cube_handle* h, float* rho);
    Overwrites the fine mesh (recomputed by the next cube_pm). */
 int cube_power_spectrum(cube_handle* h, int nbin, double* modes, double* ksum, double* delta2sum);
 
+/* No counterpart in the reference.  cube_power_spectrum without any device
+   memory allocation, for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   per-bin partial sums live in a static device array of the library (one
+   per GPU, 48 KiB, held until the process ends), so nbin is limited to 2048
+   (a 2364^3 mesh); larger nbin is rejected with CUBE_ERR_ARG.  Same
+   arguments, results and errors as cube_power_spectrum otherwise.  All
+   calls on one GPU share the array: calls that overlap on one GPU race. */
+int cube_power_spectrum_noalloc(cube_handle* h, int nbin, double* modes, double* ksum, double* delta2sum);
+
 /* Runtime halo finder -- halofind.f90:18-598 (called from main.f90:88-90
    at the z_halofind redshifts): spherical-overdensity halos of the current
    state, tile by tile as the reference runs it.  Parameters are the
