@@ -100,6 +100,28 @@ def copy_calibration():
         log("copy 4 GiB: %.3f ms" % ((time.perf_counter() - t) * 1e3))
 
 
+def host_threads():
+    """Host threads the CPU legs use: the cores this process may run on,
+    capped by OMP_NUM_THREADS when the environment sets it (the CPU share
+    given to one GPU), with the host's nproc and CPU model for the record."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info = {"nproc": os.cpu_count(), "affinity": avail, "omp_num_threads": env, "cpu_model": model}
+    return threads, info
+
+
 def make_state(cfg, seed, z):
     from cube_nufluid_amd.ic import zeldovich_ic
     return zeldovich_ic(cfg, seed=seed, z=z)
@@ -291,7 +313,7 @@ def pp_unit_bench(config, seed, cpu, reps=3):
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import Oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads, _ = host_threads()
         orc = Oracle(cfg, threads=threads)
         orc.set_state(st)
         orc.lib.orc_set_tile_limit(orc.o, 1)
@@ -303,6 +325,68 @@ def pp_unit_bench(config, seed, cpu, reps=3):
                       "sample": "oracle PP on the first of %d tiles (%d pairs, %.1f s)" % (cfg.nnt ** 3, r["npairs"], cs)}
         log("pp unit cpu: %.3g pairs/s on one tile" % (r["npairs"] / cs))
     return res
+
+
+def c4_bench(steps, warmup, seed=1004):
+    """C4: 512^3 CDM particles on a 1024^3 fine mesh with the neutrino fluid
+    (sum m_nu = 0.1 eV as three degenerate species; fluids 512^3, 256^3,
+    128^3), seeded ICs at z = 10 with the fluids from the same linear field
+    (neutrino_ic.f90 filters).  Particle-steps/s over `steps` full steps
+    (both fluid half-step sweeps, drift, PP, PM with the fluid coupled), the
+    fluid sweeps' time per step, and the sweep kernels against HBM: 3 sweeps
+    x 2 (read + write) x 20 B per cell of each fluid's (nc+2 ncb_h)^3 mesh
+    per half-step (the reference's halo-extended arrays; the periodic sweeps
+    here move nc^3 cells)."""
+    import torch
+    from cube_nufluid_amd.params import CONFIGS
+    from cube_nufluid_amd.driver import Cube
+    from cube_nufluid_amd import _lib
+    from cube_nufluid_amd.checkpoint import neu_ncb
+    cfg = CONFIGS["C4"]
+    t0 = time.perf_counter()
+    cube = Cube.from_ic(cfg, seed, cfg.z_i, z_checkpoints=[cfg.z_i, -0.9])
+    npart = cube.sim.nplocal
+    log("C4: %d particles + fluids %s, IC in %.1f s" % (npart, [cube.nu_info(n)["nc"] for n in range(3)],
+                                                         time.perf_counter() - t0))
+    try:
+        for _ in range(warmup):
+            cube.step()
+        lib = cube.lib
+        lib.cube_profile_enable(cube.h, 1)
+        lib.cube_profile_read(cube.h, None, None, 0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            cube.step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        nslot = 32
+        ms = np.zeros(nslot)
+        cnt = np.zeros(nslot, dtype=np.int64)
+        ns = lib.cube_profile_read(cube.h, _lib.ptr(ms), _lib.ptr(cnt), nslot)
+        names = []
+        for s in range(ns):
+            b = ctypes.create_string_buffer(64)
+            lib.cube_profile_slot_name(s, b, 64)
+            names.append(b.value.decode())
+        per = {names[s]: round(float(ms[s]) / steps, 3) for s in range(ns) if cnt[s]}
+        fluid_ms = per.get("fluid", 0.0)
+        infos = [cube.nu_info(n) for n in range(3)]
+        # two half-step evolves per step, each 3 sweeps of every fluid
+        fbytes = sum(2 * 3 * 2 * 20.0 * (i["nc"] + 2 * neu_ncb(i["dx"], cfg.ncb)) ** 3 for i in infos)
+        res = {"workload": "C4: %d^3 CDM particles, %d^3 fine mesh, fluids %s, %.0f Mpc/h, z_i = %.0f, "
+                           "sum m_nu = 0.1 eV" % (round(npart ** (1 / 3)), cfg.nf, [i["nc"] for i in infos],
+                                                   cfg.box, cfg.z_i),
+               "particle_steps_per_s": npart * steps / wall, "ms_per_step": wall / steps * 1e3, "steps": steps,
+               "z_after": round(float(1 / cube.a - 1), 4), "fluid_ms_per_step": fluid_ms,
+               "fluid_algorithmic_GB_per_step": round(fbytes / 1e9, 3),
+               "fluid_achieved_GBs": round(fbytes / (fluid_ms * 1e-3) / 1e9, 1) if fluid_ms else None,
+               "per_kernel_ms_per_step": per}
+        log("C4: %.3g particle-steps/s, %.1f ms/step, fluid %.1f ms/step" % (res["particle_steps_per_s"],
+                                                                          res["ms_per_step"], fluid_ms))
+        return res
+    finally:
+        cube.close()
 
 
 def ensemble_bench(n, rate):
@@ -345,12 +429,18 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default=os.environ.get("CUBE_BENCH_CONFIG", "C3"))
-    ap.add_argument("--z", type=float, default=0.0, help="redshift of the Zel'dovich start state")
+    ap.add_argument("--z", type=float, default=3.0,
+                    help="redshift of the Zel'dovich start state, evolved to z=0 before the timed steps")
+    ap.add_argument("--evolve-steps", type=int, default=400,
+                    help="cap on the CUBE steps that evolve the start state to z=0 (0: time the Zel'dovich state)")
+    ap.add_argument("--evolve-seconds", type=float, default=240.0, help="wall-clock cap on that evolution")
     ap.add_argument("--seed", type=int, default=1003)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
     ap.add_argument("--no-halofind", action="store_true", help="skip the halo-finder measurement")
     ap.add_argument("--no-pp-unit", action="store_true", help="skip the PP unit benchmark on NFW clumps")
+    ap.add_argument("--no-c4", action="store_true", help="skip the CDM + neutrino-fluid (C4) measurement")
+    ap.add_argument("--c4-only", action="store_true", help="only the C4 measurement (for profiling)")
     ap.add_argument("--halofind-only", action="store_true",
                     help="only generate the state and run the halo finder on it once (for profiling)")
     ap.add_argument("--ens-requests", type=int, default=16)
@@ -372,15 +462,38 @@ def main():
     from cube_nufluid_amd.driver import Cube
     from cube_nufluid_amd import _lib
 
+    if args.c4_only:
+        print(json.dumps({"c4": c4_bench(args.steps, args.warmup)}), flush=True)
+        return
     cfg = CONFIGS[args.config]
     z0 = args.z
     t0 = time.perf_counter()
-    # the next checkpoint redshift lies beyond the timed steps
-    cube = Cube.from_ic(cfg, args.seed, z0, device=0, z_checkpoints=[z0, min(0.0, z0) - 0.9])
+    evolve = args.evolve_steps > 0 and z0 > 0
+    # evolved: the main loop from the Zel'dovich start to the z=0 checkpoint,
+    # then the timed steps continue past it; otherwise the next checkpoint
+    # redshift lies beyond the timed steps
+    cube = Cube.from_ic(cfg, args.seed, z0, device=0, z_checkpoints=[z0, 0.0] if evolve else [z0, min(0.0, z0) - 0.9])
     npart = cube.sim.nplocal
     log("state %s: %d particles, Zel'dovich at z=%.2f generated in HBM (%.1f s incl. init)"
         % (args.config, npart, z0, time.perf_counter() - t0))
     lib = cube.lib
+    state = {"start": "seeded Zel'dovich state at z=%.2f" % z0}
+    if evolve:
+        t1 = time.perf_counter()
+        nev = 0
+        done = False
+        while nev < args.evolve_steps and time.perf_counter() - t1 < args.evolve_seconds and not done:
+            done = cube.step()
+            nev += 1
+            if nev % 25 == 0 or done:
+                log("evolve step %d: z=%.3f dt=%.4g (%.1f s)" % (nev, 1 / cube.a - 1, cube.dt, time.perf_counter() - t1))
+        z_now = float(1 / cube.a - 1)
+        state.update({"evolved_steps": nev, "evolve_seconds": round(time.perf_counter() - t1, 1),
+                      "z_reached": round(z_now, 4), "reached_z0": bool(done)})
+        if cfg.pp_on and getattr(cube, "last_pp", None) is not None:
+            state["pairs_per_particle_at_end"] = round(cube.last_pp.npairs / npart, 2)
+    state["description"] = ("%s evolved by the CUBE step on the GPU to z=%.3f in %d steps" % (
+        state["start"], state["z_reached"], state["evolved_steps"]) if evolve else state["start"])
     if args.halofind_only:
         print(json.dumps({"halofind": halofind_c3(cube, args.config)}), flush=True)
         cube.close()
@@ -397,7 +510,7 @@ def main():
         cube.step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    nslot = 16
+    nslot = 32
     ms = np.zeros(nslot)
     cnt = np.zeros(nslot, dtype=np.int64)
     ns = lib.cube_profile_read(cube.h, _lib.ptr(ms), _lib.ptr(cnt), nslot)
@@ -428,8 +541,11 @@ def main():
         pairs = float(cube.last_pp.npairs)
         pp_info = {"bound": "fp32 valu", "pairs_per_step": pairs, "mean_launch_ms": round(pp_s * 1e3, 4),
                    "pairs_per_s": pairs / pp_s, "achieved_tflops": round(30 * pairs / pp_s / 1e12, 3),
-                   "peak_tflops": FP32_PEAK_TFLOPS, "flops_per_pair": 30}
+                   "peak_tflops": FP32_PEAK_TFLOPS, "flops_per_pair": 30,
+                   "pairs_per_particle": round(pairs / npart, 2)}
     value = npart * args.steps / wall
+    state["target_particle_steps_per_s"] = 1.5e9
+    state["meets_target_on_this_state"] = bool(value >= 1.5e9)
     out = {
         "metric": "particle-steps/s (PM+PP, %d^3 on %d^3 mesh)" % (round(npart ** (1 / 3)), cfg.nf),
         "value": value,
@@ -442,9 +558,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Zel'dovich state, EH no-wiggle LCDM)",
+        "data": "synthetic: %s (EH no-wiggle LCDM)" % state["description"],
         "config": {"workload": args.config, "particles": npart, "fine_mesh": cfg.nf, "coarse_mesh": cfg.nc,
-                   "box_mpc_h": cfg.box, "z_start": z0, "izipx": cfg.izipx, "izipv": cfg.izipv,
+                   "box_mpc_h": cfg.box, "z_start": z0, "state": state, "izipx": cfg.izipx, "izipv": cfg.izipv,
                    "pp": bool(cfg.pp_on), "pp_range": cfg.pp_range, "parallelism": "single GPU"},
         "roofline": roofline,
         "pp": pp_info,
@@ -461,13 +577,16 @@ def main():
         out["halofind"]["evolved"] = halofind_evolved("C2", 1002, cpu=not args.no_cpu_baseline)
     if not args.no_pp_unit:
         out["pp_unit"] = pp_unit_bench("C2", 1002, cpu=not args.no_cpu_baseline)
+    if not args.no_c4:
+        out["c4"] = c4_bench(min(args.steps, 5), 1)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ens_requests, args.ens_rate)
     if st_cpu is not None:
-        threads = min(16, os.cpu_count() or 1)
+        threads, hinfo = host_threads()
         per_tile = npart / cfg.nnt ** 3
         tiles = args.cpu_tiles or (1 if per_tile > 4e6 else min(threads, cfg.nnt ** 3))
         out["cpu_baseline"] = cpu_baseline(cfg, st_cpu, last_dt, tiles, threads)
+        out["cpu_baseline"]["host"] = hinfo
     print(json.dumps(out), flush=True)
 
 

@@ -68,7 +68,8 @@ class PMOut(ctypes.Structure):
 
 class ICParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("np_nc", c_i32), ("nk", c_i32), ("a", c_f32), ("box", c_f32),
-                ("vfac", c_f64), ("lnk", ctypes.c_void_p), ("lnp", ctypes.c_void_p)]
+                ("vfac", c_f64), ("lnk", ctypes.c_void_p), ("lnp", ctypes.c_void_p), ("f_cdm", c_f32),
+                ("reserved", c_i32)]
 
 
 assert ctypes.sizeof(SimHeader) == 224
@@ -131,6 +132,7 @@ SIGNATURES = {
     "cube_ens_clock": (c_f64, [_P]),
     "cube_ens_destroy": (c_i32, [_P]),
     "cube_ic_zeldovich": (c_i32, [_P, ctypes.POINTER(ICParams), ctypes.POINTER(SimHeader)]),
+    "cube_nu_ic": (c_i32, [_P, ctypes.POINTER(ICParams)]),
     "cube_profile_enable": (c_i32, [_P, c_i32]),
     "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),
     "cube_profile_slot_name": (c_i32, [c_i32, ctypes.c_char_p, ctypes.c_size_t]),

@@ -205,7 +205,7 @@ extern "C" int cube_profile_enable(cube_handle* h, int on) {
 static const char* kSlotNames[PS_NSLOT] = {"drift_pass1", "drift_scan", "drift_scatter", "drift_sort_big", "drift_write",
                                            "fluid", "pp", "cic_coarse",
                                            "fft_coarse", "cic_fine", "fft_x_r2c", "fft_y_fwd", "fft_zconv",
-                                           "fft_y_inv", "fft_x_c2r", "kick"};
+                                           "fft_y_inv", "fft_x_c2r", "kick", "fluid_density", "fluid_gravity"};
 
 extern "C" int cube_profile_read(cube_handle* h, double* ms, int64_t* count, int nslot) {
   if (!h || nslot < 0) return CUBE_ERR_ARG;
@@ -354,7 +354,11 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   // coarse mesh: force_c for all three components (pm.f90:188-326); the
   // third inverse also forms max |force_c|^2 from the two already written
   if ((r = cic_coarse(h, &mass_c))) return r;
-  if (nu && (r = fl_density_coarse(h, a_mid))) return r;
+  double mass_cnu = 0;
+  int prd = nu ? prof_begin(h, PS_FLUID_DENS) : -1;
+  if (nu && (r = fl_density_coarse(h, a_mid, &mass_cnu))) return r;
+  if (nu) prof_end(h, prd);
+  mass_c += mass_cnu;
   if ((r = mesh_fwd_xy(h, h->coarse))) return r;
   const int64_t Nc = h->coarse.N;
   const char* fc = (const char*)h->fc;
@@ -370,7 +374,11 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   // S (F0) and R (F1), component 2 in place in W (the spectrum's last use)
   // and inverted into X (F2).  All three stay resident for one fused kick.
   if ((r = cic_fine(h, &mass_f))) return r;
-  if (nu && (r = fl_density_fine(h, a_mid))) return r;
+  double mass_nu = 0;
+  prd = nu ? prof_begin(h, PS_FLUID_DENS) : -1;
+  if (nu && (r = fl_density_fine(h, a_mid, &mass_nu))) return r;
+  if (nu) prof_end(h, prd);
+  mass_f += mass_nu;
   if ((r = mesh_t_fwd(h, h->fine, h->fine.R, h->fine.W))) return r;
   const int64_t Nf = h->fine.N;
   const void* Ff[3] = {h->fine.S, h->fine.R, h->fine.X};
@@ -384,9 +392,11 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   // the kick also forms max |F|^2 (dt_fine) from the force slabs it stages
   if ((r = kick_all(h, Ff, pf, a_mid, dt, c_old, c_new, vmx, f2f, probe_f, probe_c))) return r;
   const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
+  const int prg = nu_kick ? prof_begin(h, PS_FLUID_GRAV) : -1;
   for (int d = 0; nu_kick && d < 3; d++)
     if ((r = fl_gravity_fine(h, d, Ff[d], pf[d], a_mid, dt))) return r;
   if (nu_kick && (r = fl_gravity_finish(h, a_mid, dt))) return r;
+  if (nu_kick) prof_end(h, prg);
   uint32_t hv[5];
   HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 32, sizeof(hv), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));

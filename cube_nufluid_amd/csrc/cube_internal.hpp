@@ -45,7 +45,7 @@ struct Mesh {
 // Per-kernel timing with HIP events on the handle's stream (diagnostic).
 enum ProfSlot {
   PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT_SCATTER, PS_DRIFT_SORT, PS_DRIFT_WRITE, PS_FLUID, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
-  PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_NSLOT
+  PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_FLUID_DENS, PS_FLUID_GRAV, PS_NSLOT
 };
 struct ProfRec { int slot; hipEvent_t a, b; };
 
@@ -305,8 +305,8 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
 // drift.hip
 int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
 // fluid.hip: PM coupling of the neutrino fluid
-int fl_density_coarse(cube_handle* h, float a_mid);
-int fl_density_fine(cube_handle* h, float a_mid);
+int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added);
+int fl_density_fine(cube_handle* h, float a_mid, double* mass_added);
 int fl_gravity_fine(cube_handle* h, int d, const void* F, int64_t pitch, float a_mid, float dt);
 int fl_gravity_finish(cube_handle* h, float a_mid, float dt);
 void fl_free(NuState& s);

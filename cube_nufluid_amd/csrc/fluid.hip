@@ -197,27 +197,51 @@ __global__ void k_fl_density_centres(FlSet s, int64_t nf, float* D) {
   D[(z * nf + y) * (nf + 2) + x] = d;
 }
 // fine mesh (pm.f90:57-72): node n gets f_nu * density(n + 0.5), or the
-// homogeneous f_nu before the fluid starts
+// homogeneous f_nu before the fluid starts.  Per-block sums of what was
+// added go to part[blockIdx.x]: the reference's testrhof sums rho_f after
+// the fluid is in (pm.f90:107).
 template <typename T>
-__global__ void k_fl_add_fine(T* R, const float* D, int64_t nf, float fnu, int homog) {
+__global__ __launch_bounds__(256) void k_fl_add_fine(T* R, const float* D, int64_t nf, float fnu, int homog, double* part) {
+  __shared__ double red[4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf * nf * nf) return;
-  const int64_t x = i % nf, y = (i / nf) % nf, z = i / (nf * nf), q = (z * nf + y) * (nf + 2) + x;
-  R[q] = R[q] + (T)(homog ? fnu : D[q] * fnu);
+  double added = 0;
+  if (i < nf * nf * nf) {
+    const int64_t x = i % nf, y = (i / nf) % nf, z = i / (nf * nf), q = (z * nf + y) * (nf + 2) + x;
+    const T v = (T)(homog ? fnu : D[q] * fnu);
+    R[q] = R[q] + v;
+    added = (double)v;
+  }
+  for (int o = 32; o > 0; o >>= 1) added += __shfl_xor(added, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = added;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 // coarse mesh (pm.f90:198-242): every fine-cell centre's f_nu*density CIC
 // onto coarse nodes at cell centres, gathered per node (2*ncell taps per
-// dimension, the reference's dx1/dx2 weights)
+// dimension, the reference's dx1/dx2 weights).  Per-block sums of what was
+// added go to part[blockIdx.x] (testrhoc sums r3 with the fluid in,
+// pm.f90:280).
 template <typename T>
-__global__ void k_fl_add_coarse(T* Rc, const float* D, int64_t nf, int ncc, float fnu, int homog) {
+__device__ __forceinline__ double fl_coarse_node(T* Rc, const float* D, int64_t nf, int ncc, float fnu, int homog, int64_t i);
+template <typename T>
+__global__ __launch_bounds__(256) void k_fl_add_coarse(T* Rc, const float* D, int64_t nf, int ncc, float fnu, int homog,
+                                                       double* part) {
+  __shared__ double red[4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n3 = (int64_t)ncc * ncc * ncc;
-  if (i >= n3) return;
+  double added = i < (int64_t)ncc * ncc * ncc ? fl_coarse_node(Rc, D, nf, ncc, fnu, homog, i) : 0.0;
+  for (int o = 32; o > 0; o >>= 1) added += __shfl_xor(added, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = added;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+template <typename T>
+__device__ __forceinline__ double fl_coarse_node(T* Rc, const float* D, int64_t nf, int ncc, float fnu, int homog, int64_t i) {
   const int cx = (int)(i % ncc), cy = (int)((i / ncc) % ncc), cz = (int)(i / ((int64_t)ncc * ncc));
   const int64_t q = ((int64_t)cz * ncc + cy) * (ncc + 2) + cx;
   if (homog) {
-    Rc[q] = Rc[q] + (T)(fnu * (float)(CUBE_NCELL * CUBE_NCELL * CUBE_NCELL));
-    return;
+    const T v = (T)(fnu * (float)(CUBE_NCELL * CUBE_NCELL * CUBE_NCELL));
+    Rc[q] = Rc[q] + v;
+    return (double)v;
   }
   double acc = 0;
   const int c[3] = {cx, cy, cz};
@@ -241,6 +265,7 @@ __global__ void k_fl_add_coarse(T* Rc, const float* D, int64_t nf, int ncc, floa
         acc += (double)(rnu * wt[0][jx] * wt[1][jy] * wt[2][jz]);
       }
   Rc[q] = Rc[q] + (T)acc;
+  return (double)(T)acc;
 }
 
 // ---------------------------------------------------------------- gravity
@@ -492,7 +517,7 @@ static FlSet fl_set(const NuState& s) {
 
 // Coarse-mesh density (before the coarse FFT); leaves the fine-cell-centre
 // density in fine.W for fl_density_fine.
-int fl_density_coarse(cube_handle* h, float a_mid) {
+int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added) {
   NuState& s = h->nu;
   const int homog = 1.0f / a_mid - 1.0f > s.par.z_i_nu;
   const int64_t nf = h->nf;
@@ -502,24 +527,33 @@ int fl_density_coarse(cube_handle* h, float a_mid) {
     HIP_TRY(h, hipGetLastError());
   }
   const int64_t n3c = h->n3c;
+  const int64_t nb = (int64_t)nblk(n3c);
+  if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   if (h->prec == 4)
-    k_fl_add_coarse<float><<<nblk(n3c), 256, 0, h->stream>>>((float*)h->coarse.R, D, nf, (int)h->nc, s.par.f_neu, homog);
+    k_fl_add_coarse<float><<<(unsigned)nb, 256, 0, h->stream>>>((float*)h->coarse.R, D, nf, (int)h->nc, s.par.f_neu, homog,
+                                                                h->dred);
   else
-    k_fl_add_coarse<double><<<nblk(n3c), 256, 0, h->stream>>>((double*)h->coarse.R, D, nf, (int)h->nc, s.par.f_neu, homog);
+    k_fl_add_coarse<double><<<(unsigned)nb, 256, 0, h->stream>>>((double*)h->coarse.R, D, nf, (int)h->nc, s.par.f_neu, homog,
+                                                                 h->dred);
   HIP_TRY(h, hipGetLastError());
+  if (mass_added) return reduce_doubles(h, h->dred, nb, 1, mass_added);
   return 0;
 }
 
-int fl_density_fine(cube_handle* h, float a_mid) {
+int fl_density_fine(cube_handle* h, float a_mid, double* mass_added) {
   NuState& s = h->nu;
   const int homog = 1.0f / a_mid - 1.0f > s.par.z_i_nu;
   const int64_t nf = h->nf;
+  const int64_t nb = (int64_t)nblk(nf * nf * nf);
+  if (nb > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   if (h->prec == 4)
-    k_fl_add_fine<float><<<nblk(nf * nf * nf), 256, 0, h->stream>>>((float*)h->fine.R, (const float*)h->fine.W, nf, s.par.f_neu, homog);
+    k_fl_add_fine<float><<<(unsigned)nb, 256, 0, h->stream>>>((float*)h->fine.R, (const float*)h->fine.W, nf, s.par.f_neu, homog,
+                                                              h->dred);
   else
-    k_fl_add_fine<double><<<nblk(nf * nf * nf), 256, 0, h->stream>>>((double*)h->fine.R, (const float*)h->fine.W, nf, s.par.f_neu,
-                                                                     homog);
+    k_fl_add_fine<double><<<(unsigned)nb, 256, 0, h->stream>>>((double*)h->fine.R, (const float*)h->fine.W, nf, s.par.f_neu,
+                                                               homog, h->dred);
   HIP_TRY(h, hipGetLastError());
+  if (mass_added) return reduce_doubles(h, h->dred, nb, 1, mass_added);
   return 0;
 }
 

@@ -251,7 +251,9 @@ extern "C" int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_
   h->sigma_vi = sigma;
   h->sigma_vi_new = sigma;
   h->vz_max = vz_max;
-  h->mass_p = (float)((double)h->nf * h->nf * h->nf / (double)n);
+  // initial_conditions.f90:943, real(4): f_cdm*nf_global**3/npglobal
+  const float f_cdm = ip->f_cdm > 0 ? ip->f_cdm : 1.0f;
+  h->mass_p = f_cdm * (float)(h->nf * h->nf * h->nf) / (float)n;
   h->ftemp_pp = 0;
   cube_sim_header& s = h->sim;
   memset(&s, 0, sizeof s);
@@ -267,5 +269,151 @@ extern "C" int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_
   s.z_i_nu = 10.0f;
   s.vz_max = vz_max;
   if (sim_out) *sim_out = s;
+  return CUBE_OK;
+}
+
+// ------------------------------------------------------------ fluid ICs
+// Neutrino fluid initial conditions (utilities/neutrino_ic.f90:40-181): the
+// CDM linear density and velocity of the same seeded field, filtered per
+// fluid by the free-streaming transfer functions
+//   density  tf = 1/(1 + l + l^2),
+//   velocity tf = (1 + 1.5 l + 2 l^2)/(1 + l + l^2)^2,
+// l = k/kfs, kfs = sqrt(1.5 omega_m/(1+z_i_nu))*100 / v_n (v_n the fluid's
+// Gauss-Laguerre velocity in km/s), then block-averaged onto the fluid mesh
+// (hg_set_fld, hydrogr.f90:288-308): rho = <1 + delta>, m = rho <v>,
+// E = rho cs2/(g-1) + m^2/(2 rho) per component.  The CDM linear fields live
+// on the particle lattice here (ng = np_nc*nc per dimension; the reference
+// reads them on the fine grid), so the block factor is ng/nc_fluid.
+__global__ void k_nu_ic_mult(const float2* C, float2* W, int64_t ng, int field, double kf, double scale, const double* lnk,
+                             const double* lnp, int nk, float kfs, float box) {
+  const int64_t H = ng / 2 + 1;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= H * ng * ng) return;
+  const int64_t kx = i % H, ky = (i / H) % ng, kz = i / (H * ng);
+  const int64_t sx = kx, sy = ky < ng / 2 ? ky : ky - ng, sz = kz < ng / 2 ? kz : kz - ng;
+  float2 out{0.f, 0.f};
+  const int e = field - 1;
+  const int64_t se = e == 0 ? sx : (e == 1 ? sy : sz);
+  const bool odd_nyq = field > 0 && (se == ng / 2 || se == -ng / 2);
+  if ((sx | sy | sz) != 0 && !odd_nyq) {
+    const double k2 = kf * kf * (double)(sx * sx + sy * sy + sz * sz);
+    const double lk = 0.5 * log(k2);
+    double lp;
+    if (lk <= lnk[0]) lp = lnp[0];
+    else if (lk >= lnk[nk - 1]) lp = lnp[nk - 1];
+    else {
+      int lo = 0, hi = nk - 1;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lnk[mid] <= lk) lo = mid; else hi = mid;
+      }
+      const double t = (lk - lnk[lo]) / (lnk[hi] - lnk[lo]);
+      lp = lnp[lo] + t * (lnp[hi] - lnp[lo]);
+    }
+    // neutrino_ic.f90:88-98 and 150-160, in real(4)
+    const float kr = sqrtf((float)(sx * sx) + (float)(sy * sy) + (float)(sz * sz));
+    const float kp = kr * 2.0f * PI_F / box;
+    const float ll = kp / kfs;
+    const float q = 1.0f + ll + ll * ll;
+    const float tf = field == 0 ? 1.0f / q : (1.0f + 1.5f * ll + 2.0f * ll * ll) / (q * q);
+    const float2 c = C[i];
+    const double a = sqrt(exp(lp)) * scale * (double)tf;
+    if (field == 0) {
+      out = float2{(float)((double)c.x * a), (float)((double)c.y * a)};
+    } else {
+      const double s = (kf * (double)se) / k2 * a;  // i k_e/k^2
+      out = float2{(float)(-(double)c.y * s), (float)((double)c.x * s)};
+    }
+  }
+  W[i] = out;
+}
+
+// block average of the lattice field (row pitch ng+2) onto fluid field q:
+// q = 0 stores <1 + delta>, q = 1..3 stores <v>; sums in float, x fastest
+__global__ void k_nu_ic_block(const float* R, int64_t ng, int b, int q, float* F, int nc) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n3 = (int64_t)nc * nc * nc;
+  if (i >= n3) return;
+  const int64_t x = i % nc, y = (i / nc) % nc, z = i / ((int64_t)nc * nc);
+  float s = 0.f;
+  for (int c = 0; c < b; c++)
+    for (int bb = 0; bb < b; bb++)
+      for (int a = 0; a < b; a++) {
+        const float v = R[((z * b + c) * ng + (y * b + bb)) * (ng + 2) + x * b + a];
+        s += q == 0 ? 1.0f + v : v;
+      }
+  F[(int64_t)q * n3 + i] = (float)((double)s / ((double)b * b * b));
+}
+
+// momenta and energy from rho and <v> (neutrino_ic.f90:110-113, 170-177)
+__global__ void k_nu_ic_finish(float* F, int nc, float cs2, float g) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n3 = (int64_t)nc * nc * nc;
+  if (i >= n3) return;
+  const float rho = F[i];
+  float e = rho * cs2 / (g - 1.0f);
+  for (int d = 1; d <= 3; d++) {
+    const float m = rho * F[(int64_t)d * n3 + i];
+    F[(int64_t)d * n3 + i] = m;
+    e = e + 0.5f * (m * m) / rho;
+  }
+  F[4 * n3 + i] = e;
+}
+
+extern "C" int cube_nu_ic(cube_handle* h, const cube_ic_params* ip) {
+  if (!h || !ip || !ip->lnk || !ip->lnp || ip->nk < 2 || ip->np_nc < 1) return CUBE_ERR_ARG;
+  if (!h->nu.on) return cube_fail(h, CUBE_ERR_ARG, "cube_nu_ic needs cube_nu_init first");
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  NuState& s = h->nu;
+  const int64_t ng = h->nc * ip->np_nc;
+  if (ng > 2048 || (ng & (ng - 1))) return cube_fail(h, CUBE_ERR_ARG, "lattice %lld must be a power of two <= 2048", (long long)ng);
+  for (int n = 0; n < 3; n++)
+    if (s.nc[n] > ng || ng % s.nc[n])
+      return cube_fail(h, CUBE_ERR_ARG, "fluid %d of %d cells is finer than the %lld^3 particle lattice", n, s.nc[n],
+                       (long long)ng);
+  const cube_nu_params& p = s.par;
+  Mesh m;
+  int r = mesh_alloc(h, m, ng, 4, false, false);
+  if (r) return r;
+  double *dlnk, *dlnp;
+  HIP_TRY(h, hipMalloc(&dlnk, sizeof(double) * ip->nk));
+  HIP_TRY(h, hipMalloc(&dlnp, sizeof(double) * ip->nk));
+  HIP_TRY(h, hipMemcpy(dlnk, ip->lnk, sizeof(double) * ip->nk, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(dlnp, ip->lnp, sizeof(double) * ip->nk, hipMemcpyHostToDevice));
+  const int64_t n = ng * ng * ng;
+  k_ic_noise<<<(unsigned)((n + IC_NT - 1) / IC_NT), IC_NT, 0, h->stream>>>((float*)m.R, ng, splitmix64(ip->seed));
+  if ((r = mesh_fft3_r2c(h, m))) return r;
+  const double box = ip->box, kf = 2.0 * M_PI / box;
+  const double sdelta = sqrt((double)n / (box * box * box)) / (double)n;
+  const double svel = sdelta * ((double)h->nf / box) * ip->vfac;
+  // neutrino_ic.f90:12 (real(4) parameter) and :79-82
+  const float kfs1 = sqrtf(1.5f * p.omega_m / (1.0f + p.z_i_nu)) * 100.0f;
+  const float beta = 5966.34f;   // dfld_beta_eV (hydrodf.f90:9)
+  const float sol = 2.998e5f;    // dfld_sol (hydrodf.f90:11)
+  static const float gl_v[3] = {0.41577456f, 2.29428036f, 6.28994508f};
+  const int64_t H = ng / 2 + 1;
+  for (int fl = 0; fl < 3; fl++) {
+    const float vneu = gl_v[fl] / beta * sol / p.m_ev;
+    const float kfs = kfs1 / vneu;
+    const int nc = s.nc[fl];
+    const int b = (int)(ng / nc);
+    const int64_t n3 = (int64_t)nc * nc * nc;
+    for (int field = 0; field < 4; field++) {
+      k_nu_ic_mult<<<(unsigned)((H * ng * ng + IC_NT - 1) / IC_NT), IC_NT, 0, h->stream>>>(
+          (const float2*)m.R, (float2*)m.W, ng, field, kf, field == 0 ? sdelta : svel, dlnk, dlnp, ip->nk, kfs,
+          (float)box);
+      if ((r = mesh_pass(h, m, m.W, 2, true))) return r;
+      if ((r = mesh_pass(h, m, m.W, 1, true))) return r;
+      if ((r = mesh_c2r(h, m, m.W, m.W, ng + 2))) return r;
+      k_nu_ic_block<<<(unsigned)((n3 + IC_NT - 1) / IC_NT), IC_NT, 0, h->stream>>>((const float*)m.W, ng, b, field,
+                                                                                 s.F[fl], nc);
+    }
+    k_nu_ic_finish<<<(unsigned)((n3 + IC_NT - 1) / IC_NT), IC_NT, 0, h->stream>>>(s.F[fl], nc, s.cs2[fl], p.gamma);
+  }
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(dlnk);
+  (void)hipFree(dlnp);
+  mesh_free(m);
   return CUBE_OK;
 }

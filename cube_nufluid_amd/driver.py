@@ -10,7 +10,7 @@ import time
 import numpy as np
 
 from . import _lib
-from .checkpoint import BoxState, write_checkpoint
+from .checkpoint import BoxState, output_name, write_checkpoint
 from .params import NCELL, read_kernel_tables
 
 f32 = np.float32
@@ -69,12 +69,48 @@ class Cube:
         ip.seed, ip.np_nc, ip.nk, ip.a, ip.box = seed, cfg.np_nc, len(lnk), a, cfg.box
         ip.vfac = velocity_factor(cfg, a)
         ip.lnk, ip.lnp = _lib.ptr(lnk), _lib.ptr(lnp)
+        ip.f_cdm = cfg.f_cdm
         sim = _lib.SimHeader()
         _lib.check(self.lib, h, self.lib.cube_ic_zeldovich(h, ctypes.byref(ip), ctypes.byref(sim)))
         sim.h0, sim.omega_m, sim.omega_l, sim.s8 = cfg.h0, cfg.omega_m, cfg.omega_l, cfg.sigma8
         sim.vsim2phys = (150.0 / a) * cfg.box * np.sqrt(cfg.omega_m) / cfg.nf
+        sim.z_i_nu = cfg.z_i_nu
         self.sim = sim
         self._init_time(np.float32(a), z_checkpoints or [z, 0.0], z_halofind)
+        if cfg.m_nu_ev > 0:
+            # the neutrino IC tool's fluids from the same linear field
+            # (neutrino_ic.f90), read at start as main.f90:25-32 does
+            self.nu_init(m_ev=cfg.m_nu_ev, f_neu=cfg.f_neu, z_i_nu=cfg.z_i_nu)
+            _lib.check(self.lib, h, self.lib.cube_nu_ic(h, ctypes.byref(ip)))
+            self.nu_seed_dt()
+        return self
+
+    @classmethod
+    def from_checkpoint(cls, cfg, opath, z_checkpoints, device=0, np_max=None, z_halofind=None, exact=False):
+        """Resume from the checkpoint at z_checkpoints[0] (initialize.f90:
+        40-61 with the list edited to start there, particle_initialization.f90:
+        34-82, and for a neutrino run the fluids from the z-prefixed neu1
+        file of that checkpoint, read as main.f90:25-32 reads its neu file).
+
+        As in the reference, a = 1/(1+z) is recomputed from the list, which
+        differs in its last bits from the a the run reached when it wrote
+        the checkpoint.  exact=True takes a, t and tau from the header
+        instead, so that a resumed CDM run repeats the uninterrupted one bit
+        for bit.  dt_neu restarts at dt_max either way, as in the reference."""
+        import os
+        from .checkpoint import read_checkpoint
+        z0 = z_checkpoints[0]
+        st = read_checkpoint(opath, z0, cfg.nc, izipx=cfg.izipx, izipv=cfg.izipv)
+        self = cls(cfg, st, device=device, np_max=np_max, z_checkpoints=z_checkpoints, z_halofind=z_halofind)
+        if exact:
+            self.a, self.t, self.tau = f32(st.sim.a), f32(st.sim.t), f32(st.sim.tau)
+            self.a_mid = self.a
+        if cfg.m_nu_ev > 0:
+            fn = output_name(opath, z0, "neu1")
+            if not os.path.exists(fn):
+                raise FileNotFoundError("neutrino fluid checkpoint %s missing" % fn)
+            self.nu_restart(fn)
+            self.nu_seed_dt()
         return self
 
     def _init_time(self, a0, z_checkpoints, z_halofind=None):
@@ -165,6 +201,39 @@ class Cube:
     def nu_set(self, n, f):
         f = np.ascontiguousarray(f, dtype=np.float32)
         _lib.check(self.lib, self.h, self.lib.cube_nu_set_fluid(self.h, n, _lib.ptr(f)))
+
+    def nu_checkpoint(self, fn):
+        """dfld_checkpoint (hydrodf.f90:128-146) of the resident fluids."""
+        import os
+        from .checkpoint import write_neu_checkpoint
+        os.makedirs(os.path.dirname(fn) or ".", exist_ok=True)
+        info = [self.nu_info(n) for n in range(3)]
+        beta = f32(f32(5966.34) * f32(self.nu_params.m_ev))     # d%beta, written as beta/dfld_beta_eV
+        write_neu_checkpoint(fn, f32(beta / f32(5966.34)), self.nu_params.gamma, True, [i["weight"] for i in info],
+                             [self.nu_get(n) for n in range(3)], [i["dx"] for i in info], ncb=self.cfg.ncb)
+
+    def nu_restart(self, fn, f_neu=None, z_i_nu=None):
+        """dfld_restart (hydrodf.f90:158-185): mass, gamma and fluids from a
+        neu file; the weights are recomputed by the setup, as there."""
+        from .checkpoint import read_neu_checkpoint
+        with open(fn, "rb") as f:
+            m_ev, gamma = np.fromfile(f, dtype=np.float32, count=2)
+        self.nu_init(m_ev=float(m_ev), f_neu=f_neu if f_neu is not None else self.cfg.f_neu,
+                     z_i_nu=z_i_nu if z_i_nu is not None else self.cfg.z_i_nu, gamma=float(gamma))
+        info = [self.nu_info(n) for n in range(3)]
+        _, _, _, _, fluids = read_neu_checkpoint(fn, [i["nc"] for i in info], [i["dx"] for i in info], ncb=self.cfg.ncb)
+        for n in range(3):
+            self.nu_set(n, fluids[n])
+
+    def nu_seed_dt(self):
+        """dt_neu of the fluids as loaded.  The reference starts dt_neu at
+        dt_max = 1 (initialize.f90:40); with the fluids live from the first
+        step (NEUTRINO_IC, fluid and CDM both starting at z_i_nu) that first
+        step can exceed the fast fluid's CFL limit many times over, so the
+        CFL step of the loaded fluids is taken instead (a zero-length
+        evolve leaves the fields unchanged and returns it)."""
+        self.dt_neu = DT_MAX
+        self.fluid_evolve(f32(0.0), 1, self.a)
 
     def fluid_evolve(self, dt, direction, a):
         dtn = ctypes.c_float(float(self.dt_neu))
@@ -272,7 +341,10 @@ class Cube:
                     st = self.download()
                     st.sim.cur_checkpoint = self.cur_checkpoint
                     st.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
-                    write_checkpoint(st, opath, self.z_checkpoints[self.cur_checkpoint - 1])
+                    zc = self.z_checkpoints[self.cur_checkpoint - 1]
+                    if nu:  # main.f90:75-81, before the particle files
+                        self.nu_checkpoint(output_name(opath, zc, "neu1"))
+                    write_checkpoint(st, opath, zc)
                 self.cur_checkpoint += 1
             if self.halofind_step:
                 self._halofind_step(opath)

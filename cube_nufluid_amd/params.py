@@ -35,6 +35,10 @@ class BoxConfig:
     wde: float = -1.0       # (:100)
     z_i: float = 49.0
     np_max_factor: float = 1.0
+    # neutrino fluid (parameters.f90:73-94): mass of each of the dneu = 3
+    # degenerate species in eV; 0 runs CDM alone
+    m_nu_ev: float = 0.0
+    z_i_nu: float = 10.0
 
     @property
     def nt(self):
@@ -51,6 +55,18 @@ class BoxConfig:
         return 1.0 - self.omega_m - omega_r
 
     @property
+    def f_neu(self):
+        """omega_nu/omega_m with omega_nu = dneu*Mneu/93.14/h0^2, dneu = 3
+        (parameters.f90:81-93); omega_m is the total matter density."""
+        return 3.0 * self.m_nu_ev / 93.14 / self.h0 ** 2 / self.omega_m
+
+    @property
+    def f_cdm(self):
+        """omega_cdm/omega_m (parameters.f90:94): the matter that is not in
+        the neutrino fluid (no gas)."""
+        return 1.0 - self.f_neu
+
+    @property
     def npart(self):
         return (self.nc * self.np_nc) ** 3
 
@@ -61,7 +77,7 @@ class BoxConfig:
     def mass_p(self, npglobal=None):
         # util/initial_conditions.f90:943: f_cdm*nf_global**3/npglobal (no neutrinos: f_cdm=1)
         n = self.npart if npglobal is None else npglobal
-        return np.float32(float(self.nf) ** 3 / n)
+        return np.float32(np.float32(self.f_cdm) * np.float32(float(self.nf) ** 3) / np.float32(n))
 
 
 # Workloads of the benchmark configurations (seeded, EH no-wiggle LCDM).
@@ -70,6 +86,10 @@ CONFIGS = {
     "C1pp": BoxConfig(nc=32, nnt=2, ncb=6, box=100.0, pp_on=True, pp_range=1),
     "C2": BoxConfig(nc=128, nnt=4, ncb=6, izipx=1, izipv=2, box=400.0, pp_on=True, pp_range=1),
     "C3": BoxConfig(nc=512, nnt=4, ncb=6, izipx=2, izipv=2, box=1000.0, pp_on=True, pp_range=1),
+    # CDM + neutrino fluid: sum m_nu = 0.1 eV as three degenerate species,
+    # fluids on 512^3, 256^3 and 128^3 cells, started at z = 10
+    "C4": BoxConfig(nc=256, nnt=4, ncb=6, izipx=2, izipv=2, box=500.0, pp_on=True, pp_range=1, z_i=10.0,
+                    m_nu_ev=0.1 / 3, z_i_nu=10.0),
 }
 
 

@@ -228,6 +228,9 @@ typedef struct cube_ic_params {
   double vfac;        /* velocity per fine-cell displacement, a^2 H(a) f(a)   */
   const double* lnk;  /* ln k, k in h/Mpc, increasing                         */
   const double* lnp;  /* ln P(k), (Mpc/h)^3, at scale factor a                */
+  float f_cdm;        /* omega_cdm/omega_m: mass_p_cdm = f_cdm nf^3/np
+                         (initial_conditions.f90:943); 0 reads as 1          */
+  int32_t reserved;
 } cube_ic_params;
 int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_sim_header* sim_out);
 
@@ -248,6 +251,14 @@ typedef struct cube_nu_params {
    then on cube_pm adds the fluid density to both PM meshes and the PM
    forces to the fluid (pm.f90:55-74, 157-171, 198-242, 374-422). */
 int cube_nu_init(cube_handle* h, const cube_nu_params* p);
+/* Replaces the neutrino IC tool (utilities/neutrino_ic.f90:40-181, whose
+   output main.f90:25-32 reads through dfld_restart): fluid fields from the
+   same seeded linear field as cube_ic_zeldovich with the same ip, filtered
+   per fluid by 1/(1+l+l^2) (density) and (1+1.5l+2l^2)/(1+l+l^2)^2
+   (velocity), l = k/k_fs, block-averaged onto each fluid mesh; momenta
+   rho<v>, energy rho cs2/(g-1) + |m|^2/(2 rho).  Needs cube_nu_init; the
+   fluid meshes must not be finer than the particle lattice. */
+int cube_nu_ic(cube_handle* h, const cube_ic_params* ip);
 /* Geometry of fluid n (0..2): cells per dimension, cell size in fine cells,
    cs^2 in simulation units, density weight. */
 int cube_nu_info(cube_handle* h, int n, int32_t* nc, float* dx, float* cs2, float* weight);

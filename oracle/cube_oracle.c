@@ -950,14 +950,17 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
   if (!probe) o->sigma_vi = o->sigma_vi_new; /* pm.f90:180 */
   const float sig_c = probe ? sig_new : o->sigma_vi;
 
-  /* ---- coarse PM, pm.f90:188-372 */
+  /* ---- coarse PM, pm.f90:188-372.  The coarse density always covers the
+     whole box (a tile limit restricts only the kicks below), so the coarse
+     force of a limited sample is the full run's; its time is box-wide
+     work, counted in t_global. */
+  const double t_glob0 = omp_get_wtime();
   const int64_t n3c = nc * nc * nc, hc = nc / 2 + 1, T = nt + 4;
   float* r3 = (float*)calloc(n3c, sizeof(float));
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1) collapse(3)
   for (int64_t tz = 0; tz < nnt; tz++)
     for (int64_t ty = 0; ty < nnt; ty++)
       for (int64_t tx = 0; tx < nnt; tx++) {
-        if (o->tile_limit > 0 && (tz * nnt + ty) * nnt + tx >= o->tile_limit) continue;
         float* r3t = (float*)calloc(T * T * T, sizeof(float)); /* r3t(-1:nt+2)^3 */
 #define RT(i, j, k) r3t[(((k) + 1) * T + ((j) + 1)) * T + ((i) + 1)]
         for (int il = 0; il < 3; il++)
@@ -1020,7 +1023,6 @@ static int pm_impl(orc_t* o, float a_mid, float dt, double* out, double* probe, 
           }
   }
   free(Dc);
-  const double t_glob0 = omp_get_wtime();
   double testrhoc = 0;
   for (int64_t q = 0; q < n3c; q++) testrhoc += (double)r3[q];
   float* cre = (float*)malloc(sizeof(float) * hc * nc * nc);

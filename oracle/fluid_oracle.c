@@ -122,9 +122,13 @@ static void buffer(float* E, int nc, int ncb) {
 /* one sweep along axis ax (0 x, 1 y, 2 z): the normal momentum goes first */
 static void sweep(float* E, int nc, int ncb, int ax, float g, float dt, float dx) {
   const int ne = nc + 2 * ncb;
+  const int perm[3][3] = {{1, 2, 3}, {2, 1, 3}, {3, 2, 1}}; /* reference's h1d(2:4) per axis */
+  /* lines of one sweep are independent: threads take whole lines */
+#pragma omp parallel
+  {
   float* line = (float*)malloc(sizeof(float) * 5 * ne);
   float* w = (float*)malloc(sizeof(float) * 9 * ne);
-  const int perm[3][3] = {{1, 2, 3}, {2, 1, 3}, {3, 2, 1}}; /* reference's h1d(2:4) per axis */
+#pragma omp for collapse(2) schedule(static)
   for (int b = 0; b < ne; b++)
     for (int a = 0; a < ne; a++) {
       for (int t = 0; t < ne; t++) {
@@ -145,6 +149,7 @@ static void sweep(float* E, int nc, int ncb, int ax, float g, float dt, float dx
     }
   free(line);
   free(w);
+  }
 }
 
 /* h3_timestep over x-lines of the extended array */

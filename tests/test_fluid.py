@@ -2,6 +2,9 @@
 exact invariants of the finite-volume scheme, the order-free form of the
 gravity update checked against the sequential one, the 'neu' checkpoint
 format, and the GPU fluid (through the C ABI) against the restatement."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -197,8 +200,11 @@ def test_gpu_pm_with_fluid_matches_oracle():
                  [i["nc"] for i in infos], [neu_ncb(i["dx"]) for i in infos], fl)
     a_mid, dt = np.float32(0.25), np.float32(0.3)
     cube.a_mid, cube.dt = a_mid, dt
-    cube.particle_mesh()
-    orc.pm(float(a_mid), float(dt))
+    gpm = cube.particle_mesh()
+    opm = orc.pm(float(a_mid), float(dt))
+    # testrhof / testrhoc sum the meshes with the fluid in (pm.f90:107,280)
+    assert abs(gpm.mass_f - opm["mass_f"]) <= 1e-6 * opm["mass_f"]
+    assert abs(gpm.mass_c - opm["mass_c"]) <= 1e-6 * opm["mass_c"]
     # The two sides' forces agree to FP32 FFT rounding (about 1e-5 relative
     # RMS), so the momentum increments agree to that order; an index, pitch
     # or sign error in the coupling shows up at order 1.  The oracle adds the
@@ -232,3 +238,54 @@ def test_gpu_pm_with_fluid_matches_oracle():
     cube.close()
 
 
+
+
+def _nu_ic_restatement():
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import neutrino_ic
+    return neutrino_ic
+
+
+def test_neutrino_ic_filters_closed_form():
+    """neutrino_ic.f90:97,159: both transfer functions are 1 at l = 0, fall
+    as 1/l^2 (density) and 2/l^2 (velocity) for l >> 1, and take the exact
+    values 1/3 and 4.5/9 = 1/2 at l = 1."""
+    nu = _nu_ic_restatement()
+    assert nu.tf_density(0.0) == 1.0 and nu.tf_velocity(0.0) == 1.0
+    assert nu.tf_density(1.0) == pytest.approx(1 / 3, rel=1e-7)
+    assert nu.tf_velocity(1.0) == pytest.approx(4.5 / 9, rel=1e-7)
+    assert nu.tf_density(1e3) * 1e6 == pytest.approx(1.0, rel=2e-3)
+    assert nu.tf_velocity(1e3) * 1e6 == pytest.approx(2.0, rel=2e-3)
+
+
+def test_neutrino_ic_plane_wave_and_mean():
+    """The restated fluid IC of one seeded field: the mean density of every
+    fluid is 1 (the filters keep k = 0 out), the energy is rho cs2/(g-1)
+    plus the kinetic part, and on the fundamental modes each fluid's density
+    contrast is the CDM particles' contrast times the density filter at
+    l = k/kfs_n (for the fastest fluid l ~ 6 there, a 40-fold suppression)."""
+    from cube_nufluid_amd.params import BoxConfig
+    from cube_nufluid_amd.ic import zeldovich_ic
+    nu = _nu_ic_restatement()
+    import power
+    cfg = BoxConfig(nc=16, nnt=1, ncb=4, box=500.0, m_nu_ev=0.1 / 3, z_i=10.0)
+    fl = nu.fluid_ic(cfg, seed=9, z=10.0, m_ev=cfg.m_nu_ev, z_i_nu=10.0)
+    setup = nu.fluid_setup(cfg, cfg.m_nu_ev)
+    st = zeldovich_ic(cfg, seed=9, z=10.0)
+    kfs1 = np.sqrt(1.5 * cfg.omega_m / 11.0) * 100.0
+    for n, (f, (nc, dx, cs2)) in enumerate(zip(fl, setup)):
+        assert f.shape == (5, nc, nc, nc)
+        assert abs(f[0].mean(dtype=np.float64) - 1) < 1e-6
+        kin = 0.5 * (f[1].astype(np.float64) ** 2 + f[2] ** 2 + f[3] ** 2) / f[0]
+        np.testing.assert_allclose(f[4], f[0] * cs2 / (5 / 3 - 1) + kin, rtol=1e-5)
+        pos = power.positions_fine(st, cfg.nc, cfg.nt, cfg.nnt, cfg.izipx) * nc / cfg.nf
+        rho_p = power.cic_density(pos, nc)
+        a = np.fft.rfftn(rho_p / rho_p.mean() - 1)
+        b = np.fft.rfftn(f[0].astype(np.float64) - 1)
+        vneu = nu.GL_V[n] / (5966.34 * cfg.m_nu_ev) * 2.998e5
+        want = float(nu.tf_density((2 * np.pi / cfg.box) / (kfs1 / vneu)))
+        # the CIC nodes sit on cell edges and the fluid cells are centred half
+        # a cell further on, a pure phase: compare magnitudes.  The CIC and
+        # block-average windows differ by up to 5% on an 8-cell mesh.
+        for idx in ((0, 0, 1), (0, 1, 0), (1, 0, 0)):
+            assert abs(abs(b[idx] / a[idx]) / want - 1) < 0.06, (n, idx, b[idx] / a[idx], want)
