@@ -389,11 +389,35 @@ def c4_bench(steps, warmup, seed=1004):
         cube.close()
 
 
+def ensemble_small(n, rate):
+    """The same serving comparison on boxes of {32,64,128}^3 particles only:
+    boxes whose steps leave the GPU idle when run alone."""
+    from cube_nufluid_amd.serving import poisson_requests, run_ensemble
+    reqs = poisson_requests(n, rate, seed=7, sizes=(32, 64, 128))
+    mix = run_ensemble(reqs, max_live=64)
+    plain = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1)
+    key = lambda r: (r["npd"], r["seed"])
+    same = {key(r): r["hash"] for r in mix["results"]} == {key(r): r["hash"] for r in plain["results"]}
+    log("ensemble, small boxes only: %.3g vs %.3g particle-steps/s one at a time (%.2fx)"
+        % (mix["particle_steps_per_s"], plain["particle_steps_per_s"],
+           mix["particle_steps_per_s"] / plain["particle_steps_per_s"]))
+    return {"workload": "%d seeded LCDM PM+PP boxes, {32,64,128}^3 particles, z=49 to 0, Poisson arrivals %.1f/s"
+                        % (n, rate),
+            "p50_s": round(mix["p50"], 4), "p99_s": round(mix["p99"], 4),
+            "particle_steps_per_s": mix["particle_steps_per_s"],
+            "plain_particle_steps_per_s": plain["particle_steps_per_s"],
+            "speedup_vs_one_at_a_time": mix["particle_steps_per_s"] / plain["particle_steps_per_s"],
+            "same_result_alone_and_in_mix": bool(same)}
+
+
 def ensemble_bench(n, rate):
     """Ensemble serving: n seeded LCDM PM+PP boxes drawn uniformly from
     {32,64,128,256}^3 particles (mesh 2x particles per dimension), Zel'dovich
     start at z=49, run to z=0, arriving as a Poisson process; served
-    concurrently by the native engine, then the same boxes one at a time."""
+    concurrently by the native engine, then the same boxes one at a time.
+    A box takes the steps CUBE's time-step limits give it from z=49 to 0
+    (the ra <= 0.2 expansion limit alone needs about 25 steps; the force
+    and velocity limits add the rest), not a fixed 50."""
     from cube_nufluid_amd.serving import poisson_requests, run_ensemble
     reqs = poisson_requests(n, rate, seed=5)
     log("ensemble: %d requests, sizes %s, Poisson rate %.1f/s" % (n, sorted(set(r["npd"] for r in reqs)), rate))
@@ -420,6 +444,8 @@ def ensemble_bench(n, rate):
             "plain_seconds_by_size": {str(k): round(v, 3) for k, v in sorted(by_size.items())},
             "plain_time_over_largest_boxes_time": plain["span"] / by_size[big],
             "steps": sorted(set(r["steps"] for r in mix["results"])),
+            "steps_by_size": {str(k): sorted(set(r["steps"] for r in mix["results"] if r["npd"] == k))
+                              for k in sorted(set(r["npd"] for r in mix["results"]))},
             "same_result_alone_and_in_mix": bool(same)}
 
 
@@ -431,19 +457,23 @@ def main():
     ap.add_argument("--config", default=os.environ.get("CUBE_BENCH_CONFIG", "C3"))
     ap.add_argument("--z", type=float, default=3.0,
                     help="redshift of the Zel'dovich start state, evolved to z=0 before the timed steps")
-    ap.add_argument("--evolve-steps", type=int, default=400,
+    ap.add_argument("--evolve-steps", type=int, default=200,
                     help="cap on the CUBE steps that evolve the start state to z=0 (0: time the Zel'dovich state)")
-    ap.add_argument("--evolve-seconds", type=float, default=240.0, help="wall-clock cap on that evolution")
+    ap.add_argument("--evolve-seconds", type=float, default=150.0, help="wall-clock cap on that evolution")
     ap.add_argument("--seed", type=int, default=1003)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble-serving measurement")
     ap.add_argument("--no-halofind", action="store_true", help="skip the halo-finder measurement")
-    ap.add_argument("--no-pp-unit", action="store_true", help="skip the PP unit benchmark on NFW clumps")
+    ap.add_argument("--pp-unit", action="store_true", help="also run the PP unit benchmark on NFW clumps (C2 size)")
+    ap.add_argument("--halofind-evolved", action="store_true",
+                    help="also time the halo finder on a C2 box evolved z=49 -> 0, with the C restatement beside it")
     ap.add_argument("--no-c4", action="store_true", help="skip the CDM + neutrino-fluid (C4) measurement")
     ap.add_argument("--c4-only", action="store_true", help="only the C4 measurement (for profiling)")
     ap.add_argument("--halofind-only", action="store_true",
                     help="only generate the state and run the halo finder on it once (for profiling)")
-    ap.add_argument("--ens-requests", type=int, default=16)
+    ap.add_argument("--ens-requests", type=int, default=48)
+    ap.add_argument("--ens-small-requests", type=int, default=24,
+                    help="requests of the small-box ensemble ({32,64,128}^3 particles); 0 skips it")
     ap.add_argument("--ens-rate", type=float, default=6.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--copy-calib", action="store_true", help="only run the copy kernel used to calibrate PMC bytes")
     ap.add_argument("--cpu-tiles", type=int, default=0,
@@ -574,13 +604,16 @@ def main():
     last_dt = float(cube.dt) or 0.05
     cube.close()
     if not args.no_halofind:
-        out["halofind"]["evolved"] = halofind_evolved("C2", 1002, cpu=not args.no_cpu_baseline)
-    if not args.no_pp_unit:
+        if args.halofind_evolved:
+            out["halofind"]["evolved"] = halofind_evolved("C2", 1002, cpu=not args.no_cpu_baseline)
+    if args.pp_unit:
         out["pp_unit"] = pp_unit_bench("C2", 1002, cpu=not args.no_cpu_baseline)
     if not args.no_c4:
         out["c4"] = c4_bench(min(args.steps, 5), 1)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ens_requests, args.ens_rate)
+        if args.ens_small_requests > 0:
+            out["ensemble"]["small_boxes_only"] = ensemble_small(args.ens_small_requests, args.ens_rate * 4)
     if st_cpu is not None:
         threads, hinfo = host_threads()
         per_tile = npart / cfg.nnt ** 3

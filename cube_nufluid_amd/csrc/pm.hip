@@ -10,7 +10,6 @@
 #include "cube_internal.hpp"
 
 #define CIC_NT 256
-#define KICK_NT 256
 
 // --------------------------------------------------------------- helpers
 // Load cstart for the cells of a SPAN^3 region around block origin c0 (lo
@@ -287,8 +286,8 @@ struct KickMeshes {
   const T* c;  // [3][nc^3]
 };
 
-template <typename T, int BX>
-__global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
+template <typename T, int BX, int NT>
+__global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                                   int64_t n3c, const float* __restrict__ vfield, KickMeshes<T> F, int64_t Nf,
                                                   float a_mid, float dt, double c_old, double c_new,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
@@ -301,13 +300,13 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
   __shared__ T sF[NFS];
   __shared__ T sC[NCS];
   __shared__ int64_t cs[BX + 1];
-  __shared__ float red[4][KICK_NT / 64];
+  __shared__ float red[4][NT / 64];
   const int64_t r0 = (int64_t)blockIdx.x * BX;
   int32_t c0[3];
   cell_of_rank(g, r0, c0);
   const int tid = threadIdx.x;
   const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
-  for (int i = tid; i <= BX; i += KICK_NT) cs[i] = cstart[r0 + i];
+  for (int i = tid; i <= BX; i += NT) cs[i] = cstart[r0 + i];
   const int64_t p0 = cstart[r0], p1 = cstart[r0 + BX];
   // this thread's first particle: its position and velocity codes are
   // requested before the slabs and its old velocities decoded while they
@@ -322,52 +321,45 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
 #pragma unroll
     for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
   }
-  // stage the fine slab (75 rows of SX nodes, a wavefront per row and a
-  // lane per node) and the coarse slab (27 rows of CX nodes), all of a
-  // lane's loads issued before its LDS stores
+  // stage the fine slab (75 rows of SX nodes) and the coarse slab (27 rows
+  // of CX nodes), dealt to the threads in row-major order so consecutive
+  // lanes read consecutive nodes of a row; all of a thread's loads are
+  // issued before its LDS stores
   {
-    constexpr int NW = KICK_NT / 64, RPW = (75 + NW - 1) / NW, XC = (SX + 63) / 64, RPC = (27 + NW - 1) / NW;
-    const int lane = tid & 63, wv = tid >> 6;
-    T v[RPW][XC], vc[RPC];
+    constexpr int KF = (75 * SX + NT - 1) / NT, KC = (27 * CX + NT - 1) / NT;
+    T v[KF], vc[KC];
 #pragma unroll
-    for (int k = 0; k < RPW; k++) {
-      const int row = wv + k * NW;  // row = (d*5 + z)*5 + y
+    for (int k = 0; k < KF; k++) {
+      const int e = tid + k * NT;
+      const int row = e / SX, x = e - row * SX;  // row = (d*5 + z)*5 + y
       const int y = row % 5, z = (row / 5) % 5, d = row / 25;
       const T* f = d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]);
       const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
       const uint32_t gy = (uint32_t)(4 * c0[1] + y) & mf, gz = (uint32_t)(4 * c0[2] + z) & mf;
-      const T* rowp = f + (int64_t)(gz * (uint32_t)Nf + gy) * P;
-#pragma unroll
-      for (int j = 0; j < XC; j++) {
-        const int x = lane + 64 * j;
-        v[k][j] = (row < 75 && x < SX) ? rowp[(uint32_t)(4 * c0[0] + x) & mf] : (T)0;
-      }
+      v[k] = e < 75 * SX ? f[(int64_t)(gz * (uint32_t)Nf + gy) * P + ((uint32_t)(4 * c0[0] + x) & mf)] : (T)0;
     }
 #pragma unroll
-    for (int k = 0; k < RPC; k++) {
-      const int row = wv + k * NW;  // row = (d*3 + z)*3 + y
+    for (int k = 0; k < KC; k++) {
+      const int e = tid + k * NT;
+      const int row = e / CX, xl = e - row * CX;  // row = (d*3 + z)*3 + y
       const int y = row % 3, z = (row / 3) % 3, d = row / 9;
-      const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
+      const uint32_t gx = (uint32_t)(c0[0] - 1 + xl) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
                      gz = (uint32_t)(c0[2] - 1 + z) & mc;
-      vc[k] = (row < 27 && lane < CX) ? F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx] : (T)0;
+      vc[k] = e < 27 * CX ? F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx] : (T)0;
     }
     if (pa < p1) {
 #pragma unroll
       for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_old, va[e], g.izipv);
     }
 #pragma unroll
-    for (int k = 0; k < RPW; k++) {
-      const int row = wv + k * NW;
-#pragma unroll
-      for (int j = 0; j < XC; j++) {
-        const int x = lane + 64 * j;
-        if (row < 75 && x < SX) sF[row * SX + x] = v[k][j];
-      }
+    for (int k = 0; k < KF; k++) {
+      const int e = tid + k * NT;
+      if (e < 75 * SX) sF[e] = v[k];
     }
 #pragma unroll
-    for (int k = 0; k < RPC; k++) {
-      const int row = wv + k * NW;
-      if (row < 27 && lane < CX) sC[row * CX + lane] = vc[k];
+    for (int k = 0; k < KC; k++) {
+      const int e = tid + k * NT;
+      if (e < 27 * CX) sC[e] = vc[k];
     }
   }
   __syncthreads();
@@ -375,7 +367,7 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
   // far faces; the workgroups' owned nodes tile the mesh), summed in float
   // in component order: the dt_fine input of pm.f90
   float f2loc = 0.f;
-  for (int i = tid; i < 4 * BX * 16; i += KICK_NT) {
+  for (int i = tid; i < 4 * BX * 16; i += NT) {
     const int x = i % (4 * BX), y = (i / (4 * BX)) & 3, z = i / (16 * BX);
     const int o = (z * 5 + y) * SX + x;
     const float a = (float)sF[o], b = (float)sF[FROW + o], c = (float)sF[2 * FROW + o];
@@ -386,7 +378,7 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
   }
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax[3] = {0.f, 0.f, 0.f};
-  for (int64_t p = pa; p < p1; p += KICK_NT) {
+  for (int64_t p = pa; p < p1; p += NT) {
     // this particle's codes and old velocities; the next particle's codes
     // are requested now and decoded at the end of this one
     uint32_t xu[3];
@@ -396,7 +388,7 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       xu[e] = xa[e];
       vo[e] = vo_a[e];
     }
-    const int64_t pn = p + KICK_NT;
+    const int64_t pn = p + NT;
     if (pn < p1) {
 #pragma unroll
       for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pn + e, g.izipx);
@@ -446,13 +438,39 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
       Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
-    T Fk[3][8], Ck[3][8];
+    if (probe_f) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        double sfv = 0, scv = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          T Fv;
+          if (in_slab) {
+            Fv = sF[d * FROW + sf[q]];
+          } else {
+            const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+            const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
+            Fv = F.f[d][(int64_t)rowf * F.pf[d] + (ox ? f2[0] : f1[0])];
+          }
+          sfv += (double)(Fv * Fw[q]);
+          scv += (double)(sC[d * CROW + sc[q]] * Cw[q]);
+        }
+        probe_f[3 * p + d] = sfv;
+        probe_c[3 * p + d] = scv;
+      }
+      continue;
+    }
+    // fine kick: each tap's F*k*w in the mesh precision added to the old
+    // velocity in tap order, straight from the slab (no per-tap arrays
+    // stay live through the codec stages)
+    double v[3], x[3], tl[3], th[3], vn[3];
+    int32_t n[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      T Fv[8], Cv[8];
+      v[d] = vo[d];
       if (in_slab) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) Fv[q] = sF[d * FROW + sf[q]];
+        for (int q = 0; q < 8; q++) v[d] = v[d] + (double)(sF[d * FROW + sf[q]] * kf * Fw[q]);
       } else {
         const T* Ff = F.f[d];
         const int64_t P = F.pf[d];
@@ -460,44 +478,15 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
         for (int q = 0; q < 8; q++) {
           const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
           const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
-          Fv[q] = Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])];
+          v[d] = v[d] + (double)(Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])] * kf * Fw[q]);
         }
       }
-#pragma unroll
-      for (int q = 0; q < 8; q++) Cv[q] = sC[d * CROW + sc[q]];
-      if (probe_f) {
-        double sfv = 0, scv = 0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          sfv += (double)(Fv[q] * Fw[q]);
-          scv += (double)(Cv[q] * Cw[q]);
-        }
-        probe_f[3 * p + d] = sfv;
-        probe_c[3 * p + d] = scv;
-        continue;
-      }
-      // the per-tap terms (F*k*w in the mesh precision), added below in tap order
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        Fk[d][q] = Fv[q] * kf * Fw[q];
-        Ck[d][q] = Cv[q] * kf * Cw[q];
-      }
-    }
-    if (probe_f) continue;
-    // codec stages for the three components together: decode old; fine kick;
-    // encode (guess, thresholds, finish); decode new; coarse kick; encode
-    double v[3], x[3], tl[3], th[3], vn[3];
-    int32_t n[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      v[d] = vo[d];
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[d] = v[d] + (double)Fk[d][q];
+      // codec: encode (guess, thresholds, finish) with the new sigma, and
+      // the guess's decoded value requested with its thresholds
       x[d] = c_new * v[d];
       n[d] = venc_guess(kmax, g.izipv, x[d]);
       tl[d] = venc_lo(venc, kmax, n[d]);
       th[d] = venc_hi(venc, kmax, n[d]);
-      // the guess's decoded value, requested with its thresholds
       vn[d] = vdecode(vdt_new, n[d], g.izipv);
     }
 #pragma unroll
@@ -505,10 +494,11 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
       const int32_t m = venc_finish(venc, kmax, x[d], n[d], tl[d], th[d]);
       v[d] = m == n[d] ? vn[d] : vdecode(vdt_new, m, g.izipv);
     }
+    // coarse kick from the coarse slab, then encode
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) v[d] = v[d] + (double)Ck[d][q];
+      for (int q = 0; q < 8; q++) v[d] = v[d] + (double)(sC[d * CROW + sc[q]] * kf * Cw[q]);
       const float av = (float)fabs(v[d] + (double)vf[d]);
       vmax[d] = fmaxf(vmax[d], av);
       x[d] = c_new * v[d];
@@ -534,14 +524,14 @@ __global__ __launch_bounds__(KICK_NT) void k_kick(Geo g, const void* __restrict_
     __syncthreads();
     if (tid < 4) {
       float m = red[tid][0];
-      for (int i = 1; i < KICK_NT / 64; i++) m = fmaxf(m, red[tid][i]);
+      for (int i = 1; i < NT / 64; i++) m = fmaxf(m, red[tid][i]);
       if (tid < 3 && !probe_f) atomicMax(vmax_bits + tid, __float_as_uint(m));
       if (tid == 3 && f2max_bits) atomicMax(f2max_bits, __float_as_uint(m));
     }
   }
 }
 
-template <typename T, int BX>
+template <typename T, int BX, int NT>
 static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
                         double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
                         double* probe_f, double* probe_c) {
@@ -552,7 +542,7 @@ static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t p
     F.pf[d] = pf[d];
   }
   F.c = (const T*)h->fc;
-  k_kick<T, BX><<<(unsigned)(h->n3c / BX), KICK_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
+  k_kick<T, BX, NT><<<(unsigned)(h->n3c / BX), NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
                                                                     a_mid, dt, c_old, c_new, vo, vn, h->venc, h->vkmax, vmax_bits,
                                                                     f2max_bits, probe_f, probe_c);
 }
@@ -562,11 +552,11 @@ static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3
                      double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
                      double* probe_f, double* probe_c) {
   if (h->nt % 32 == 0)
-    kick_launch<T, 32>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+    kick_launch<T, 32, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   else if (h->nt % 16 == 0)
-    kick_launch<T, 16>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+    kick_launch<T, 16, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   else if (h->nt % 4 == 0)
-    kick_launch<T, 4>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+    kick_launch<T, 4, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
   else
     return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
   return 0;

@@ -26,6 +26,16 @@
 
 struct PPcand { float x, y, z; int32_t fc; };
 
+// The r > rsoft test of ext_pp_force.f90:101-106 is decided as the
+// reference decides it, sqrt(sum(xvec21**2)) > rsoft in double.  The float
+// r2 is within 3 ulps of the exact sum of the squared (exact FP32) offsets,
+// so only pairs that close to rsoft^2 are re-tested in double.
+__device__ __forceinline__ bool pf_outside_rsoft(float r2, float dx, float dy, float dz, float rsoft, float rsoft2) {
+  if (fabsf(r2 - rsoft2) > 6e-7f * rsoft2) return r2 > rsoft2;
+  const double e2 = (double)dx * (double)dx + (double)dy * (double)dy + (double)dz * (double)dz;
+  return sqrt(e2) > (double)rsoft;
+}
+
 __device__ __forceinline__ int32_t pack_fc(int a, int b, int c) {  // each in [-8, 23]
   return ((a + 8) & 31) | (((b + 8) & 31) << 5) | (((c + 8) & 31) << 10);
 }
@@ -138,8 +148,8 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
           npl++;
           float dx = cj.x - xi[0], dy = cj.y - xi[1], dz = cj.z - xi[2];
           float r2 = dx * dx + dy * dy + dz * dz;
+          if (!pf_outside_rsoft(r2, dx, dy, dz, rsoft, rsoft * rsoft)) continue;
           float r = sqrtf(r2);
-          if (!(r > rsoft)) continue;
           float rc = r * (1.0f / 16.0f);
           float rc3 = rc * rc * rc;
           float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
@@ -335,9 +345,9 @@ struct PPacc<double> {
 };
 
 // pair term of ext_pp_force.f90:99-108 for r > rsoft: m pc(r/16) dx/r^3
-__device__ __forceinline__ bool pf_pair(float dx, float dy, float dz, float mass, float rsoft2, float& sc) {
+__device__ __forceinline__ bool pf_pair(float dx, float dy, float dz, float mass, float rsoft, float rsoft2, float& sc) {
   const float r2 = dx * dx + dy * dy + dz * dz;
-  if (!(r2 > rsoft2)) return false;
+  if (!pf_outside_rsoft(r2, dx, dy, dz, rsoft, rsoft2)) return false;
   const float ri = rsqrtf(r2);
   const float r = r2 * ri;
   const float rc = r * (1.0f / 16.0f);
@@ -346,11 +356,12 @@ __device__ __forceinline__ bool pf_pair(float dx, float dy, float dz, float mass
   sc = mass * pc * (ri * ri * ri);
   return true;
 }
-// the same pair term with the r > rsoft test as a select; r2 > rsoft2 > 0
-// keeps the hardware reciprocal square root away from denormal inputs
-__device__ __forceinline__ float pf_pair_sel(float dx, float dy, float dz, float mass, float rsoft2) {
+// the same pair term with the r > rsoft test as a select; an included pair
+// has r2 > rsoft2 > 0 to rounding, which keeps the hardware reciprocal
+// square root away from denormal inputs
+__device__ __forceinline__ float pf_pair_sel(float dx, float dy, float dz, float mass, float rsoft, float rsoft2) {
   const float r2 = dx * dx + dy * dy + dz * dz;
-  const bool ok = r2 > rsoft2;
+  const bool ok = pf_outside_rsoft(r2, dx, dy, dz, rsoft, rsoft2);
   const float ri = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
   const float r = r2 * ri;
   const float rc = r * (1.0f / 16.0f);
@@ -359,19 +370,20 @@ __device__ __forceinline__ float pf_pair_sel(float dx, float dy, float dz, float
   const float sc = mass * pc * (ri * ri * ri);
   return ok ? sc : 0.0f;
 }
-__device__ __forceinline__ bool pf_pair(double dx, double dy, double dz, double mass, double rsoft2, double& sc) {
+__device__ __forceinline__ bool pf_pair(double dx, double dy, double dz, double mass, double rsoft, double rsoft2, double& sc) {
   const double r2 = dx * dx + dy * dy + dz * dz;
-  if (!(r2 > rsoft2)) return false;
+  (void)rsoft2;
   const double r = sqrt(r2);
+  if (!(r > rsoft)) return false;
   const double rc = r * (1.0 / 16.0);
   const double rc3 = rc * rc * rc;
   const double pc = 1.0 - 1.75 * rc3 + 0.75 * rc3 * rc * rc;
   sc = mass * pc / (r2 * r);
   return true;
 }
-__device__ __forceinline__ double pf_pair_sel(double dx, double dy, double dz, double mass, double rsoft2) {
+__device__ __forceinline__ double pf_pair_sel(double dx, double dy, double dz, double mass, double rsoft, double rsoft2) {
   double sc;
-  return pf_pair(dx, dy, dz, mass, rsoft2, sc) ? sc : 0.0;
+  return pf_pair(dx, dy, dz, mass, rsoft, rsoft2, sc) ? sc : 0.0;
 }
 
 // run-start tables: int32 entries (dense pool), or two 16-bit entries per
@@ -427,7 +439,8 @@ template <typename RT, bool ROWSUM, typename Acc, typename EntT, typename RunT>
 __device__ __forceinline__ void pf_force(const EntT* ent, const RunT start, int W, int pr, const PPent& ti, float mass_f,
                                          float rsoft_f, Acc& acc, unsigned long long& npl) {
   const int wx = (int)floorf(ti.x) + pr, wy = (int)floorf(ti.y) + pr, wz = (int)floorf(ti.z) + pr;
-  const RT mass = (RT)mass_f, rsoft2 = (RT)rsoft_f * (RT)rsoft_f, tx = (RT)ti.x, ty = (RT)ti.y, tz = (RT)ti.z;
+  const RT mass = (RT)mass_f, rsoft = (RT)rsoft_f, rsoft2 = (RT)rsoft_f * (RT)rsoft_f, tx = (RT)ti.x, ty = (RT)ti.y,
+           tz = (RT)ti.z;
   for (int kz = wz - pr; kz <= wz + pr; kz++)
     for (int ky = wy - pr; ky <= wy + pr; ky++) {
       const int row = (kz * W + ky) * W;
@@ -441,12 +454,12 @@ __device__ __forceinline__ void pf_force(const EntT* ent, const RunT start, int 
         if (ROWSUM) {
           // branch-free: an excluded pair adds sc = 0 (a signed zero, which
           // leaves the running sum unchanged)
-          sc = pf_pair_sel(dx, dy, dz, mass, rsoft2);
+          sc = pf_pair_sel(dx, dy, dz, mass, rsoft, rsoft2);
           sx += sc * dx;
           sy += sc * dy;
           sz += sc * dz;
         } else {
-          if (!pf_pair(dx, dy, dz, mass, rsoft2, sc)) continue;
+          if (!pf_pair(dx, dy, dz, mass, rsoft, rsoft2, sc)) continue;
           acc.add(0, sc * dx);
           acc.add(1, sc * dy);
           acc.add(2, sc * dz);
