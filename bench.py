@@ -85,6 +85,21 @@ def read_traffic(kernel_slot, workload):
     return None
 
 
+def hbm_ceiling(nbytes=4 << 30, iters=5):
+    """Best copy rate (read + write GB/s) of the library's streaming-copy
+    probe over its four launch variants: the practical HBM ceiling."""
+    from cube_nufluid_amd import _lib
+    lib = _lib.load()
+    out = np.zeros(4)
+    rc = lib.cube_hbm_copy_probe(0, nbytes, iters, _lib.ptr(out))
+    if rc:
+        return None
+    log("HBM copy probe (%d MiB): plain %.0f / %.0f GB/s, nontemporal %.0f / %.0f GB/s (8 / 32 workgroups per CU)"
+        % (nbytes >> 20, *out))
+    return {"gbs": round(float(out.max()), 1), "variants_gbs": [round(float(v), 1) for v in out],
+            "kernel": "float4 grid-stride copy, 4 vectors per thread per iteration, %d MiB each way" % (nbytes >> 20)}
+
+
 def copy_calibration():
     """Three plain 4 GiB device copies (torch copy kernel): under
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE they give the counters' reading
@@ -486,6 +501,7 @@ def main():
 
     if args.copy_calib:
         copy_calibration()
+        hbm_ceiling()
         return
     import torch
     from cube_nufluid_amd.params import CONFIGS
@@ -603,6 +619,10 @@ def main():
         out["halofind"] = {"state": halofind_c3(cube, args.config)}
     last_dt = float(cube.dt) or 0.05
     cube.close()
+    ceil = hbm_ceiling()
+    if ceil:
+        roofline["measured_ceiling"] = ceil
+        roofline["frac_of_measured_ceiling"] = round(achieved / ceil["gbs"], 4)
     if not args.no_halofind:
         if args.halofind_evolved:
             out["halofind"]["evolved"] = halofind_evolved("C2", 1002, cpu=not args.no_cpu_baseline)

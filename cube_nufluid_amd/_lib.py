@@ -136,6 +136,7 @@ SIGNATURES = {
     "cube_profile_enable": (c_i32, [_P, c_i32]),
     "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),
     "cube_profile_slot_name": (c_i32, [c_i32, ctypes.c_char_p, ctypes.c_size_t]),
+    "cube_hbm_copy_probe": (c_i32, [c_i32, c_i64, c_i32, _P]),
 }
 
 _lib = None

@@ -63,19 +63,25 @@ def write_checkpoint(state, opath, z, image=1):
 
 def read_checkpoint(opath, z, nc, image=1, izipx=None, izipv=None):
     """particle_initialization.f90:34-82; a header whose izipx/izipv differ
-    from the requested ones is rejected as in :140-144."""
+    from the requested ones is rejected as in :41-45."""
     with open(output_name(opath, z, "info", image), "rb") as f:
         sim = SimHeader.from_bytes(f.read())
     if (izipx is not None and sim.izipx != izipx) or (izipv is not None and sim.izipv != izipv):
         raise ValueError("zip format incompatible: file %d/%d, expected %s/%s" % (sim.izipx, sim.izipv, izipx, izipv))
     n = sim.nplocal
-    xp = np.fromfile(output_name(opath, z, "xp", image), dtype=xdtype(sim.izipx), count=3 * n).reshape(n, 3)
-    vp = np.fromfile(output_name(opath, z, "vp", image), dtype=xdtype(sim.izipv), count=3 * n).reshape(n, 3)
-    rhoc = np.fromfile(output_name(opath, z, "np", image), dtype=np.int32, count=nc ** 3)
-    vfield = np.fromfile(output_name(opath, z, "vc", image), dtype=np.float32, count=3 * nc ** 3).reshape(-1, 3)
-    pid = np.fromfile(output_name(opath, z, "id", image), dtype=np.int32, count=n)
-    if xp.shape[0] != n or vp.shape[0] != n or pid.shape[0] != n or rhoc.shape[0] != nc ** 3:
-        raise ValueError("truncated checkpoint")
+
+    def load(content, dtype, count):
+        fn = output_name(opath, z, content, image)
+        a = np.fromfile(fn, dtype=dtype, count=count)
+        if a.size != count:
+            raise ValueError("truncated checkpoint: %s holds %d of %d values" % (fn, a.size, count))
+        return a
+
+    xp = load("xp", xdtype(sim.izipx), 3 * n).reshape(n, 3)
+    vp = load("vp", xdtype(sim.izipv), 3 * n).reshape(n, 3)
+    rhoc = load("np", np.int32, nc ** 3)
+    vfield = load("vc", np.float32, 3 * nc ** 3).reshape(-1, 3)
+    pid = load("id", np.int32, n)
     if int(rhoc.sum(dtype=np.int64)) != n:
         raise ValueError("sum(np) = %d != nplocal = %d" % (int(rhoc.sum()), n))
     return BoxState(sim, xp, vp, pid, rhoc, vfield)

@@ -27,6 +27,7 @@ SOURCES = {
     "fft.hip": [],
     "ic.hip": ["-ffp-contract=off"],
     "pk.hip": [],
+    "probe.hip": [],
     "serve.hip": ["-ffp-contract=off"],
     "fluid.hip": ["-ffp-contract=off"],
     "halo.hip": ["-ffp-contract=off"],

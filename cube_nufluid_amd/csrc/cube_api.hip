@@ -125,8 +125,14 @@ extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const 
     return fail(cube_fail(h, CUBE_ERR_ARG, "nnt=%lld must divide nc=%lld", (long long)p->nnt, (long long)p->nc));
   h->nt = p->nc / p->nnt;
   if (h->nt % 8) return fail(cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 8", (long long)h->nt));
-  if (p->ncb < 1 || p->ncb > h->nt)
-    return fail(cube_fail(h, CUBE_ERR_ARG, "ncb=%lld outside [1, nt]", (long long)p->ncb));
+  // the fine force of a node depends on the density within 15 fine cells
+  // (kernel_f.f90's 16^3 table); pm.f90:79-82 deposits the particles of
+  // coarse cells 2-ncb..nt+ncb-1, so a tile's density is complete that far
+  // out only for 4*(ncb-1) >= 16, and only then does the one periodic fine
+  // mesh used here give the reference's per-tile force
+  if (p->ncb < 5 || p->ncb > h->nt)
+    return fail(cube_fail(h, CUBE_ERR_ARG, "ncb=%lld outside [5, nt]: the fine force needs 4*(ncb-1) >= 16 buffer cells",
+                          (long long)p->ncb));
   if ((p->izipx != 1 && p->izipx != 2) || (p->izipv != 1 && p->izipv != 2))
     return fail(cube_fail(h, CUBE_ERR_ARG, "izipx/izipv must be 1 or 2"));
   if (p->precision != 4 && p->precision != 8)
@@ -393,9 +399,9 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   if ((r = kick_all(h, Ff, pf, a_mid, dt, c_old, c_new, vmx, f2f, probe_f, probe_c))) return r;
   const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
   const int prg = nu_kick ? prof_begin(h, PS_FLUID_GRAV) : -1;
-  for (int d = 0; nu_kick && d < 3; d++)
-    if ((r = fl_gravity_fine(h, d, Ff[d], pf[d], a_mid, dt))) return r;
-  if (nu_kick && (r = fl_gravity_finish(h, a_mid, dt))) return r;
+  if (nu_kick && (r = fl_gravity_fine(h, Ff, pf, a_mid, dt))) return r;
+  void* const Fw[3] = {h->fine.S, h->fine.R, h->fine.X};
+  if (nu_kick && (r = fl_gravity_finish(h, Fw, pf, a_mid, dt))) return r;
   if (nu_kick) prof_end(h, prg);
   uint32_t hv[5];
   HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 32, sizeof(hv), hipMemcpyDeviceToHost, h->stream));

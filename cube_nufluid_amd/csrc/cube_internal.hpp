@@ -307,8 +307,9 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
 // fluid.hip: PM coupling of the neutrino fluid
 int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added);
 int fl_density_fine(cube_handle* h, float a_mid, double* mass_added);
-int fl_gravity_fine(cube_handle* h, int d, const void* F, int64_t pitch, float a_mid, float dt);
-int fl_gravity_finish(cube_handle* h, float a_mid, float dt);
+int fl_gravity_fine(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt);
+// reuses the fine-force buffers Ff for the coarse acceleration at fine-cell centres
+int fl_gravity_finish(cube_handle* h, void* const Ff[3], const int64_t pf[3], float a_mid, float dt);
 void fl_free(NuState& s);
 
 static inline Geo geo_of(const cube_handle* h) {

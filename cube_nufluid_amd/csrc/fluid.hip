@@ -269,10 +269,12 @@ __device__ __forceinline__ double fl_coarse_node(T* Rc, const float* D, int64_t 
 }
 
 // ---------------------------------------------------------------- gravity
-// Fill A (pitch nf+2) with component d of the coarse force CIC-interpolated
-// at every fine-cell centre, times a_mid*dt/6/pi (pm.f90:387-404).
+// Fill A_d (row pitch p_d) with component d of the coarse force
+// CIC-interpolated at every fine-cell centre, times a_mid*dt/6/pi
+// (pm.f90:387-404).  Fc holds the three components [3][ncc^3].
 template <typename T>
-__global__ void k_fl_coarse_at_centres(const T* Fc, int ncc, int64_t nf, float a_mid, float dt, float* A) {
+__global__ void k_fl_coarse_at_centres(const T* Fc, int ncc, int64_t nf, float a_mid, float dt, float* A0, float* A1, float* A2,
+                                       int64_t p0, int64_t p1, int64_t p2) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nf * nf * nf) return;
   const int64_t x = i % nf, y = (i / nf) % nf, z = i / (nf * nf);
@@ -288,117 +290,145 @@ __global__ void k_fl_coarse_at_centres(const T* Fc, int ncc, int64_t nf, float a
     k2[d] = f + 1 >= ncc ? f + 1 - ncc : f + 1;
   }
   const float kf = a_mid * dt / 6.0f / PI_F;
-  const int ord[8] = {0, 1, 2, 3, 4, 5, 6, 7};
-  double acc = 0;
+  const int64_t n3c = (int64_t)ncc * ncc * ncc;
+  double acc[3] = {0, 0, 0};
   for (int t = 0; t < 8; t++) {
-    const int o = ord[t], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-    const T fv = Fc[((int64_t)(oz ? k2[2] : k1[2]) * ncc + (oy ? k2[1] : k1[1])) * ncc + (ox ? k2[0] : k1[0])];
-    acc += (double)((float)fv * kf * (ox ? c2[0] : c1[0]) * (oy ? c2[1] : c1[1]) * (oz ? c2[2] : c1[2]));
+    const int ox = t & 1, oy = (t >> 1) & 1, oz = t >> 2;
+    const int64_t q = ((int64_t)(oz ? k2[2] : k1[2]) * ncc + (oy ? k2[1] : k1[1])) * ncc + (ox ? k2[0] : k1[0]);
+    for (int d = 0; d < 3; d++) {
+      const float fv = (float)Fc[d * n3c + q];
+      acc[d] += (double)(fv * kf * (ox ? c2[0] : c1[0]) * (oy ? c2[1] : c1[1]) * (oz ? c2[2] : c1[2]));
+    }
   }
-  A[(z * nf + y) * (nf + 2) + x] = (float)acc;
+  const int64_t r = z * nf + y;
+  A0[r * p0 + x] = (float)acc[0];
+  A1[r * p1 + x] = (float)acc[1];
+  A2[r * p2 + x] = (float)acc[2];
 }
 
-// The same gather, tiled: a workgroup takes a TX x TY x TZ block of fluid
-// cells, stages the fine region they reach ((T+1)*DX cells per dimension,
-// with the node shift and kf scaling of the fine force applied) in LDS,
-// and TPC threads share each cell's taps (split over z planes), so every
-// fine value is read from HBM about (1+1/T)^3 times instead of 8 times.
-// The CIC weight of fine cell m for fluid cell c depends only on
-// j = m - (c DX - DX/2): (j+0.5)/DX below DX, 2-(j+0.5)/DX above, exact in
-// float.  Partial sums are combined in a fixed order, so results are
-// deterministic; they differ from the one-thread-per-cell order only by
-// double rounding.
-template <typename T, bool FINE, int DX, int TX, int TY, int TZ, int TPC>
-__global__ __launch_bounds__(256) void k_fl_grav_tile(const T* A, int64_t ap, int64_t nf, int nc, int d, float a_mid,
-                                                      float dt, double* acc) {
-  constexpr int EX = (TX + 1) * DX, EY = (TY + 1) * DX, EZ = (TZ + 1) * DX, NCELL_WG = TX * TY * TZ;
-  // a cell's taps are shared by TPC = PZ * PY threads: PZ groups of z
-  // planes, each split into PY groups of y rows
-  constexpr int SPAN = 2 * DX, PZ = TPC < SPAN ? TPC : SPAN, PY = TPC / PZ, PLANES = SPAN / PZ, ROWS = SPAN / PY;
-  static_assert(NCELL_WG * TPC == 256, "256 threads per workgroup");
-  static_assert(SPAN % PZ == 0 && TPC % PZ == 0 && SPAN % PY == 0, "taps split evenly");
-  __shared__ float sA[EZ * EY * EX];
-  __shared__ double sJ[TPC > 1 ? 256 : 1], sQ[TPC > 1 ? 256 : 1];
-  const int ntx = nc / TX, nty = nc / TY;
-  const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
-  const int c0x = bx * TX, c0y = by * TY, c0z = bz * TZ;
-  const int m0x = c0x * DX - DX / 2, m0y = c0y * DX - DX / 2, m0z = c0z * DX - DX / 2;
-  const float kf = a_mid * dt / 6.0f / PI_F;
+// The gather of all three components at once, as separable sums.  With
+// dV = w(jx) w(jy) w(jz) and a_d the acceleration on fine cell (jx,jy,jz)
+// of the window,
+//   J_d = sum_z w sum_y w sum_x w a_d,
+//   Q   = sum dV |a|^2 - sum dV^2 |a|^2   (both separable, weights w and w^2),
+// so each fine value enters three 1-D passes instead of (2 DX)^3 products.
+// A workgroup owns a TXY x TXY column of fluid cells and CZ cells of it in
+// z, and marches through the (CZ+1) DX fine planes those cells reach:
+// per plane, the x pass (one item per window row and cell, reading the
+// three components from HBM) writes five partial sums per row to LDS, and
+// one thread per cell folds them along y and adds the plane, weighted, to
+// the two cells whose z windows hold it.  Sums are in double in a fixed
+// order; they differ from the per-tap products only by rounding.
+template <typename T, bool FINE, int DX, int TXY, int CZ>
+__global__ __launch_bounds__(256) void k_fl_grav_sep(const T* A0, const T* A1, const T* A2, int64_t p0, int64_t p1, int64_t p2,
+                                                     int64_t nf, int nc, float a_mid, float dt, double* acc) {
+  constexpr int SPAN = 2 * DX, EY = (TXY + 1) * DX, NX = EY * TXY;
+  static_assert(TXY * TXY <= 256, "one thread per cell of the column");
+  __shared__ double sX[5][NX];
+  const int ncol = nc / TXY;
+  const int bx = blockIdx.x % ncol, by = (blockIdx.x / ncol) % ncol, bz = blockIdx.x / (ncol * ncol);
+  const int c0x = bx * TXY, c0y = by * TXY, c0z = bz * CZ;
   const int s = FINE ? 1 : 0;  // fine force: node on the cell's right edge
-  for (int e = threadIdx.x; e < EZ * EY * EX; e += 256) {
-    const int lx = e % EX, ly = (e / EX) % EY, lz = e / (EX * EY);
-    const int64_t ax = ((int64_t)(m0x + lx + s) + nf) % nf, ay = ((int64_t)(m0y + ly + s) + nf) % nf,
-                  az = ((int64_t)(m0z + lz + s) + nf) % nf;
-    const float fv = (float)A[(az * nf + ay) * ap + ax];
-    sA[e] = FINE ? fv * kf : fv;
-  }
-  __syncthreads();
-  const int cell = threadIdx.x / TPC, part = threadIdx.x % TPC;
-  const int tx = cell % TX, ty = (cell / TX) % TY, tz = cell / (TX * TY);
-  float w[SPAN];
-#pragma unroll
-  for (int j = 0; j < SPAN; j++) w[j] = j < DX ? ((float)j + 0.5f) / (float)DX : 2.0f - ((float)j + 0.5f) / (float)DX;
+  const int64_t m0x = (int64_t)c0x * DX - DX / 2 + s, m0y = (int64_t)c0y * DX - DX / 2 + s,
+                m0z = (int64_t)c0z * DX - DX / 2 + s;
+  const float kf = a_mid * dt / 6.0f / PI_F;
   const double inv3 = 1.0 / ((double)DX * (double)DX * (double)DX);
-  double J = 0, Q = 0;
-  const int gz = part / PY, gy = part % PY;
-  for (int pz = 0; pz < PLANES; pz++) {
-    const int jz = gz * PLANES + pz;
-    const float wz = w[jz];
-    const float* pl = sA + ((tz * DX + jz) * EY + ty * DX) * EX + tx * DX;
-    for (int ry = 0; ry < ROWS; ry++) {
-      const int jy = gy * ROWS + ry;
-      const float wy = w[jy];
+  double w[SPAN];
+#pragma unroll
+  for (int j = 0; j < SPAN; j++) w[j] = j < DX ? ((double)j + 0.5) / DX : 2.0 - ((double)j + 0.5) / DX;
+  const int tx = threadIdx.x % TXY, ty = threadIdx.x / TXY;
+  const bool owner = threadIdx.x < TXY * TXY;
+  double cur[5] = {0, 0, 0, 0, 0}, prev[5] = {0, 0, 0, 0, 0};
+  const int64_t n3 = (int64_t)nc * nc * nc;
+  for (int pz = 0; pz < (CZ + 1) * DX; pz++) {
+    const int64_t mz = (m0z + pz + nf) % nf;
+    for (int e = threadIdx.x; e < NX; e += 256) {
+      const int ry = e / TXY, cx = e % TXY;
+      const int64_t my = (m0y + ry + nf) % nf;
+      const int64_t r0 = (mz * nf + my) * p0, r1 = (mz * nf + my) * p1, r2 = (mz * nf + my) * p2;
+      double x0 = 0, x1 = 0, x2 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int jx = 0; jx < SPAN; jx++) {
-        const float dV = w[jx] * wy * wz;
-        const double a = (double)pl[jy * EX + jx] * inv3;
-        J += (double)dV * a;
-        Q += (double)dV * (1.0 - (double)dV) * a * a;
+        const int64_t mx = (m0x + (int64_t)cx * DX + jx + nf) % nf;
+        float f0 = (float)A0[r0 + mx], f1 = (float)A1[r1 + mx], f2 = (float)A2[r2 + mx];
+        if (FINE) {
+          f0 = f0 * kf;
+          f1 = f1 * kf;
+          f2 = f2 * kf;
+        }
+        const double a0 = (double)f0 * inv3, a1 = (double)f1 * inv3, a2 = (double)f2 * inv3;
+        const double q = a0 * a0 + a1 * a1 + a2 * a2;
+        x0 += w[jx] * a0;
+        x1 += w[jx] * a1;
+        x2 += w[jx] * a2;
+        s1 += w[jx] * q;
+        s2 += w[jx] * w[jx] * q;
+      }
+      sX[0][e] = x0;
+      sX[1][e] = x1;
+      sX[2][e] = x2;
+      sX[3][e] = s1;
+      sX[4][e] = s2;
+    }
+    __syncthreads();
+    if (owner) {
+      double y[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int jy = 0; jy < SPAN; jy++) {
+        const int e = (ty * DX + jy) * TXY + tx;
+        y[0] += w[jy] * sX[0][e];
+        y[1] += w[jy] * sX[1][e];
+        y[2] += w[jy] * sX[2][e];
+        y[3] += w[jy] * sX[3][e];
+        y[4] += w[jy] * w[jy] * sX[4][e];
+      }
+      // plane pz is position j of cell l = pz / DX's window and j + DX of
+      // cell l - 1's
+      const int j = pz % DX, l = pz / DX;
+      const double wa = w[j], wb = w[j + DX];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        cur[k] += wa * y[k];
+        prev[k] += wb * y[k];
+      }
+      cur[4] += wa * wa * y[4];
+      prev[4] += wb * wb * y[4];
+      if (j == DX - 1) {
+        if (l >= 1) {
+          const int64_t i = ((int64_t)(c0z + l - 1) * nc + (c0y + ty)) * nc + (c0x + tx);
+          acc[i] += prev[0];
+          acc[n3 + i] += prev[1];
+          acc[2 * n3 + i] += prev[2];
+          acc[3 * n3 + i] += prev[3] - prev[4];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+          prev[k] = cur[k];
+          cur[k] = 0;
+        }
       }
     }
-  }
-  const int64_t n3 = (int64_t)nc * nc * nc;
-  const int64_t i = ((int64_t)(c0z + tz) * nc + (c0y + ty)) * nc + (c0x + tx);
-  if (TPC == 1) {
-    acc[(int64_t)d * n3 + i] += J;
-    acc[3 * n3 + i] += Q;
-    return;
-  }
-  sJ[threadIdx.x] = J;
-  sQ[threadIdx.x] = Q;
-  __syncthreads();
-  if (part == 0) {
-    double Js = 0, Qs = 0;
-    for (int k = 0; k < TPC; k++) {
-      Js += sJ[threadIdx.x + k];
-      Qs += sQ[threadIdx.x + k];
-    }
-    acc[(int64_t)d * n3 + i] += Js;
-    acc[3 * n3 + i] += Qs;
+    __syncthreads();
   }
 }
 
 template <typename T, bool FINE>
-static bool fl_grav_tiled(cube_handle* h, const T* A, int64_t ap, int n, int d, float a_mid, float dt) {
+static bool fl_grav_sep(cube_handle* h, const T* const A[3], const int64_t pa[3], int n, float a_mid, float dt) {
   NuState& s = h->nu;
   const int nc = s.nc[n];
   const int dx = (int)s.dx[n];
   const int64_t nf = h->nf;
-  if (dx == 2 && nc % 8 == 0 && nc % 4 == 0) {
-    const unsigned g = (unsigned)((nc / 8) * (nc / 8) * (nc / 4));
-    k_fl_grav_tile<T, FINE, 2, 8, 8, 4, 1><<<g, 256, 0, h->stream>>>(A, ap, nf, nc, d, a_mid, dt, s.acc[n]);
-    return true;
+#define FL_SEP(DX_, TXY_, CZ_)                                                                                        \
+  if (dx == DX_ && nc % TXY_ == 0 && nc % CZ_ == 0) {                                                                 \
+    const unsigned g = (unsigned)((nc / TXY_) * (nc / TXY_) * (nc / CZ_));                                            \
+    k_fl_grav_sep<T, FINE, DX_, TXY_, CZ_><<<g, 256, 0, h->stream>>>(A[0], A[1], A[2], pa[0], pa[1], pa[2], nf, nc, a_mid, \
+                                                                      dt, s.acc[n]);                                  \
+    return true;                                                                                                      \
   }
-  if (dx == 4 && nc % 4 == 0) {
-    const unsigned g = (unsigned)((nc / 4) * (nc / 4) * (nc / 4));
-    k_fl_grav_tile<T, FINE, 4, 4, 4, 4, 4><<<g, 256, 0, h->stream>>>(A, ap, nf, nc, d, a_mid, dt, s.acc[n]);
-    return true;
-  }
-  if (dx == 8 && nc % 2 == 0) {
-    const unsigned g = (unsigned)((nc / 2) * (nc / 2) * (nc / 2));
-    k_fl_grav_tile<T, FINE, 8, 2, 2, 2, 32><<<g, 256, 0, h->stream>>>(A, ap, nf, nc, d, a_mid, dt, s.acc[n]);
-    return true;
-  }
+  FL_SEP(2, 16, 8)
+  FL_SEP(4, 16, 8)
+  FL_SEP(8, 8, 8)
+#undef FL_SEP
   return false;
 }
 
@@ -661,21 +691,35 @@ int fl_density_fine(cube_handle* h, float a_mid, double* mass_added) {
   return 0;
 }
 
-// Fine-force component d (in fine.W) into the accumulators of every active fluid.
-int fl_gravity_fine(cube_handle* h, int d, const void* F, int64_t pitch, float a_mid, float dt) {
+// The three fine-force components (Ff[d], row pitch pf[d]) into the
+// accumulators of every active fluid.
+int fl_gravity_fine(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt) {
   NuState& s = h->nu;
   for (int n = 0; n < 3; n++) {
     if (!(a_mid >= s.ai[n])) continue;
     const int64_t n3 = (int64_t)s.nc[n] * s.nc[n] * s.nc[n];
-    if (h->prec == 4 && fl_grav_tiled<float, true>(h, (const float*)F, pitch, n, d, a_mid, dt))
-      ;
-    else if (h->prec == 4)
-      k_fl_grav_gather<float, true><<<nblk(n3), 256, 0, h->stream>>>((const float*)F, pitch, h->nf, s.nc[n], s.dx[n], d, a_mid,
-                                                                    dt, s.acc[n]);
-    else
-      k_fl_grav_gather<double, true><<<nblk(n3), 256, 0, h->stream>>>((const double*)F, pitch, h->nf, s.nc[n], s.dx[n], d, a_mid,
-                                                                     dt, s.acc[n]);
-    HIP_TRY(h, hipGetLastError());
+    if (h->prec == 4) {
+      const float* A[3] = {(const float*)Ff[0], (const float*)Ff[1], (const float*)Ff[2]};
+      if (fl_grav_sep<float, true>(h, A, pf, n, a_mid, dt)) {
+        HIP_TRY(h, hipGetLastError());
+        continue;
+      }
+    } else {
+      const double* A[3] = {(const double*)Ff[0], (const double*)Ff[1], (const double*)Ff[2]};
+      if (fl_grav_sep<double, true>(h, A, pf, n, a_mid, dt)) {
+        HIP_TRY(h, hipGetLastError());
+        continue;
+      }
+    }
+    for (int d = 0; d < 3; d++) {
+      if (h->prec == 4)
+        k_fl_grav_gather<float, true><<<nblk(n3), 256, 0, h->stream>>>((const float*)Ff[d], pf[d], h->nf, s.nc[n], s.dx[n], d,
+                                                                      a_mid, dt, s.acc[n]);
+      else
+        k_fl_grav_gather<double, true><<<nblk(n3), 256, 0, h->stream>>>((const double*)Ff[d], pf[d], h->nf, s.nc[n], s.dx[n],
+                                                                       d, a_mid, dt, s.acc[n]);
+      HIP_TRY(h, hipGetLastError());
+    }
   }
   return 0;
 }
@@ -692,8 +736,10 @@ static int fl_apply(cube_handle* h, float a_mid) {
 }
 
 // After the fine kicks: apply the fine impulses, then gather and apply the
-// coarse ones (pm.f90:374-422 runs after the fine phase).
-int fl_gravity_finish(cube_handle* h, float a_mid, float dt) {
+// coarse ones (pm.f90:374-422 runs after the fine phase).  The coarse
+// acceleration at fine-cell centres goes into the fine-force buffers
+// Ff[d] (row pitch pf[d]), which the fine phase no longer needs.
+int fl_gravity_finish(cube_handle* h, void* const Ff[3], const int64_t pf[3], float a_mid, float dt) {
   NuState& s = h->nu;
   bool any = false;
   for (int n = 0; n < 3; n++) any = any || a_mid >= s.ai[n];
@@ -701,22 +747,23 @@ int fl_gravity_finish(cube_handle* h, float a_mid, float dt) {
   int r;
   if ((r = fl_apply(h, a_mid))) return r;
   const int64_t nf = h->nf;
-  float* A = (float*)h->fine.W;
-  for (int d = 0; d < 3; d++) {
-    const char* Fc = (const char*)h->fc + (size_t)d * h->n3c * h->prec;
-    if (h->prec == 4)
-      k_fl_coarse_at_centres<float><<<nblk(nf * nf * nf), 256, 0, h->stream>>>((const float*)Fc, (int)h->nc, nf, a_mid, dt, A);
-    else
-      k_fl_coarse_at_centres<double><<<nblk(nf * nf * nf), 256, 0, h->stream>>>((const double*)Fc, (int)h->nc, nf, a_mid, dt, A);
-    HIP_TRY(h, hipGetLastError());
-    for (int n = 0; n < 3; n++) {
-      if (!(a_mid >= s.ai[n])) continue;
-      const int64_t n3 = (int64_t)s.nc[n] * s.nc[n] * s.nc[n];
-      if (!fl_grav_tiled<float, false>(h, A, nf + 2, n, d, a_mid, dt))
-        k_fl_grav_gather<float, false><<<nblk(n3), 256, 0, h->stream>>>(A, nf + 2, nf, s.nc[n], s.dx[n], d, a_mid, dt,
+  float* A[3] = {(float*)Ff[0], (float*)Ff[1], (float*)Ff[2]};
+  if (h->prec == 4)
+    k_fl_coarse_at_centres<float><<<nblk(nf * nf * nf), 256, 0, h->stream>>>((const float*)h->fc, (int)h->nc, nf, a_mid, dt,
+                                                                             A[0], A[1], A[2], pf[0], pf[1], pf[2]);
+  else
+    k_fl_coarse_at_centres<double><<<nblk(nf * nf * nf), 256, 0, h->stream>>>((const double*)h->fc, (int)h->nc, nf, a_mid, dt,
+                                                                              A[0], A[1], A[2], pf[0], pf[1], pf[2]);
+  HIP_TRY(h, hipGetLastError());
+  const float* Ac[3] = {A[0], A[1], A[2]};
+  for (int n = 0; n < 3; n++) {
+    if (!(a_mid >= s.ai[n])) continue;
+    const int64_t n3 = (int64_t)s.nc[n] * s.nc[n] * s.nc[n];
+    if (!fl_grav_sep<float, false>(h, Ac, pf, n, a_mid, dt))
+      for (int d = 0; d < 3; d++)
+        k_fl_grav_gather<float, false><<<nblk(n3), 256, 0, h->stream>>>(A[d], pf[d], nf, s.nc[n], s.dx[n], d, a_mid, dt,
                                                                         s.acc[n]);
-      HIP_TRY(h, hipGetLastError());
-    }
+    HIP_TRY(h, hipGetLastError());
   }
   return fl_apply(h, a_mid);
 }

@@ -5,6 +5,7 @@ All time-stepping scalars are real(4) in the reference; they are kept as
 numpy float32 here so the dt/a sequence is computed with the same rounding.
 """
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -96,9 +97,12 @@ class Cube:
         differs in its last bits from the a the run reached when it wrote
         the checkpoint.  exact=True takes a, t and tau from the header
         instead, so that a resumed CDM run repeats the uninterrupted one bit
-        for bit.  dt_neu restarts at dt_max either way, as in the reference."""
-        import os
-        from .checkpoint import read_checkpoint
+        for bit.  The fluids' dt_neu restarts at their CFL step (the
+        reference restarts it at dt_max).  z_checkpoints is a list or the
+        path of a z_checkpoint.txt."""
+        from .checkpoint import read_checkpoint, read_z_list
+        if isinstance(z_checkpoints, (str, os.PathLike)):
+            z_checkpoints = read_z_list(z_checkpoints)
         z0 = z_checkpoints[0]
         st = read_checkpoint(opath, z0, cfg.nc, izipx=cfg.izipx, izipv=cfg.izipv)
         self = cls(cfg, st, device=device, np_max=np_max, z_checkpoints=z_checkpoints, z_halofind=z_halofind)
@@ -114,7 +118,11 @@ class Cube:
         return self
 
     def _init_time(self, a0, z_checkpoints, z_halofind=None):
-        # initialize.f90:40-53 and particle_initialization.f90:37-39
+        # initialize.f90:40-61 and particle_initialization.f90:37-39; a path
+        # names a z_checkpoint.txt
+        if isinstance(z_checkpoints, (str, os.PathLike)):
+            from .checkpoint import read_z_list
+            z_checkpoints = read_z_list(z_checkpoints)
         self.z_checkpoints = list(z_checkpoints) if z_checkpoints else [float(1.0 / a0 - 1.0), 0.0]
         self.cur_checkpoint = 1
         self.a = f32(1.0 / (1.0 + f32(self.z_checkpoints[0])))
@@ -204,7 +212,6 @@ class Cube:
 
     def nu_checkpoint(self, fn):
         """dfld_checkpoint (hydrodf.f90:128-146) of the resident fluids."""
-        import os
         from .checkpoint import write_neu_checkpoint
         os.makedirs(os.path.dirname(fn) or ".", exist_ok=True)
         info = [self.nu_info(n) for n in range(3)]

@@ -43,7 +43,7 @@ enum cube_status {
 typedef struct cube_params {
   int64_t nc;        /* coarse cells per dim (parameters.f90:29)             */
   int64_t nnt;       /* tiles per dim (:28); nt = nc/nnt                     */
-  int64_t ncb;       /* coarse buffer cells (:47)                            */
+  int64_t ncb;       /* coarse buffer cells (:47), 5 <= ncb <= nt            */
   int64_t izipx;     /* bytes per position component, 1 or 2 (:8)           */
   int64_t izipv;     /* bytes per velocity component, 1 or 2 (:9)           */
   int64_t np_max;    /* particle capacity of the resident arrays            */
@@ -335,6 +335,14 @@ int cube_ens_destroy(cube_ensemble* e);
 int cube_profile_enable(cube_handle* h, int on);
 int cube_profile_read(cube_handle* h, double* ms, int64_t* count, int nslot);
 int cube_profile_slot_name(int slot, char* buf, size_t n);
+
+/* HBM copy-bandwidth probe (no counterpart in the reference; a measuring
+   aid for scoring kernels against HBM): times a streaming float4 copy of
+   `bytes` bytes, iters times after one warm launch, in four variants
+   ({plain, nontemporal} x {8, 32} workgroups per CU) and writes each
+   variant's best rate, read plus write bytes per second / 1e9, to
+   gbps[0..3].  Allocates and frees 2*bytes of device memory. */
+int cube_hbm_copy_probe(int device, int64_t bytes, int iters, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -206,6 +206,12 @@ def test_gpu_error_returns():
     with pytest.raises(_lib.CubeError) as e:
         Cube.from_ic(small, seed=1, z=5.0)
     assert e.value.code == 2
+    # a coarse buffer whose deposited particles do not reach the fine
+    # kernel's 15-cell support (4*(ncb-1) < 16)
+    thin = BoxConfig(nc=16, nnt=1, ncb=4, box=50.0)
+    with pytest.raises(_lib.CubeError) as e:
+        Cube(thin, st)
+    assert e.value.code == 1 and "ncb" in str(e.value)
 
 
 # --------------------------------------------------------- sizes of C2
@@ -334,3 +340,112 @@ def test_gpu_c3_size_first_tile_and_properties():
         orc.close()
     finally:
         cube.close()
+
+
+# --------------------------------------------------------- sizes of C4
+def test_gpu_c4_size_fluids():
+    """C4 at its size: 512^3 CDM particles on the 1024^3 fine mesh with the
+    three neutrino fluids on 512^3, 256^3 and 128^3 cells, generated on the
+    GPU (cube_nu_ic).  Fluid means are 1 and the fine-mesh mass with the
+    fluid coupled is nf^3; one hydro step of every fluid in each Strang
+    order equals the restatement (oracle/fluid_oracle.c) run from the same
+    downloaded fields, to rtol 1e-6, with the same CFL step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from cube_nufluid_amd.checkpoint import neu_ncb
+    from cube_nufluid_amd.driver import Cube
+    cfg = CONFIGS["C4"]
+    print("C4 from_ic", flush=True)
+    cube = Cube.from_ic(cfg, seed=1004, z=cfg.z_i)
+    try:
+        infos = [cube.nu_info(n) for n in range(3)]
+        assert [i["nc"] for i in infos] == [512, 256, 128]
+        cur = []
+        for n in range(3):
+            f = cube.nu_get(n)
+            assert abs(float(f[0].mean(dtype=np.float64)) - 1) < 1e-5
+            cur.append(f)
+        a_mid, dt0 = np.float32(1 / 11.0), np.float32(0.0)
+        cube.a_mid, cube.dt = a_mid, dt0
+        out = cube.particle_mesh()
+        print("C4 mass_f", out.mass_f, "nf^3", cfg.nf ** 3, flush=True)
+        assert out.mass_f == pytest.approx(cfg.nf ** 3, rel=2e-6)
+        for n in range(3):
+            assert np.array_equal(cube.nu_get(n), cur[n])      # dt = 0 leaves the fluids
+        dt = np.float32(0.5) * np.float32(cube.dt_neu)
+        for direction in (1, -1):
+            cube.dt_neu = np.float32(1000)
+            print("C4 fluid step, direction", direction, "dt", float(dt), flush=True)
+            cube.fluid_evolve(dt, direction, float(a_mid))
+            dt_all = 1000.0
+            for n in range(3):
+                ref, dt_all = ol.fluid_evolve(cur[n], neu_ncb(infos[n]["dx"]), np.float32(5.0 / 3.0), float(dt),
+                                              infos[n]["dx"], direction, dt_all)
+                got = cube.nu_get(n)
+                err = float(np.abs(got - ref).max())
+                print("  fluid", n, "max abs diff", err, flush=True)
+                np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
+                assert not np.array_equal(got, cur[n])
+                cur[n] = got
+            assert abs(float(cube.dt_neu) / dt_all - 1) < 1e-6
+    finally:
+        cube.close()
+
+
+# ------------------------------------------------ P(k) against the CPU path
+def test_gpu_pk_z0_against_cpu_path():
+    """P(k) at z = 0 of the GPU run against the CPU path (the oracle's
+    restatement of CUBE's step): a 32^3-particle, 100 Mpc/h box with PP,
+    Zel'dovich start at z = 49, run to z = 0 by the GPU host loop; the
+    oracle replays the same time steps (drift, PP, PM, and the closing half
+    drift) from the same initial state.  Both final states go through the
+    same estimator (oracle/power.py); P(k) agrees within 0.1% up to k_Nyq/2
+    of the fine mesh.  The oracle's wall time is printed."""
+    import time
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib as ol
+    import power
+    from cube_nufluid_amd.driver import Cube
+    from cube_nufluid_amd.ic import zeldovich_ic
+    cfg = BoxConfig(nc=16, nnt=1, ncb=6, box=100.0, pp_on=True, pp_range=1)
+    st0 = zeldovich_ic(cfg, seed=2024, z=49.0)
+    cube = Cube(cfg, st0, z_checkpoints=[49.0, 0.0])
+    seq = []
+    step_dt = cube.timestep
+
+    def timestep():
+        step_dt()
+        seq.append((float(cube.dt_old), float(cube.dt), float(cube.a_mid), bool(cube.checkpoint_step)))
+    cube.timestep = timestep
+    t = time.perf_counter()
+    n = cube.run(max_steps=5000)
+    print("GPU: %d steps to a = %.6f in %.1f s" % (n, float(cube.a), time.perf_counter() - t), flush=True)
+    assert abs(float(cube.a) - 1) < 1e-5
+    sg = cube.download()
+    cube.close()
+    o = ol.Oracle(cfg, threads=min(16, os.cpu_count() or 1))
+    o.set_state(st0)
+    t = time.perf_counter()
+    for dt_old, dt, a_mid, ck in seq:
+        o.drift(dt_old, dt)
+        o.pp(a_mid, dt)
+        o.pm(a_mid, dt)
+        if ck:
+            o.drift(0.0, dt)
+    t_cpu = time.perf_counter() - t
+    so = o.get_state()
+    o.close()
+    print("CPU path (oracle, %d threads): %d steps in %.1f s" % (o.threads, len(seq), t_cpu), flush=True)
+
+    def pk(s):
+        pos = power.positions_fine(s, cfg.nc, cfg.nt, cfg.nnt, cfg.izipx)
+        return power.power_bins(power.cic_density(pos, cfg.nf))
+    mg, kg, dg = pk(sg)
+    mo, ko, do = pk(so)
+    assert np.array_equal(mg, mo)
+    nb = cfg.nf // 4
+    err = np.abs(dg[:nb] / do[:nb] - 1)
+    print("P(k) z=0 GPU vs CPU path, max |dP/P| up to k_Nyq/2: %.2e; per bin %s" % (err.max(), np.round(err, 5)),
+          flush=True)
+    assert err.max() < 1e-3
