@@ -52,7 +52,10 @@ __global__ __launch_bounds__(FL_NT) void k_fl_sweep(float* F, int nc, float g, f
   extern __shared__ float sm[];
   const int64_t n3 = (int64_t)nc * nc * nc;
   const int nla = nc / FL_W;
-  const int a0 = (int)(blockIdx.x % nla) * FL_W, b = (int)(blockIdx.x / nla);
+  // XCD-contiguous order: the line groups adjacent in x that share cache
+  // lines (strided sweeps) run on the same XCD's L2
+  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int a0 = (int)(lb % nla) * FL_W, b = (int)(lb / nla);
   const int c1 = AX == 0 ? 1 : (AX == 1 ? 2 : 3), c2 = AX == 1 ? 1 : 2, c3 = AX == 2 ? 1 : 3;
   const int L = FL_W * nc;
   auto elem = [&](int e, int& t, int& w) {
@@ -326,7 +329,10 @@ __global__ __launch_bounds__(256) void k_fl_grav_sep(const T* A0, const T* A1, c
   static_assert(TXY * TXY <= 256, "one thread per cell of the column");
   __shared__ double sX[5][NX];
   const int ncol = nc / TXY;
-  const int bx = blockIdx.x % ncol, by = (blockIdx.x / ncol) % ncol, bz = blockIdx.x / (ncol * ncol);
+  // XCD-contiguous order: columns adjacent in x share the cache lines of
+  // their window edges on one L2
+  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = lb % ncol, by = (lb / ncol) % ncol, bz = lb / (ncol * ncol);
   const int c0x = bx * TXY, c0y = by * TXY, c0z = bz * CZ;
   const int s = FINE ? 1 : 0;  // fine force: node on the cell's right edge
   const int64_t m0x = (int64_t)c0x * DX - DX / 2 + s, m0y = (int64_t)c0y * DX - DX / 2 + s,
@@ -341,15 +347,15 @@ __global__ __launch_bounds__(256) void k_fl_grav_sep(const T* A0, const T* A1, c
   double cur[5] = {0, 0, 0, 0, 0}, prev[5] = {0, 0, 0, 0, 0};
   const int64_t n3 = (int64_t)nc * nc * nc;
   for (int pz = 0; pz < (CZ + 1) * DX; pz++) {
-    const int64_t mz = (m0z + pz + nf) % nf;
+    const int64_t mz = (m0z + pz) & (nf - 1);  // nf is a power of two
     for (int e = threadIdx.x; e < NX; e += 256) {
       const int ry = e / TXY, cx = e % TXY;
-      const int64_t my = (m0y + ry + nf) % nf;
+      const int64_t my = (m0y + ry) & (nf - 1);
       const int64_t r0 = (mz * nf + my) * p0, r1 = (mz * nf + my) * p1, r2 = (mz * nf + my) * p2;
       double x0 = 0, x1 = 0, x2 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int jx = 0; jx < SPAN; jx++) {
-        const int64_t mx = (m0x + (int64_t)cx * DX + jx + nf) % nf;
+        const int64_t mx = (m0x + (int64_t)cx * DX + jx) & (nf - 1);
         float f0 = (float)A0[r0 + mx], f1 = (float)A1[r1 + mx], f2 = (float)A2[r2 + mx];
         if (FINE) {
           f0 = f0 * kf;

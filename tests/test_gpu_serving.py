@@ -39,3 +39,25 @@ def test_mix_equals_alone_equals_python_host():
     st = cube.download()
     cube.close()
     assert fnv1a(st.xp, st.vp, st.pid) == hm[(32, 1000)][0]
+
+
+def test_c5_sizes_mix_equals_alone_z49_to_0():
+    """C5 as configured: boxes of 32^3, 64^3, 128^3 and 256^3 particles
+    (meshes 64^3 to 512^3), Zel'dovich start at z = 49, run to z = 0 with
+    CUBE's time-step limits; submitted together, every box's final state is
+    bit-identical to the same box run alone, and each takes the step count
+    it takes alone."""
+    from cube_nufluid_amd.serving import run_ensemble
+    reqs = [dict(t=0.0, npd=n, seed=1100 + i, z_i=49.0, z_f=0.0) for i, n in enumerate((256, 128, 64, 32, 128))]
+    print("C5 mix", flush=True)
+    mix = run_ensemble(reqs, max_live=8)
+    print("C5 alone", flush=True)
+    alone = run_ensemble(reqs, max_live=1)
+    key = lambda r: (r["npd"], r["seed"])
+    hm = {key(r): (r["hash"], r["steps"]) for r in mix["results"]}
+    ha = {key(r): (r["hash"], r["steps"]) for r in alone["results"]}
+    print("steps by box:", sorted((k[0], v[1]) for k, v in hm.items()), flush=True)
+    assert hm == ha
+    assert all(abs(r["a_final"] - 1.0) < 1e-5 for r in mix["results"])
+    print("mix %.3g particle-steps/s over %.1f s, alone %.3g over %.1f s" % (
+        mix["particle_steps_per_s"], mix["span"], alone["particle_steps_per_s"], alone["span"]), flush=True)
