@@ -405,23 +405,39 @@ def c4_bench(steps, warmup, seed=1004):
 
 
 def ensemble_small(n, rate):
-    """The same serving comparison on boxes of {32,64,128}^3 particles only:
-    boxes whose steps leave the GPU idle when run alone."""
+    """The same serving comparison on boxes of {32,64,128}^3 particles only,
+    boxes whose steps leave the GPU idle when run alone: one at a time with
+    synchronising phases (the plain form), and concurrently with
+    synchronising phases, with one synchronisation per step, and with each
+    step replayed as a hipGraph."""
     from cube_nufluid_amd.serving import poisson_requests, run_ensemble
     reqs = poisson_requests(n, rate, seed=7, sizes=(32, 64, 128))
-    mix = run_ensemble(reqs, max_live=64)
-    plain = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1)
     key = lambda r: (r["npd"], r["seed"])
-    same = {key(r): r["hash"] for r in mix["results"]} == {key(r): r["hash"] for r in plain["results"]}
-    log("ensemble, small boxes only: %.3g vs %.3g particle-steps/s one at a time (%.2fx)"
-        % (mix["particle_steps_per_s"], plain["particle_steps_per_s"],
-           mix["particle_steps_per_s"] / plain["particle_steps_per_s"]))
+    plain = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1, step_mode=0)
+    ref = {key(r): r["hash"] for r in plain["results"]}
+    runs, same = {}, True
+    for mode in (0, 1, 2):
+        runs[mode] = run_ensemble(reqs, max_live=64, step_mode=mode)
+        same = same and {key(r): r["hash"] for r in runs[mode]["results"]} == ref
+    alone_graph = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1, step_mode=2)
+    same = same and {key(r): r["hash"] for r in alone_graph["results"]} == ref
+    base = plain["particle_steps_per_s"]
+    log("ensemble, small boxes only: one at a time %.3g particle-steps/s; concurrent %.3g (sync phases), %.3g (one "
+        "sync per step), %.3g (hipGraph steps) = %.2fx / %.2fx / %.2fx; one at a time with hipGraph steps %.3g"
+        % (base, runs[0]["particle_steps_per_s"], runs[1]["particle_steps_per_s"], runs[2]["particle_steps_per_s"],
+           runs[0]["particle_steps_per_s"] / base, runs[1]["particle_steps_per_s"] / base,
+           runs[2]["particle_steps_per_s"] / base, alone_graph["particle_steps_per_s"]))
     return {"workload": "%d seeded LCDM PM+PP boxes, {32,64,128}^3 particles, z=49 to 0, Poisson arrivals %.1f/s"
                         % (n, rate),
-            "p50_s": round(mix["p50"], 4), "p99_s": round(mix["p99"], 4),
-            "particle_steps_per_s": mix["particle_steps_per_s"],
-            "plain_particle_steps_per_s": plain["particle_steps_per_s"],
-            "speedup_vs_one_at_a_time": mix["particle_steps_per_s"] / plain["particle_steps_per_s"],
+            "p50_s": round(runs[2]["p50"], 4), "p99_s": round(runs[2]["p99"], 4),
+            "plain_particle_steps_per_s": base,
+            "concurrent_particle_steps_per_s": {"sync_phases": runs[0]["particle_steps_per_s"],
+                                                "one_sync_per_step": runs[1]["particle_steps_per_s"],
+                                                "hipgraph_steps": runs[2]["particle_steps_per_s"]},
+            "speedup_vs_one_at_a_time": {"sync_phases": runs[0]["particle_steps_per_s"] / base,
+                                         "one_sync_per_step": runs[1]["particle_steps_per_s"] / base,
+                                         "hipgraph_steps": runs[2]["particle_steps_per_s"] / base},
+            "one_at_a_time_hipgraph_particle_steps_per_s": alone_graph["particle_steps_per_s"],
             "same_result_alone_and_in_mix": bool(same)}
 
 
@@ -429,17 +445,18 @@ def ensemble_bench(n, rate):
     """Ensemble serving: n seeded LCDM PM+PP boxes drawn uniformly from
     {32,64,128,256}^3 particles (mesh 2x particles per dimension), Zel'dovich
     start at z=49, run to z=0, arriving as a Poisson process; served
-    concurrently by the native engine, then the same boxes one at a time.
-    A box takes the steps CUBE's time-step limits give it from z=49 to 0
-    (the ra <= 0.2 expansion limit alone needs about 25 steps; the force
-    and velocity limits add the rest), not a fixed 50."""
+    concurrently by the native engine with each step replayed as a hipGraph,
+    then the same boxes one at a time in the plain form (synchronising
+    phases).  A box takes the steps CUBE's time-step limits give it from
+    z=49 to 0 (the ra <= 0.2 expansion limit alone needs about 25 steps; the
+    force and velocity limits add the rest), not a fixed 50."""
     from cube_nufluid_amd.serving import poisson_requests, run_ensemble
     reqs = poisson_requests(n, rate, seed=5)
     log("ensemble: %d requests, sizes %s, Poisson rate %.1f/s" % (n, sorted(set(r["npd"] for r in reqs)), rate))
-    mix = run_ensemble(reqs, max_live=64, log=log)
-    log("ensemble (concurrent): p50 %.3f s, p99 %.3f s, %.3g particle-steps/s"
+    mix = run_ensemble(reqs, max_live=64, log=log, step_mode=2)
+    log("ensemble (concurrent, hipGraph steps): p50 %.3f s, p99 %.3f s, %.3g particle-steps/s"
         % (mix["p50"], mix["p99"], mix["particle_steps_per_s"]))
-    plain = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1)
+    plain = run_ensemble([dict(r, t=0.0) for r in reqs], max_live=1, step_mode=0)
     log("one at a time: %.3g particle-steps/s over %.2f s" % (plain["particle_steps_per_s"], plain["span"]))
     key = lambda r: (r["npd"], r["seed"])
     same = {key(r): r["hash"] for r in mix["results"]} == {key(r): r["hash"] for r in plain["results"]}
@@ -452,6 +469,7 @@ def ensemble_bench(n, rate):
     log("one at a time, seconds by size: %s" % {k: round(v, 2) for k, v in sorted(by_size.items())})
     return {"workload": "C5: %d seeded LCDM PM+PP boxes, {32,64,128,256}^3 particles, z=49 to 0, Poisson arrivals "
                         "%.1f/s" % (n, rate),
+            "step_mode": "hipGraph steps (mix); synchronising phases (one at a time)",
             "p50_s": round(mix["p50"], 4), "p99_s": round(mix["p99"], 4),
             "particle_steps_per_s": mix["particle_steps_per_s"],
             "plain_particle_steps_per_s": plain["particle_steps_per_s"],

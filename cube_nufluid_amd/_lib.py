@@ -90,7 +90,7 @@ class NuParams(ctypes.Structure):
 
 
 class EnsParams(ctypes.Structure):
-    _fields_ = [("device", c_i32), ("max_live", c_i32), ("mem_budget", c_i64)]
+    _fields_ = [("device", c_i32), ("max_live", c_i32), ("mem_budget", c_i64), ("step_mode", c_i32), ("reserved", c_i32)]
 
 
 class EnsResult(ctypes.Structure):
@@ -137,6 +137,8 @@ SIGNATURES = {
     "cube_profile_read": (c_i32, [_P, _P, _P, c_i32]),
     "cube_profile_slot_name": (c_i32, [c_i32, ctypes.c_char_p, ctypes.c_size_t]),
     "cube_hbm_copy_probe": (c_i32, [c_i32, c_i64, c_i32, _P]),
+    "cube_set_step_mode": (c_i32, [_P, c_i32]),
+    "cube_step_stats": (c_i32, [_P, _P, _P]),
 }
 
 _lib = None

@@ -76,6 +76,15 @@ static const CodecTables& codec_tables(int izipv) {
   });
   return tabs[izipv];
 }
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+int fill_u32(cube_handle* h, void* p, int64_t n, uint32_t v) {
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+  k_fill_u32<<<(unsigned)std::max<int64_t>(nb, 1), 256, 0, h->stream>>>((uint32_t*)p, n, v);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
 __global__ void k_vtab_fill(const float* __restrict__ dec, int n, double c, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (double)dec[i] / c;
@@ -103,6 +112,12 @@ int codec_build(cube_handle* h) {
   HIP_TRY(h, hipMemcpy(h->vdec, t.dec.data(), sizeof(float) * t.dec.size(), hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(h->venc, t.thr.data(), sizeof(double) * t.thr.size(), hipMemcpyHostToDevice));
   h->vkmax = t.kmax;
+  HIP_TRY(h, hipMalloc(&h->vdt_new, sizeof(double) * t.dec.size()));
+  HIP_TRY(h, hipMalloc(&h->dscal, sizeof(double) * 4));
+  HIP_TRY(h, hipMalloc(&h->drep, sizeof(StepReport)));
+  HIP_TRY(h, hipHostMalloc(&h->rep, sizeof(StepReport), hipHostMallocDefault));
+  HIP_TRY(h, hipMemset(h->drep, 0, sizeof(StepReport)));
+  memset(h->rep, 0, sizeof(StepReport));
   return 0;
 }
 
@@ -110,6 +125,7 @@ int codec_build(cube_handle* h) {
 static bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 extern "C" int cube_init(const cube_params* p, const float* kern_f_table, const float* kern_c_table, cube_handle** out) {
+  CUBE_EXCLUSIVE;
   if (!p || !kern_f_table || !kern_c_table || !out) return CUBE_ERR_ARG;
   *out = nullptr;
   cube_handle* h = new cube_handle();
@@ -242,15 +258,18 @@ extern "C" int cube_profile_slot_name(int slot, char* buf, size_t n) {
 }
 
 extern "C" int cube_finalize(cube_handle* h) {
+  CUBE_EXCLUSIVE;
   if (!h) return CUBE_OK;
   for (ProfRec& r : h->prof_recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
   for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->rep) (void)hipHostFree(h->rep);
   void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->order, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
                   h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc, h->vdt, h->dbig,
-                  h->dscratch_own ? h->dscratch : nullptr, h->pp_meta, h->pp_red};
+                  h->dscratch_own ? h->dscratch : nullptr, h->pp_meta, h->pp_red, h->vdt_new, h->dscal, h->drep};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   mesh_free(h->fine);
@@ -271,6 +290,7 @@ extern "C" int cube_last_error(cube_handle* h, char* buf, size_t n) {
 // ------------------------------------------------------------------ state
 extern "C" int cube_upload_state(cube_handle* h, const cube_sim_header* sim, const void* xp, const void* vp,
                                  const int32_t* pid, const int32_t* rhoc, const float* vfield) {
+  CUBE_EXCLUSIVE;
   if (!h || !sim || !rhoc || !vfield || (sim->nplocal > 0 && (!xp || !vp || !pid))) return CUBE_ERR_ARG;
   if (sim->izipx != h->p.izipx || sim->izipv != h->p.izipv)
     return cube_fail(h, CUBE_ERR_FORMAT, "zip format incompatible: header %lld/%lld, handle %lld/%lld",
@@ -297,11 +317,12 @@ extern "C" int cube_upload_state(cube_handle* h, const cube_sim_header* sim, con
   h->sigma_vi_new = sim->sigma_vi;
   h->vz_max = sim->vz_max;
   h->mass_p = sim->mass_p_cdm;
-  return CUBE_OK;
+  return sigma_new_to_device(h);
 }
 
 extern "C" int cube_download_state(cube_handle* h, cube_sim_header* sim, void* xp, void* vp, int32_t* pid, int32_t* rhoc,
                                    float* vfield) {
+  CUBE_EXCLUSIVE;
   if (!h) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -330,12 +351,8 @@ extern "C" int cube_drift(cube_handle* h, float dt_old, float dt, cube_drift_out
   return drift_run(h, dt_old, dt, out);
 }
 
-extern "C" int cube_pp(cube_handle* h, float a_mid, float dt, cube_pp_out* out) {
-  if (!h) return CUBE_ERR_ARG;
-  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
-  cube_pp_out o{};
-  int r = pp_run(h, a_mid, dt, vcoef(h->sigma_vi), &o, nullptr);
-  if (r) return r;
+// dt_pp of ext_pp_force.f90:118-128 from the PP results
+int pp_step_finish(cube_handle* h, float a_mid, cube_pp_out& o) {
   // 'ftemp' in ext_pp_force.f90:18,121 is SAVEd and never reset: dt_pp uses
   // the running maximum over every tile of every call.
   h->ftemp_pp = std::max(h->ftemp_pp, o.f2_max_pp);
@@ -343,28 +360,38 @@ extern "C" int cube_pp(cube_handle* h, float a_mid, float dt, cube_pp_out* out) 
   o.f2_max_pp = h->ftemp_pp;
   o.dt_pp = 1.0f * sqrtf(1.0f) / (sqrtf(h->ftemp_pp) * a_mid * GG);
   h->sim.dt_pp = o.dt_pp;
-  if (out) *out = o;
   if (o.itest1 != h->np)
     return cube_fail(h, CUBE_ERR_CHECK, "itest1=%lld /= nplocal=%lld", (long long)o.itest1, (long long)h->np);
   return CUBE_OK;
 }
 
-static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, double* probe_f, double* probe_c) {
+extern "C" int cube_pp(cube_handle* h, float a_mid, float dt, cube_pp_out* out) {
+  if (!h) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  cube_pp_out o{};
+  int r = pp_run(h, a_mid, dt, vcoef(h->sigma_vi), &o, nullptr);
+  if (r) return r;
+  r = pp_step_finish(h, a_mid, o);
+  if (out) *out = o;
+  return r;
+}
+
+// particle_mesh (pm.f90:2-443) as one launch sequence; the masses, f2
+// maxima and vmax go to h->drep for pm_finish
+static int pm_launch(cube_handle* h, float a_mid, float dt, double* probe_f, double* probe_c) {
   int r;
-  double mass_c = 0, mass_f = 0;
   uint32_t* f2c = h->uflags + 32;
   uint32_t* f2f = h->uflags + 33;
   uint32_t* vmx = h->uflags + 34;  // 3 entries
-  HIP_TRY(h, hipMemsetAsync(h->uflags + 32, 0, sizeof(uint32_t) * 8, h->stream));
+  StepReport* dr = h->drep;
+  if ((r = fill_u32(h, h->uflags + 32, 8, 0u))) return r;
   const bool nu = h->nu.on;
   // coarse mesh: force_c for all three components (pm.f90:188-326); the
   // third inverse also forms max |force_c|^2 from the two already written
-  if ((r = cic_coarse(h, &mass_c))) return r;
-  double mass_cnu = 0;
+  if ((r = cic_coarse(h, nullptr, &dr->mass[0]))) return r;
   int prd = nu ? prof_begin(h, PS_FLUID_DENS) : -1;
-  if (nu && (r = fl_density_coarse(h, a_mid, &mass_cnu))) return r;
+  if (nu && (r = fl_density_coarse(h, a_mid, nullptr, &dr->mass[1]))) return r;
   if (nu) prof_end(h, prd);
-  mass_c += mass_cnu;
   if ((r = mesh_fwd_xy(h, h->coarse))) return r;
   const int64_t Nc = h->coarse.N;
   const char* fc = (const char*)h->fc;
@@ -379,12 +406,10 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
   // spectrum in W; components 0 and 1 are convolved into X and inverted into
   // S (F0) and R (F1), component 2 in place in W (the spectrum's last use)
   // and inverted into X (F2).  All three stay resident for one fused kick.
-  if ((r = cic_fine(h, &mass_f))) return r;
-  double mass_nu = 0;
+  if ((r = cic_fine(h, nullptr, &dr->mass[2]))) return r;
   prd = nu ? prof_begin(h, PS_FLUID_DENS) : -1;
-  if (nu && (r = fl_density_fine(h, a_mid, &mass_nu))) return r;
+  if (nu && (r = fl_density_fine(h, a_mid, nullptr, &dr->mass[3]))) return r;
   if (nu) prof_end(h, prd);
-  mass_f += mass_nu;
   if ((r = mesh_t_fwd(h, h->fine, h->fine.R, h->fine.W))) return r;
   const int64_t Nf = h->fine.N;
   const void* Ff[3] = {h->fine.S, h->fine.R, h->fine.X};
@@ -394,37 +419,44 @@ static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, do
     if ((r = mesh_t_zconv(h, h->fine, d, h->fine.W, spec))) return r;
     if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d]))) return r;
   }
-  const double c_old = vcoef(h->sigma_vi), c_new = vcoef(h->sigma_vi_new);
-  // the kick also forms max |F|^2 (dt_fine) from the force slabs it stages
-  if ((r = kick_all(h, Ff, pf, a_mid, dt, c_old, c_new, vmx, f2f, probe_f, probe_c))) return r;
+  // the kick decodes with sigma_vi and encodes with sigma_vi_new, whose
+  // coefficient the drift left in h->dscal; it also forms max |F|^2
+  // (dt_fine) from the force slabs it stages
+  if ((r = kick_all(h, Ff, pf, a_mid, dt, vcoef(h->sigma_vi), vmx, f2f, probe_f, probe_c))) return r;
   const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
   const int prg = nu_kick ? prof_begin(h, PS_FLUID_GRAV) : -1;
   if (nu_kick && (r = fl_gravity_fine(h, Ff, pf, a_mid, dt))) return r;
   void* const Fw[3] = {h->fine.S, h->fine.R, h->fine.X};
   if (nu_kick && (r = fl_gravity_finish(h, Fw, pf, a_mid, dt))) return r;
   if (nu_kick) prof_end(h, prg);
-  uint32_t hv[5];
-  HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 32, sizeof(hv), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  float f[5];
-  memcpy(f, hv, sizeof(f));
-  if (out) {
-    out->f2_max_coarse = f[0];
-    out->f2_max_fine = f[1];
-    out->vmax[0] = f[2]; out->vmax[1] = f[3]; out->vmax[2] = f[4];
-    out->mass_f = mass_f;
-    out->mass_c = mass_c;
-  }
+  HIP_TRY(h, hipMemcpyAsync(dr->pmflags, h->uflags + 32, sizeof(dr->pmflags), hipMemcpyDeviceToDevice, h->stream));
   return 0;
 }
 
-extern "C" int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out) {
-  if (!h) return CUBE_ERR_ARG;
-  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
-  cube_pm_out o{};
-  int r = pm_forces(h, a_mid, dt, &o, nullptr, nullptr);
+static void pm_outputs(cube_handle* h, const StepReport* rp, cube_pm_out* out) {
+  float f[5];
+  memcpy(f, rp->pmflags, sizeof(f));
+  out->f2_max_coarse = f[0];
+  out->f2_max_fine = f[1];
+  out->vmax[0] = f[2]; out->vmax[1] = f[3]; out->vmax[2] = f[4];
+  // testrhoc / testrhof include the fluid's mass once it is coupled
+  out->mass_c = rp->mass[0] + (h->nu.on ? rp->mass[1] : 0.0);
+  out->mass_f = rp->mass[2] + (h->nu.on ? rp->mass[3] : 0.0);
+}
+
+static int pm_forces(cube_handle* h, float a_mid, float dt, cube_pm_out* out, double* probe_f, double* probe_c) {
+  int r = pm_launch(h, a_mid, dt, probe_f, probe_c);
   if (r) return r;
-  h->sigma_vi = h->sigma_vi_new;  // pm.f90:180
+  HIP_TRY(h, hipMemcpyAsync(h->rep, h->drep, sizeof(StepReport), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (out) pm_outputs(h, h->rep, out);
+  return 0;
+}
+
+// the time-step inputs of pm.f90:427-443 from the PM results; sigma_vi
+// takes the new value (pm.f90:180)
+static void pm_finish(cube_handle* h, float a_mid, cube_pm_out& o) {
+  h->sigma_vi = h->sigma_vi_new;
   const float GG = 1.0f / 6.0f / PI_F;
   o.dt_fine = sqrtf(1.0f / (sqrtf(o.f2_max_fine) * a_mid * GG));
   o.dt_coarse = sqrtf((float)CUBE_NCELL / (sqrtf(o.f2_max_coarse) * a_mid * GG));
@@ -436,6 +468,24 @@ extern "C" int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out) 
   h->sim.dt_coarse = o.dt_coarse;
   h->sim.dt_vmax = o.dt_vmax;
   h->sim.vz_max = o.vz_max;
+}
+
+int pm_launch_step(cube_handle* h, float a_mid, float dt) { return pm_launch(h, a_mid, dt, nullptr, nullptr); }
+int pm_finish_step(cube_handle* h, const StepReport* rp, float a_mid, cube_pm_out* out) {
+  cube_pm_out o{};
+  pm_outputs(h, rp, &o);
+  pm_finish(h, a_mid, o);
+  if (out) *out = o;
+  return CUBE_OK;
+}
+
+extern "C" int cube_pm(cube_handle* h, float a_mid, float dt, cube_pm_out* out) {
+  if (!h) return CUBE_ERR_ARG;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  cube_pm_out o{};
+  int r = pm_forces(h, a_mid, dt, &o, nullptr, nullptr);
+  if (r) return r;
+  pm_finish(h, a_mid, o);
   if (out) *out = o;
   return CUBE_OK;
 }
@@ -474,6 +524,7 @@ extern "C" int cube_expansion(float a0, float dt0, float omega_m, float omega_l,
 
 // ------------------------------------------------------------- probes
 extern "C" int cube_probe_pm_forces(cube_handle* h, double* f_fine, double* f_coarse) {
+  CUBE_EXCLUSIVE;
   if (!h || !f_fine || !f_coarse) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   double *pf, *pc;
@@ -490,6 +541,7 @@ extern "C" int cube_probe_pm_forces(cube_handle* h, double* f_fine, double* f_co
 }
 
 extern "C" int cube_probe_pp_forces(cube_handle* h, double* f_pp) {
+  CUBE_EXCLUSIVE;
   if (!h || !f_pp) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   double* pf;
@@ -502,6 +554,7 @@ extern "C" int cube_probe_pp_forces(cube_handle* h, double* f_pp) {
 }
 
 extern "C" int cube_probe_density(cube_handle* h, float* rho) {
+  CUBE_EXCLUSIVE;
   if (!h || !rho) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   int r = cic_fine(h, nullptr);

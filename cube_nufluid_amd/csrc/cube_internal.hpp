@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 #include "cube_hip.h"
@@ -61,6 +63,23 @@ struct NuState {
   double* acc[3] = {nullptr, nullptr, nullptr};
 };
 
+// Device results of one step (drift, PP, PM), gathered on the device and
+// read by the host with one copy: the deferred step modes launch the whole
+// step without a host synchronisation and check it afterwards.
+struct StepReport {
+  uint32_t dflags[8];     // drift: max displacement per axis, too far / crowded, frame conflicts
+  int64_t npnew;          // drift: particles after the drift
+  double dsum[3];         // drift: sum_c, sum_r, sum_s
+  float sigma_new;        // drift: sigma_vi_new (update_particle.f90:213), formed on the device
+  int32_t pad0;
+  uint32_t ppflags[12];   // PP: uflags[8..19] (dense blocks, work items, pool fallbacks, ...)
+  uint64_t pp_pairs, pp_targets;
+  uint32_t pp_f2;         // PP: max |F|^2 bits over the slots
+  int32_t pad1;
+  double mass[4];         // PM: testrhoc, its fluid share, testrhof, its fluid share
+  uint32_t pmflags[8];    // PM: uflags[32..39] (f2 coarse, f2 fine, vmax[3])
+};
+
 struct cube_handle {
   cube_params p;
   int64_t nc, nt, nnt, ncb, nf, n3c;
@@ -93,6 +112,21 @@ struct cube_handle {
   uint32_t* uflags = nullptr; // small device scratch for maxima and flags
   int64_t* scan_tmp = nullptr;
   int64_t scan_tmp_n = 0;
+  // step report (device) and its pinned host copy; the new-sigma velocity
+  // coefficient and decode table, formed on the device by the drift and
+  // read by the kick
+  StepReport* drep = nullptr;
+  StepReport* rep = nullptr;
+  double* dscal = nullptr;    // [0] vcoef(sigma_vi_new)
+  double* vdt_new = nullptr;  // decoded-velocity table for dscal[0]
+  // step mode: 0 each call synchronises; 1 drift, PP and PM are launched
+  // back to back and checked after one synchronisation; 2 as 1, the
+  // launches captured into a hipGraph and replayed (kernel arguments
+  // updated in place)
+  int step_mode = 0;
+  bool deferred = false;      // inside a deferred step: phases do not synchronise
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_updates = 0, graph_instantiations = 0;
   // scalars mirrored from CUBE's module state
   float sigma_vi = 0, sigma_vi_new = 0, vz_max = 0, mass_p = 0;
   float ftemp_pp = 0;         // ext_pp_force.f90:18 running max (SAVE)
@@ -134,6 +168,19 @@ int cube_fail(cube_handle* h, int code, const char* fmt, ...);
 
 // ------------------------------------------------------------- codec (host)
 int codec_build(cube_handle* h);  // vdec/venc tables from the host libm
+// dscal[0] and vdt_new from the host's sigma_vi_new (after an upload, an IC
+// or a failed drift); the drift forms them on the device otherwise
+int sigma_new_to_device(cube_handle* h);
+// fill n 32-bit words with v on the handle's stream (a kernel, so that a
+// captured step holds no memset node)
+int fill_u32(cube_handle* h, void* p, int64_t n, uint32_t v);
+// While one thread captures a step into a hipGraph, no other thread may run
+// a legacy-stream operation (synchronous copies and memsets) or a
+// device-synchronising one (hipFree): the runtime rejects them.  Captures
+// hold this lock shared; the entry points that make such calls (init,
+// finalize, up/download, ICs, probes, P(k), halo finder) hold it exclusively.
+std::shared_mutex& capture_mutex();
+#define CUBE_EXCLUSIVE std::unique_lock<std::shared_mutex> capture_lock_(capture_mutex())
 
 // ------------------------------------------------------------- device helpers
 struct Geo {  // box geometry passed by value to kernels
@@ -271,6 +318,8 @@ const double* vtab_for(cube_handle* h, double c);
 // scan.hip
 int scan_counts(cube_handle* h, const int32_t* cnt, int64_t n, int64_t* cstart);
 int reduce_doubles(cube_handle* h, const double* part, int64_t nparts, int nfields, double* host_out);
+// the same fixed-order sums into device memory, no synchronisation
+int reduce_doubles_dev(cube_handle* h, const double* part, int64_t nparts, int nfields, double* dev_out);
 // fft.hip
 int mesh_alloc(cube_handle* h, Mesh& m, int64_t N, int prec, bool with_S, bool with_K);
 int mesh_pass(cube_handle* h, Mesh& m, void* buf, int axis, bool inverse);    // strided y (1) / z (2) pass
@@ -293,20 +342,33 @@ int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch);
 int fine_kernel_build(cube_handle* h, const float* table);
 int coarse_kernel_build(cube_handle* h, const float* table);
 // pm.hip
-int cic_fine(cube_handle* h, double* mass);
+// mass: host destination of the deposited mass (synchronises), or mass_dev:
+// device destination (no synchronisation)
+int cic_fine(cube_handle* h, double* mass, double* mass_dev = nullptr);
 int pk_scratch_setup(cube_handle* h);  // device address of the P(k) scratch array on h's GPU
-int cic_coarse(cube_handle* h, double* mass);
+int cic_coarse(cube_handle* h, double* mass, double* mass_dev = nullptr);
 // fine force components Ff[d] (row pitch pf[d]) and the coarse force fc
 // f2max_bits (optional): max |F|^2 over all fine nodes, from the staged slabs
+// new-sigma coefficient and table from h->dscal / h->vdt_new
 int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-             double c_new, uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f, double* probe_c);
+             uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f, double* probe_c);
 // pp.hip
 int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe);
+int pp_launch_step(cube_handle* h, float a_mid, float dt, double c_old, double* probe);
+int pp_finish(cube_handle* h, const StepReport* rp, cube_pp_out* out);
+int pp_step_finish(cube_handle* h, float a_mid, cube_pp_out& o);  // ftemp running max, dt_pp, itest1 check
+// PM launch sequence (results into h->drep) and its outputs, dt inputs and sigma update
+int pm_launch_step(cube_handle* h, float a_mid, float dt);
+int pm_finish_step(cube_handle* h, const StepReport* rp, float a_mid, cube_pm_out* out);
 // drift.hip
 int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out);
+// the drift's launch sequence (results into h->drep, state buffers swapped)
+// and its checks on a copied report
+int drift_launch_step(cube_handle* h, float dt_old, float dt, int* nlayer);
+int drift_finish(cube_handle* h, const StepReport* rp, int nlayer, cube_drift_out* out);
 // fluid.hip: PM coupling of the neutrino fluid
-int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added);
-int fl_density_fine(cube_handle* h, float a_mid, double* mass_added);
+int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added, double* mass_dev = nullptr);
+int fl_density_fine(cube_handle* h, float a_mid, double* mass_added, double* mass_dev = nullptr);
 int fl_gravity_fine(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt);
 // reuses the fine-force buffers Ff for the coarse acceleration at fine-cell centres
 int fl_gravity_finish(cube_handle* h, void* const Ff[3], const int64_t pf[3], float a_mid, float dt);

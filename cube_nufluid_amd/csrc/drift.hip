@@ -135,7 +135,12 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
           lmax[e] = max(lmax[e], ad);
         }
         if (far) {
+          // flagged: the drift fails and its result is discarded; the
+          // particle is counted in its own cell so that the later passes
+          // of the same launch sequence stay in bounds
           bad |= 1;
+          disp[p] = pack_disp(0, 0, 0);
+          rd = r;
         } else {
           disp[p] = pack_disp(del[0], del[1], del[2]);
           rd = rank_of(g, wrapc(c[0] + del[0], g.nc), wrapc(c[1] + del[1], g.nc), wrapc(c[2] + del[2], g.nc));
@@ -605,36 +610,60 @@ __global__ __launch_bounds__(DR_NT) void k_drift_write(Geo g, const uint64_t* __
                             vp_o, vfield_o, rhoc_o, part, scs, svf, red, svr);
 }
 
-int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
+// sigma_vi_new = sqrt(sum v_res^2 / N) / sqrt(3) (update_particle.f90:213,
+// in the host's operation order) and the new velocity coefficient
+// vcoef(sigma_vi_new) = sqrt(pi/2) / (sigma * 2.5), formed on the device so
+// that the kick can read them without a host round trip
+__global__ void k_sigma_new(const double* __restrict__ dsum, double npg, float sqrt3f, float sqrtpi2f, float* sigma_out,
+                            double* cnew_out) {
+  const float sg = (float)(sqrt(dsum[1] / npg) / (double)sqrt3f);
+  *sigma_out = sg;
+  *cnew_out = (double)sqrtpi2f / ((double)sg * 2.5);
+}
+__global__ void k_vtab_fill_dev(const float* __restrict__ dec, int n, const double* __restrict__ c, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)dec[i] / *c;
+}
+__global__ void k_set_double(double* p, double v) { *p = v; }
+
+int sigma_new_to_device(cube_handle* h) {
+  const int n = h->p.izipv == 1 ? 256 : 65536;
+  k_set_double<<<1, 1, 0, h->stream>>>(h->dscal, vcoef(h->sigma_vi_new));
+  k_vtab_fill_dev<<<(n + 255) / 256, 256, 0, h->stream>>>(h->vdec, n, h->dscal, h->vdt_new);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
+
+// update_particle.f90:57-226 as one launch sequence.  The results the host
+// needs (flags, npcheck, the sigma sums) are gathered in h->drep; the state
+// buffers are swapped at once and swapped back by drift_finish if the drift
+// failed.  Unless the handle is inside a deferred step, the report is
+// copied and checked before returning.
+static int drift_launch(cube_handle* h, float dt_old, float dt, int* nlayer_out) {
   Geo g = geo_of(h);
   const float dt_mid = (dt_old + dt) / 2;
   const int nlayer = 2 * (int)ceilf(dt_mid * h->vz_max / (float)CUBE_NCELL) + 1;
+  *nlayer_out = nlayer;
+  if (nlayer > 31) return cube_fail(h, CUBE_ERR_CAPACITY, "nlayer=%d above 31 (dt*vz_max too large)", nlayer);
   const double c_old = vcoef(h->sigma_vi);
   const double* vdt = vtab_for(h, c_old);
   const int dmax = (int)(h->ncb < DMAX ? h->ncb : DMAX);
   uint32_t* flags = h->uflags;  // [0..2] max disp, [3] too far / crowded, [4] frame conflicts, [6] crowded cells
-  HIP_TRY(h, hipMemsetAsync(flags, 0, sizeof(uint32_t) * 8, h->stream));
-  HIP_TRY(h, hipMemsetAsync(h->rhoc2, 0, sizeof(int32_t) * h->n3c, h->stream));
-  int64_t nb1 = (h->n3c + DR_CH - 1) / DR_CH;
+  const int64_t nb1 = (h->n3c + DR_CH - 1) / DR_CH;
+  const int64_t nb3 = (h->n3c + DR_NT - 1) / DR_NT;
+  const int64_t nbw = (h->n3c + DW_CELLS - 1) / DW_CELLS;
+  if (nb3 * 3 > h->dred_n - 16 || nbw * 3 > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
+  int r;
+  if ((r = fill_u32(h, flags, 8, 0u)) || (r = fill_u32(h, h->rhoc2, h->n3c, 0u))) return r;
   int pid1 = prof_begin(h, PS_DRIFT1);
   k_drift_pass1<<<(unsigned)nb1, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, dt_mid, c_old, vdt, dmax,
                                                          h->disp, h->rhoc2, flags);
   prof_end(h, pid1);
   HIP_TRY(h, hipGetLastError());
-  uint32_t hf[8];
-  HIP_TRY(h, hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  if (hf[3] & 1)
-    return cube_fail(h, CUBE_ERR_CAPACITY, "a particle moved more than %d coarse cells in one drift (buffer ncb=%lld)", dmax,
-                     (long long)h->ncb);
-  if (hf[3] & 2) return cube_fail(h, CUBE_ERR_CAPACITY, "a coarse cell holds more than 2^%d particles", DR_LBITS);
-  if (nlayer > 31) return cube_fail(h, CUBE_ERR_CAPACITY, "nlayer=%d above 31 (dt*vz_max too large)", nlayer);
   int pid2 = prof_begin(h, PS_DRIFT_SCAN);
-  int r = scan_counts(h, h->rhoc2, h->n3c, h->cstart2);
+  r = scan_counts(h, h->rhoc2, h->n3c, h->cstart2);
   prof_end(h, pid2);
   if (r) return r;
-  int64_t nb3 = (h->n3c + DR_NT - 1) / DR_NT;
-  if (nb3 * 3 > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   uint64_t* keys = (uint64_t*)h->order;
   uint32_t* nbig = flags + 6;
   int pid3 = prof_begin(h, PS_DRIFT_SCATTER);
@@ -648,39 +677,89 @@ int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
   prof_end(h, pid4);
   HIP_TRY(h, hipGetLastError());
   int pid5 = prof_begin(h, PS_DRIFT_WRITE);
-  const int64_t nbw = (h->n3c + DW_CELLS - 1) / DW_CELLS;
-  if (nbw * 3 > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   k_drift_write<<<(unsigned)nbw, DR_NT, 0, h->stream>>>(g, keys, h->cstart2, h->n3c, h->cstart, h->xp, h->vp, h->pid, h->vfield,
                                                          dt_mid, c_old, vdt, h->venc, h->vkmax, h->xp2, h->pid2,
                                                          (double*)h->dscratch, h->vp2, h->vfield2, h->rhoc2, h->dred);
   prof_end(h, pid5);
   HIP_TRY(h, hipGetLastError());
-  double sums[3];
-  if ((r = reduce_doubles(h, h->dred, nbw, 3, sums))) return r;
-  int64_t npnew;
-  HIP_TRY(h, hipMemcpy(&npnew, h->cstart2 + h->n3c, sizeof(int64_t), hipMemcpyDeviceToHost));
-  HIP_TRY(h, hipMemcpy(hf, flags, sizeof(hf), hipMemcpyDeviceToHost));
-  if (npnew != h->np) return cube_fail(h, CUBE_ERR_CHECK, "npcheck %lld != nplocal %lld", (long long)npnew, (long long)h->np);
+  StepReport* d = h->drep;
+  if ((r = reduce_doubles_dev(h, h->dred, nbw, 3, d->dsum))) return r;
+  const int n = h->p.izipv == 1 ? 256 : 65536;
+  k_sigma_new<<<1, 1, 0, h->stream>>>(d->dsum, (double)h->np, sqrtf(3.0f), sqrtf(PI_F / 2), &d->sigma_new, h->dscal);
+  k_vtab_fill_dev<<<(n + 255) / 256, 256, 0, h->stream>>>(h->vdec, n, h->dscal, h->vdt_new);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(&d->npnew, h->cstart2 + h->n3c, sizeof(int64_t), hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(d->dflags, flags, sizeof(d->dflags), hipMemcpyDeviceToDevice, h->stream));
   std::swap(h->xp, h->xp2);
   std::swap(h->vp, h->vp2);
   std::swap(h->pid, h->pid2);
   std::swap(h->rhoc, h->rhoc2);
   std::swap(h->vfield, h->vfield2);
   std::swap(h->cstart, h->cstart2);
+  return 0;
+}
+
+static void drift_unswap(cube_handle* h) {
+  std::swap(h->xp, h->xp2);
+  std::swap(h->vp, h->vp2);
+  std::swap(h->pid, h->pid2);
+  std::swap(h->rhoc, h->rhoc2);
+  std::swap(h->vfield, h->vfield2);
+  std::swap(h->cstart, h->cstart2);
+}
+
+// checks of update_particle.f90:90-96 and the npcheck of :150, on the
+// copied report; a failed drift leaves the state as it was
+int drift_finish(cube_handle* h, const StepReport* rp, int nlayer, cube_drift_out* out) {
+  const int dmax = (int)(h->ncb < DMAX ? h->ncb : DMAX);
+  const char* why = nullptr;
+  int code = 0;
+  char buf[160];
+  if (rp->dflags[3] & 1) {
+    snprintf(buf, sizeof buf, "a particle moved more than %d coarse cells in one drift (buffer ncb=%lld)", dmax,
+             (long long)h->ncb);
+    why = buf;
+    code = CUBE_ERR_CAPACITY;
+  } else if (rp->dflags[3] & 2) {
+    snprintf(buf, sizeof buf, "a coarse cell holds more than 2^%d particles", DR_LBITS);
+    why = buf;
+    code = CUBE_ERR_CAPACITY;
+  } else if (rp->npnew != h->np) {
+    snprintf(buf, sizeof buf, "npcheck %lld != nplocal %lld", (long long)rp->npnew, (long long)h->np);
+    why = buf;
+    code = CUBE_ERR_CHECK;
+  }
+  if (code) {
+    drift_unswap(h);
+    int r = sigma_new_to_device(h);  // the failed drift overwrote them
+    (void)r;
+    return cube_fail(h, code, "%s", why);
+  }
   const double npg = (double)h->np;
-  double std_vsim = sqrt(sums[2] / npg);
-  double std_vsim_c = sqrt(sums[0] / (double)h->nc / (double)h->nc / (double)h->nc);
-  double std_vsim_res = sqrt(sums[1] / npg);
-  h->sigma_vi_new = (float)(std_vsim_res / (double)sqrtf(3.0f));
+  double std_vsim = sqrt(rp->dsum[2] / npg);
+  double std_vsim_c = sqrt(rp->dsum[0] / (double)h->nc / (double)h->nc / (double)h->nc);
+  double std_vsim_res = sqrt(rp->dsum[1] / npg);
+  h->sigma_vi_new = rp->sigma_new;
   if (out) {
     out->sigma_vi_new = h->sigma_vi_new;
     out->nlayer = nlayer;
     out->std_vsim = std_vsim;
     out->std_vsim_c = std_vsim_c;
     out->std_vsim_res = std_vsim_res;
-    out->npcheck = npnew;
-    for (int e = 0; e < 3; e++) out->max_disp[e] = (int32_t)hf[e];
-    out->frame_conflicts = (int32_t)hf[4];
+    out->npcheck = rp->npnew;
+    for (int e = 0; e < 3; e++) out->max_disp[e] = (int32_t)rp->dflags[e];
+    out->frame_conflicts = (int32_t)rp->dflags[4];
   }
   return 0;
+}
+
+int drift_launch_step(cube_handle* h, float dt_old, float dt, int* nlayer) { return drift_launch(h, dt_old, dt, nlayer); }
+
+int drift_run(cube_handle* h, float dt_old, float dt, cube_drift_out* out) {
+  int nlayer = 0;
+  int r = drift_launch(h, dt_old, dt, &nlayer);
+  if (r) return r;
+  HIP_TRY(h, hipMemcpyAsync(h->rep, h->drep, sizeof(StepReport), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return drift_finish(h, h->rep, nlayer, out);
 }

@@ -530,6 +530,7 @@ static const float kGLw[3] = {0.71109301f, 0.27851773f, 0.01038926f};
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 extern "C" int cube_nu_init(cube_handle* h, const cube_nu_params* p) {
+  CUBE_EXCLUSIVE;
   if (!h || !p) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   if (!(p->m_ev > 0) || !(p->f_neu > 0) || !(p->gamma > 1))
@@ -584,6 +585,7 @@ extern "C" int cube_nu_info(cube_handle* h, int n, int32_t* nc, float* dx, float
 }
 
 extern "C" int cube_nu_set_fluid(cube_handle* h, int n, const float* fld) {
+  CUBE_EXCLUSIVE;
   if (!h || !h->nu.on || n < 0 || n > 2 || !fld) return CUBE_ERR_ARG;
   const int64_t n3 = (int64_t)h->nu.nc[n] * h->nu.nc[n] * h->nu.nc[n];
   HIP_TRY(h, hipMemcpy(h->nu.F[n], fld, sizeof(float) * 5 * n3, hipMemcpyHostToDevice));
@@ -591,6 +593,7 @@ extern "C" int cube_nu_set_fluid(cube_handle* h, int n, const float* fld) {
 }
 
 extern "C" int cube_nu_get_fluid(cube_handle* h, int n, float* fld) {
+  CUBE_EXCLUSIVE;
   if (!h || !h->nu.on || n < 0 || n > 2 || !fld) return CUBE_ERR_ARG;
   const int64_t n3 = (int64_t)h->nu.nc[n] * h->nu.nc[n] * h->nu.nc[n];
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -657,7 +660,7 @@ static FlSet fl_set(const NuState& s) {
 
 // Coarse-mesh density (before the coarse FFT); leaves the fine-cell-centre
 // density in fine.W for fl_density_fine.
-int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added) {
+int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added, double* mass_dev) {
   NuState& s = h->nu;
   const int homog = 1.0f / a_mid - 1.0f > s.par.z_i_nu;
   const int64_t nf = h->nf;
@@ -676,11 +679,12 @@ int fl_density_coarse(cube_handle* h, float a_mid, double* mass_added) {
     k_fl_add_coarse<double><<<(unsigned)nb, 256, 0, h->stream>>>((double*)h->coarse.R, D, nf, (int)h->nc, s.par.f_neu, homog,
                                                                  h->dred);
   HIP_TRY(h, hipGetLastError());
+  if (mass_dev) return reduce_doubles_dev(h, h->dred, nb, 1, mass_dev);
   if (mass_added) return reduce_doubles(h, h->dred, nb, 1, mass_added);
   return 0;
 }
 
-int fl_density_fine(cube_handle* h, float a_mid, double* mass_added) {
+int fl_density_fine(cube_handle* h, float a_mid, double* mass_added, double* mass_dev) {
   NuState& s = h->nu;
   const int homog = 1.0f / a_mid - 1.0f > s.par.z_i_nu;
   const int64_t nf = h->nf;
@@ -693,6 +697,7 @@ int fl_density_fine(cube_handle* h, float a_mid, double* mass_added) {
     k_fl_add_fine<double><<<(unsigned)nb, 256, 0, h->stream>>>((double*)h->fine.R, (const float*)h->fine.W, nf, s.par.f_neu,
                                                                homog, h->dred);
   HIP_TRY(h, hipGetLastError());
+  if (mass_dev) return reduce_doubles_dev(h, h->dred, nb, 1, mass_dev);
   if (mass_added) return reduce_doubles(h, h->dred, nb, 1, mass_added);
   return 0;
 }

@@ -825,6 +825,7 @@ struct Arena {
 }  // namespace
 
 extern "C" int cube_halofind(cube_handle* h, const cube_halo_params* hp, cube_halo_catalog* out) {
+  CUBE_EXCLUSIVE;
   if (!h || !hp || !out) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   if (!out->halos || !out->pid_words) return cube_fail(h, CUBE_ERR_ARG, "halo catalogue buffers missing");

@@ -180,6 +180,7 @@ __global__ void k_ic_encode(const int32_t* perm, const int16_t* cellc, const uin
 }
 
 extern "C" int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_sim_header* sim_out) {
+  CUBE_EXCLUSIVE;
   if (!h || !ip || !ip->lnk || !ip->lnp || ip->nk < 2 || ip->np_nc < 1) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   const int64_t ng = h->nc * ip->np_nc;
@@ -250,6 +251,7 @@ extern "C" int cube_ic_zeldovich(cube_handle* h, const cube_ic_params* ip, cube_
   h->npglobal = n;
   h->sigma_vi = sigma;
   h->sigma_vi_new = sigma;
+  if (int r = sigma_new_to_device(h)) return r;
   h->vz_max = vz_max;
   // initial_conditions.f90:943, real(4): f_cdm*nf_global**3/npglobal
   const float f_cdm = ip->f_cdm > 0 ? ip->f_cdm : 1.0f;
@@ -361,6 +363,7 @@ __global__ void k_nu_ic_finish(float* F, int nc, float cs2, float g) {
 }
 
 extern "C" int cube_nu_ic(cube_handle* h, const cube_ic_params* ip) {
+  CUBE_EXCLUSIVE;
   if (!h || !ip || !ip->lnk || !ip->lnp || ip->nk < 2 || ip->np_nc < 1) return CUBE_ERR_ARG;
   if (!h->nu.on) return cube_fail(h, CUBE_ERR_ARG, "cube_nu_ic needs cube_nu_init first");
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");

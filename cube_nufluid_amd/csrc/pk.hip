@@ -120,6 +120,7 @@ static int pk_run(cube_handle* h, int nbin, double* modes, double* ksum, double*
 }
 
 extern "C" int cube_power_spectrum(cube_handle* h, int nbin, double* modes, double* ksum, double* d2sum) {
+  CUBE_EXCLUSIVE;
   if (!h || !modes || !ksum || !d2sum) return CUBE_ERR_ARG;
   if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
   if (nbin < 1 || nbin > PK_MAXBIN) return cube_fail(h, CUBE_ERR_ARG, "nbin=%d outside [1, %d]", nbin, PK_MAXBIN);

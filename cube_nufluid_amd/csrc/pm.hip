@@ -218,7 +218,7 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
   }
 }
 
-int cic_fine(cube_handle* h, double* mass) {
+int cic_fine(cube_handle* h, double* mass, double* mass_dev) {
   Geo g = geo_of(h);
   if (h->nt % 4) return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
   int64_t nb = (h->nc / 4) * (h->nc / 4) * (h->nc / 4);
@@ -232,6 +232,7 @@ int cic_fine(cube_handle* h, double* mass) {
                                                                  h->fine.N + 2, h->dred);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
+  if (mass_dev) return reduce_doubles_dev(h, h->dred, nb, 1, mass_dev);
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
   return 0;
 }
@@ -246,7 +247,7 @@ int cic_halo_density(cube_handle* h, float* out) {
   HIP_TRY(h, hipGetLastError());
   return 0;
 }
-int cic_coarse(cube_handle* h, double* mass) {
+int cic_coarse(cube_handle* h, double* mass, double* mass_dev) {
   Geo g = geo_of(h);
   int64_t nb = (h->nc / 8) * (h->nc / 8) * (h->nc / 8);
   int pr = prof_begin(h, PS_CIC_C);
@@ -256,6 +257,7 @@ int cic_coarse(cube_handle* h, double* mass) {
     k_cic_coarse<8, double><<<(unsigned)nb, CIC_NT, 0, h->stream>>>(g, h->xp, h->cstart, h->mass_p, (double*)h->coarse.R, h->dred);
   prof_end(h, pr);
   HIP_TRY(h, hipGetLastError());
+  if (mass_dev) return reduce_doubles_dev(h, h->dred, nb, 1, mass_dev);
   if (mass) return reduce_doubles(h, h->dred, nb, 1, mass);
   return 0;
 }
@@ -289,7 +291,7 @@ struct KickMeshes {
 template <typename T, int BX, int NT>
 __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                                   int64_t n3c, const float* __restrict__ vfield, KickMeshes<T> F, int64_t Nf,
-                                                  float a_mid, float dt, double c_old, double c_new,
+                                                  float a_mid, float dt, double c_old, const double* __restrict__ cnew_p,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
                                                   const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f,
@@ -305,6 +307,7 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
   int32_t c0[3];
   cell_of_rank(g, r0, c0);
   const int tid = threadIdx.x;
+  const double c_new = *cnew_p;  // vcoef(sigma_vi_new), formed on the device by the drift
   const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
   for (int i = tid; i <= BX; i += NT) cs[i] = cstart[r0 + i];
   const int64_t p0 = cstart[r0], p1 = cstart[r0 + BX];
@@ -533,7 +536,7 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
 
 template <typename T, int BX, int NT>
 static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-                        double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
+                        const double* c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
                         double* probe_f, double* probe_c) {
   Geo g = geo_of(h);
   KickMeshes<T> F;
@@ -549,7 +552,7 @@ static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t p
 
 template <typename T>
 static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-                     double c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
+                     const double* c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
                      double* probe_f, double* probe_c) {
   if (h->nt % 32 == 0)
     kick_launch<T, 32, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
@@ -563,9 +566,10 @@ static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3
 }
 
 int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-             double c_new, uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f, double* probe_c) {
+             uint32_t* vmax_bits, uint32_t* f2max_bits, double* probe_f, double* probe_c) {
   const double* vo = vtab_for(h, c_old);
-  const double* vn = vtab_for(h, c_new);
+  const double* c_new = h->dscal;  // and its table h->vdt_new, both written by the drift
+  const double* vn = h->vdt_new;
   int pr = prof_begin(h, PS_KICK);
   int r = h->prec == 4
               ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c)

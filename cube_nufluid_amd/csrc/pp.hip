@@ -44,7 +44,8 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
                                               float mass, float rsoft, int pp_range, float a_mid, float dt, double c_old,
                                               const double* __restrict__ vdt, const double* __restrict__ venc, int32_t kmax,
                                               uint32_t* f2max_bits, unsigned long long* npairs_out,
-                                              unsigned long long* nupd_out, double* probe, const int32_t* blist, int bscale) {
+                                              unsigned long long* nupd_out, double* probe, const int32_t* blist,
+                                              const uint32_t* __restrict__ nblist, int bscale) {
   constexpr int SPAN = PP_BP + 2, NCELL = SPAN * SPAN * SPAN, FB = 4 * PP_BP;
   __shared__ int64_t beg[NCELL];
   __shared__ int32_t off[NCELL + 1];
@@ -52,20 +53,22 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
   __shared__ int ncand;
   __shared__ int32_t ioff[PP_BP * PP_BP * PP_BP + 1];
   __shared__ int64_t ibeg[PP_BP * PP_BP * PP_BP];
-  // blocks of PP_BP^3 cells; with a block list, entry blockIdx.x/bscale^3 names
-  // a coarser block of (bscale*PP_BP)^3 cells and blockIdx.x%bscale^3 the sub-block
+  // blocks of PP_BP^3 cells: entry vb/bscale^3 of the block list (its length
+  // *nblist is read on the device, so the launch needs no host round trip)
+  // names a coarser block of (bscale*PP_BP)^3 cells and vb%bscale^3 the
+  // sub-block; the workgroups stride over the list
+  const int nsub = bscale * bscale * bscale;
+  const uint32_t nvb = *nblist * (uint32_t)nsub;
+  float f2loc = 0.f;
+  unsigned long long npl = 0, ntarg = 0;
+  for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
   int c0x, c0y, c0z;
-  if (blist) {
-    const int nsub = bscale * bscale * bscale, big = blist[blockIdx.x / nsub], sub = blockIdx.x % nsub;
+  {
+    const int big = blist[vb / nsub], sub = (int)(vb % nsub);
     const int nbb = g.nc / (PP_BP * bscale);
     c0x = (big % nbb) * PP_BP * bscale + (sub % bscale) * PP_BP;
     c0y = ((big / nbb) % nbb) * PP_BP * bscale + ((sub / bscale) % bscale) * PP_BP;
     c0z = (big / (nbb * nbb)) * PP_BP * bscale + (sub / (bscale * bscale)) * PP_BP;
-  } else {
-    const int nbx = g.nc / PP_BP;
-    c0x = (blockIdx.x % nbx) * PP_BP;
-    c0y = ((blockIdx.x / nbx) % nbx) * PP_BP;
-    c0z = (blockIdx.x / (nbx * nbx)) * PP_BP;
   }
   // region cells [c0-1, c0+PP_BP], inner cells [c0, c0+PP_BP)
   for (int i = threadIdx.x; i < NCELL; i += PP_NT) {
@@ -92,8 +95,7 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
   __syncthreads();
   const int32_t ntot = off[NCELL], ninner = ioff[PP_BP * PP_BP * PP_BP];
   const int lo_fc = -pp_range, hi_fc = FB - 1 + pp_range;
-  float f2loc = 0.f;
-  unsigned long long npl = 0;
+  ntarg += (unsigned long long)ninner;
   for (int32_t ib = 0; ib < ninner; ib += PP_NT) {
     const int32_t it = ib + threadIdx.x;
     const bool act = it < ninner;
@@ -175,15 +177,17 @@ __global__ __launch_bounds__(PP_NT) void k_pp(Geo g, const void* __restrict__ xp
       f2loc = fmaxf(f2loc, (float)(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
     }
   }
+  __syncthreads();  // the next block overwrites the staged tables
+  }
   for (int o = 32; o > 0; o >>= 1) {
     f2loc = fmaxf(f2loc, __shfl_xor(f2loc, o));
     npl += __shfl_xor(npl, o);
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && (npl || ntarg)) {
     const int sl = blockIdx.x & 1023;
     atomicMax(f2max_bits + sl, __float_as_uint(f2loc));
     atomicAdd(npairs_out + sl, npl);
-    atomicAdd(nupd_out + sl, (unsigned long long)ninner);
+    atomicAdd(nupd_out + sl, ntarg);
   }
 }
 
@@ -858,7 +862,44 @@ static int pp_launch(cube_handle* h, const Geo& g, const PPctx& c, uint32_t* f2,
   return 0;
 }
 
-int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe) {
+// per-slot totals summed on the device into the step report: pairs and
+// targets (integer sums) and the f2 maximum (positive floats order as
+// their bit patterns)
+__global__ __launch_bounds__(256) void k_pp_totals(const unsigned long long* __restrict__ npr,
+                                                   const unsigned long long* __restrict__ nup,
+                                                   const uint32_t* __restrict__ f2, StepReport* d) {
+  __shared__ unsigned long long sp[256], st[256];
+  __shared__ uint32_t sf[256];
+  unsigned long long a = 0, b = 0;
+  uint32_t m = 0;
+  for (int i = threadIdx.x; i < PP_SLOTS; i += 256) {
+    a += npr[i];
+    b += nup[i];
+    m = max(m, f2[i]);
+  }
+  sp[threadIdx.x] = a;
+  st[threadIdx.x] = b;
+  sf[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sp[threadIdx.x] += sp[threadIdx.x + o];
+      st[threadIdx.x] += st[threadIdx.x + o];
+      sf[threadIdx.x] = max(sf[threadIdx.x], sf[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d->pp_pairs = sp[0];
+    d->pp_targets = st[0];
+    d->pp_f2 = sf[0];
+  }
+}
+
+// ext_pp_force as one launch sequence: fast kernel, dense prep and pairs,
+// and the chunked kernel over whatever the pool could not hold (its list
+// length is read on the device); totals and flags into h->drep
+int pp_launch_step(cube_handle* h, float a_mid, float dt, double c_old, double* probe) {
   Geo g = geo_of(h);
   const int prange = h->p.pp_range;
   if (prange < 1 || prange > 4)
@@ -874,8 +915,9 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
   unsigned long long* npr = (unsigned long long*)h->pp_red;
   unsigned long long* nup = npr + PP_SLOTS;
   uint32_t* f2 = (uint32_t*)(nup + PP_SLOTS);
-  HIP_TRY(h, hipMemsetAsync(h->uflags + 8, 0, sizeof(uint32_t) * 12, h->stream));
-  HIP_TRY(h, hipMemsetAsync(h->pp_red, 0, PP_SLOTS * (2 * sizeof(unsigned long long) + sizeof(uint32_t)), h->stream));
+  int rc;
+  if ((rc = fill_u32(h, h->uflags + 8, 12, 0u))) return rc;
+  if ((rc = fill_u32(h, h->pp_red, PP_SLOTS * (2 * sizeof(unsigned long long) + sizeof(uint32_t)) / 4, 0u))) return rc;
   // pp_meta: per 2^3-cell block (the finer of the two block sizes) 2 int64 of
   // info and 2 int32 list entries, plus the work items
   const int64_t nb2 = (h->nc / 2) * (h->nc / 2) * (h->nc / 2);
@@ -885,44 +927,40 @@ int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out
   int32_t* fall = dense + nb2;
   PPctx c{h->mass_p, h->p.rsoft, a_mid, dt, c_old, vtab_for(h, c_old), h->venc, h->vkmax, prange};
   int pr = prof_begin(h, PS_PP);
-  int rc;
 #define PP_ARGS h, g, c, f2, npr, nup, probe, dense, ndense, info, items, nitems, fall, nfall, pool_used
   if (h->prec == 4) rc = PBsel == 4 ? pp_launch<4, float>(PP_ARGS) : pp_launch<2, float>(PP_ARGS);
   else rc = PBsel == 4 ? pp_launch<4, double>(PP_ARGS) : pp_launch<2, double>(PP_ARGS);
 #undef PP_ARGS
   if (rc) return rc;
-  uint32_t hv[12];
-  HIP_TRY(h, hipMemcpyAsync(hv, h->uflags + 8, sizeof(hv), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  const uint32_t nfb = hv[3];
-  if (nfb) {  // blocks the pool could not hold: chunked kernel on their PP_BP^3 sub-blocks
-    const int bscale = PBsel / PP_BP, nsub = bscale * bscale * bscale;
-    k_pp<<<(unsigned)((int64_t)nfb * nsub), PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, prange, a_mid,
-                                                                   dt, c_old, c.vdt, h->venc, h->vkmax, f2, npr, nup, probe,
-                                                                   fall, bscale);
-    HIP_TRY(h, hipGetLastError());
-  }
+  // blocks the pool could not hold: chunked kernel on their PP_BP^3 sub-blocks
+  const int bscale = PBsel / PP_BP;
+  k_pp<<<256, PP_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->mass_p, h->p.rsoft, prange, a_mid, dt, c_old, c.vdt, h->venc,
+                                     h->vkmax, f2, npr, nup, probe, fall, nfall, bscale);
+  HIP_TRY(h, hipGetLastError());
   prof_end(h, pr);
-  std::vector<unsigned long long> red(2 * PP_SLOTS);
-  std::vector<uint32_t> f2b(PP_SLOTS);
-  HIP_TRY(h, hipMemcpyAsync(red.data(), npr, sizeof(unsigned long long) * 2 * PP_SLOTS, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipMemcpyAsync(f2b.data(), f2, sizeof(uint32_t) * PP_SLOTS, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(h, hipStreamSynchronize(h->stream));
-  unsigned long long npairs = 0, nupd = 0;
-  uint32_t f2m = 0;
-  for (int i = 0; i < PP_SLOTS; i++) {
-    npairs += red[i];
-    nupd += red[PP_SLOTS + i];
-    f2m = std::max(f2m, f2b[i]);  // non-negative floats order as their bit patterns
-  }
+  k_pp_totals<<<1, 256, 0, h->stream>>>(npr, nup, f2, h->drep);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(h->drep->ppflags, h->uflags + 8, sizeof(h->drep->ppflags), hipMemcpyDeviceToDevice, h->stream));
+  return 0;
+}
+
+int pp_finish(cube_handle* h, const StepReport* rp, cube_pp_out* out) {
   float f2max;
-  memcpy(&f2max, &f2m, 4);
+  memcpy(&f2max, &rp->pp_f2, 4);
   if (out) {
     out->f2_max_pp = f2max;
-    out->npairs = (int64_t)npairs;
-    out->itest1 = (int64_t)nupd;
-    out->dense_blocks = (int64_t)hv[1];
+    out->npairs = (int64_t)rp->pp_pairs;
+    out->itest1 = (int64_t)rp->pp_targets;
+    out->dense_blocks = (int64_t)rp->ppflags[1];
   }
-  h->pp_overflow_blocks = nfb;
+  h->pp_overflow_blocks = rp->ppflags[3];
   return 0;
+}
+
+int pp_run(cube_handle* h, float a_mid, float dt, double c_old, cube_pp_out* out, double* probe) {
+  int r = pp_launch_step(h, a_mid, dt, c_old, probe);
+  if (r) return r;
+  HIP_TRY(h, hipMemcpyAsync(h->rep, h->drep, sizeof(StepReport), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return pp_finish(h, h->rep, out);
 }

@@ -31,6 +31,7 @@ __global__ __launch_bounds__(256) void k_stream_copy(const v4f* __restrict__ src
 }
 
 extern "C" int cube_hbm_copy_probe(int device, int64_t bytes, int iters, double* gbps) {
+  CUBE_EXCLUSIVE;
   if (bytes < (1 << 20) || bytes % 16 || iters < 1 || !gbps) return CUBE_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return CUBE_ERR_HIP;
   hipDeviceProp_t prop;

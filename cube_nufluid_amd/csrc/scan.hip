@@ -101,10 +101,15 @@ __global__ void k_reduce_fields(const double* part, int64_t nparts, int nfields,
   }
   if (threadIdx.x == 0) out[f] = tmp[0];
 }
+int reduce_doubles_dev(cube_handle* h, const double* part, int64_t nparts, int nfields, double* dev_out) {
+  k_reduce_fields<<<nfields, SCAN_NT, 0, h->stream>>>(part, nparts, nfields, dev_out);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
 int reduce_doubles(cube_handle* h, const double* part, int64_t nparts, int nfields, double* host_out) {
   double* dout = h->dred + h->dred_n - 16;  // tail of the partials buffer
-  k_reduce_fields<<<nfields, SCAN_NT, 0, h->stream>>>(part, nparts, nfields, dout);
-  HIP_TRY(h, hipGetLastError());
+  int r = reduce_doubles_dev(h, part, nparts, nfields, dout);
+  if (r) return r;
   HIP_TRY(h, hipMemcpyAsync(host_out, dout, sizeof(double) * nfields, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return 0;

@@ -22,6 +22,11 @@
 #include <thread>
 #include <vector>
 
+std::shared_mutex& capture_mutex() {
+  static std::shared_mutex mu;
+  return mu;
+}
+
 // ------------------------------------------------------------------ clock
 extern "C" int cube_clock_init(cube_clock* c, const float* z_checkpoint, int n) {
   if (!c || !z_checkpoint || n < 1 || n > CUBE_MAX_CHECKPOINTS) return CUBE_ERR_ARG;
@@ -82,6 +87,90 @@ static void clock_timestep(cube_handle* h, cube_clock* c) {
   c->dt_neu = 1000.0f;
 }
 
+extern "C" int cube_set_step_mode(cube_handle* h, int mode) {
+  if (!h || mode < 0 || mode > 2) return CUBE_ERR_ARG;
+  h->step_mode = mode;
+  return CUBE_OK;
+}
+
+extern "C" int cube_step_stats(cube_handle* h, int64_t* graph_instantiations, int64_t* graph_updates) {
+  if (!h) return CUBE_ERR_ARG;
+  if (graph_instantiations) *graph_instantiations = h->graph_instantiations;
+  if (graph_updates) *graph_updates = h->graph_updates;
+  return CUBE_OK;
+}
+
+// Drift, PP and PM of one step launched back to back (step modes 1 and 2):
+// every host decision they need is known before the step (the new-sigma
+// coefficient travels on the device), the error checks are flags, and the
+// results come back in one copy of the step report, so the host waits once.
+// Mode 2 records the launches as a hipGraph and replays it, updating the
+// instantiated graph's kernel arguments (time-step values, ping-pong
+// buffers) when the launch sequence matches the previous step's.
+static int step_deferred(cube_handle* h, cube_clock* c) {
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  const bool graph = h->step_mode == 2 && !h->prof_on;
+  std::shared_lock<std::shared_mutex> capture_lk(capture_mutex(), std::defer_lock);
+  if (graph) {
+    capture_lk.lock();
+    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+  }
+  int nlayer = 0;
+  int r = drift_launch_step(h, c->dt_old, c->dt, &nlayer);
+  const bool swapped = r == 0;
+  if (!r && h->p.pp_on) r = pp_launch_step(h, c->a_mid, c->dt, vcoef(h->sigma_vi), nullptr);
+  if (!r) r = pm_launch_step(h, c->a_mid, c->dt);
+  if (!r && hipMemcpyAsync(h->rep, h->drep, sizeof(StepReport), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    r = cube_fail(h, CUBE_ERR_HIP, "step report copy failed");
+  hipGraph_t g = nullptr;
+  if (graph) {
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    capture_lk.unlock();
+    if (!r && e != hipSuccess) r = cube_fail(h, CUBE_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  }
+  if (!r && graph) {
+    if (h->gexec) {
+      hipGraphNode_t en = nullptr;
+      hipGraphExecUpdateResult res;
+      if (hipGraphExecUpdate(h->gexec, g, &en, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+        h->graph_updates++;
+      } else {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+      }
+    }
+    if (!h->gexec) {
+      if (hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+        r = cube_fail(h, CUBE_ERR_HIP, "hipGraphInstantiate failed");
+      else
+        h->graph_instantiations++;
+    }
+    if (!r && hipGraphLaunch(h->gexec, h->stream) != hipSuccess) r = cube_fail(h, CUBE_ERR_HIP, "hipGraphLaunch failed");
+  }
+  if (g) (void)hipGraphDestroy(g);
+  if (hipStreamSynchronize(h->stream) != hipSuccess && !r) r = cube_fail(h, CUBE_ERR_HIP, "hipStreamSynchronize failed");
+  if (r) {
+    // nothing of the step is kept: the drift's buffers are swapped back
+    if (swapped) {
+      std::swap(h->xp, h->xp2);
+      std::swap(h->vp, h->vp2);
+      std::swap(h->pid, h->pid2);
+      std::swap(h->rhoc, h->rhoc2);
+      std::swap(h->vfield, h->vfield2);
+      std::swap(h->cstart, h->cstart2);
+    }
+    return r;
+  }
+  if ((r = drift_finish(h, h->rep, nlayer, nullptr))) return r;
+  if (h->p.pp_on) {
+    cube_pp_out o{};
+    pp_finish(h, h->rep, &o);
+    if ((r = pp_step_finish(h, c->a_mid, o))) return r;
+  }
+  return pm_finish_step(h, h->rep, c->a_mid, nullptr);
+}
+
 extern "C" int cube_step(cube_handle* h, cube_clock* c) {
   if (!h || !c) return CUBE_ERR_ARG;
   if (c->final_step && c->checkpoint_step) return CUBE_OK;  // already past the last checkpoint
@@ -90,9 +179,13 @@ extern "C" int cube_step(cube_handle* h, cube_clock* c) {
   int r;
   // main.f90:51-55, 66-70: the fluid is evolved half a step on each side
   if (h->nu.on && (r = cube_fluid_evolve(h, c->dt / 2, 1, c->a_mid, &c->dt_neu))) return r;
-  if ((r = cube_drift(h, c->dt_old, c->dt, nullptr))) return r;
-  if (h->p.pp_on && (r = cube_pp(h, c->a_mid, c->dt, nullptr))) return r;
-  if ((r = cube_pm(h, c->a_mid, c->dt, nullptr))) return r;
+  if (h->step_mode > 0 && !h->nu.on) {
+    if ((r = step_deferred(h, c))) return r;
+  } else {
+    if ((r = cube_drift(h, c->dt_old, c->dt, nullptr))) return r;
+    if (h->p.pp_on && (r = cube_pp(h, c->a_mid, c->dt, nullptr))) return r;
+    if ((r = cube_pm(h, c->a_mid, c->dt, nullptr))) return r;
+  }
   if (h->nu.on && (r = cube_fluid_evolve(h, c->dt / 2, -1, c->a_mid, &c->dt_neu))) return r;
   if (c->checkpoint_step) {
     c->dt_old = 0;  // main.f90:74-75: the half drift closing the step
@@ -178,6 +271,7 @@ static void run_box(cube_ensemble* e, Request rq, double t_admit) {
   res.t_admit = t_admit;
   cube_handle* h = nullptr;
   int rc = cube_init(&rq.p, rq.kf.data(), rq.kc.data(), &h);
+  if (!rc) rc = cube_set_step_mode(h, e->ep.step_mode);
   if (!rc) {
     rq.ic.lnk = rq.lnk.data();
     rq.ic.lnp = rq.lnp.data();
@@ -241,7 +335,7 @@ static void schedule(cube_ensemble* e) {
 }
 
 extern "C" int cube_ens_create(const cube_ens_params* p, cube_ensemble** out) {
-  if (!p || !out || p->max_live < 1) return CUBE_ERR_ARG;
+  if (!p || !out || p->max_live < 1 || p->step_mode < 0 || p->step_mode > 2) return CUBE_ERR_ARG;
   cube_ensemble* e = new cube_ensemble();
   e->ep = *p;
   if (e->ep.mem_budget <= 0) {

@@ -3,9 +3,11 @@
 Requests are seeded LCDM PM+PP boxes of uneven size started from
 Zel'dovich initial conditions; the engine admits each as soon as its device
 footprint fits and runs all live boxes concurrently, one HIP stream and one
-host thread each.  `run_ensemble` replays a list of requests with their
-arrival times and returns per-request latencies, the aggregate
-particle-steps/s and each box's final-state hash.
+host thread each, each step either as synchronising phases or launched
+whole (one synchronisation per step, optionally replayed as a hipGraph).
+`run_ensemble` replays a list of requests with their arrival times and
+returns per-request latencies, the aggregate particle-steps/s and each
+box's final-state hash.
 """
 import ctypes
 import time
@@ -37,10 +39,10 @@ def poisson_requests(n, rate, seed, sizes=C5_SIZES, z_i=49.0, z_f=0.0):
 
 
 class Ensemble:
-    def __init__(self, max_live=64, mem_budget=0, device=0):
+    def __init__(self, max_live=64, mem_budget=0, device=0, step_mode=0):
         self.lib = _lib.load()
         ep = _lib.EnsParams()
-        ep.device, ep.max_live, ep.mem_budget = device, max_live, mem_budget
+        ep.device, ep.max_live, ep.mem_budget, ep.step_mode = device, max_live, mem_budget, step_mode
         e = ctypes.c_void_p()
         rc = self.lib.cube_ens_create(ctypes.byref(ep), ctypes.byref(e))
         if rc:
@@ -91,11 +93,13 @@ class Ensemble:
             self.e = None
 
 
-def run_ensemble(requests, max_live=64, mem_budget=0, log=None):
+def run_ensemble(requests, max_live=64, mem_budget=0, log=None, step_mode=0):
     """Submit `requests` at their arrival times (seconds after start) and
     wait for all of them.  Returns a dict with per-request results and the
-    aggregate metrics."""
-    ens = Ensemble(max_live=max_live, mem_budget=mem_budget)
+    aggregate metrics.  step_mode: cube_set_step_mode of every box (0 each
+    phase synchronises, 1 one synchronisation per step, 2 steps replayed as
+    hipGraphs)."""
+    ens = Ensemble(max_live=max_live, mem_budget=mem_budget, step_mode=step_mode)
     t0 = time.perf_counter()
     ids = {}
     results = {}

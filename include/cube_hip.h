@@ -291,6 +291,18 @@ int cube_clock_init(cube_clock* c, const float* z_checkpoint, int n);
    (main.f90:74-75).  After the final checkpoint, c->final_step and
    c->checkpoint_step are both set. */
 int cube_step(cube_handle* h, cube_clock* c);
+/* How cube_step runs drift, PP and PM (no counterpart in the reference,
+   whose loop body calls them one after another; for callers that step
+   many small boxes, where host round trips and launch costs dominate):
+   0 (default) each phase synchronises as the separate calls do; 1 the
+   three are launched back to back, their checks made on flags and their
+   results read back after one synchronisation per step; 2 as 1, with the
+   launches captured into a hipGraph whose kernel arguments are updated
+   each step.  Every mode gives the same bits.  Boxes with the neutrino
+   fluid always run mode 0. */
+int cube_set_step_mode(cube_handle* h, int mode);
+/* hipGraph instantiations and in-place updates made by step mode 2 so far. */
+int cube_step_stats(cube_handle* h, int64_t* graph_instantiations, int64_t* graph_updates);
 /* Device bytes a box of these parameters holds (plus the Zel'dovich
    generator's temporary mesh when ic_np_nc > 0), as used for admission. */
 int64_t cube_footprint(const cube_params* p, int32_t ic_np_nc);
@@ -305,6 +317,8 @@ typedef struct cube_ens_params {
   int32_t device;
   int32_t max_live;     /* concurrent boxes                                  */
   int64_t mem_budget;   /* device bytes for resident boxes; <= 0: 90% free  */
+  int32_t step_mode;    /* cube_set_step_mode for every box                  */
+  int32_t reserved;
 } cube_ens_params;
 typedef struct cube_ens_result {
   int64_t id;

@@ -36,7 +36,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.PPOut) == 32
     assert ctypes.sizeof(_lib.PMOut) == 56
     assert ctypes.sizeof(_lib.CubeClock) == 320
-    assert ctypes.sizeof(_lib.EnsParams) == 16
+    assert ctypes.sizeof(_lib.EnsParams) == 24
     assert ctypes.sizeof(_lib.EnsResult) == 192
     from cube_nufluid_amd import halo
     assert ctypes.sizeof(halo.cube_halo_params) == 80
