@@ -112,6 +112,7 @@ SIGNATURES = {
     "cube_probe_pm_forces": (c_i32, [_P, _P, _P]),
     "cube_probe_pp_forces": (c_i32, [_P, _P]),
     "cube_probe_density": (c_i32, [_P, _P]),
+    "cube_probe_vencode": (c_i32, [_P, _P, ctypes.c_int64, _P]),
     "cube_power_spectrum": (c_i32, [_P, c_i32, _P, _P, _P]),
     "cube_power_spectrum_noalloc": (c_i32, [_P, c_i32, _P, _P, _P]),
     "cube_halofind": (c_i32, [_P, _P, _P]),

@@ -79,6 +79,38 @@ static const CodecTables& codec_tables(int izipv) {
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ __launch_bounds__(256) void k_slots_reduce(const uint32_t* __restrict__ slots, int nf, uint32_t or_mask,
+                                                      uint32_t add_mask, uint32_t* out) {
+  __shared__ uint32_t red[256];
+  for (int f = 0; f < nf; f++) {
+    const bool is_or = (or_mask >> f) & 1, is_add = (add_mask >> f) & 1;
+    uint32_t a = 0;
+    for (int s = threadIdx.x; s < MSLOTS; s += 256) {
+      const uint32_t v = slots[s * MSLOT_FIELDS + f];
+      a = is_or ? (a | v) : (is_add ? a + v : max(a, v));
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        const uint32_t b = red[threadIdx.x + o];
+        red[threadIdx.x] = is_or ? (red[threadIdx.x] | b) : (is_add ? red[threadIdx.x] + b : max(red[threadIdx.x], b));
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (is_or) atomicOr(out + f, red[0]);
+      else if (is_add) atomicAdd(out + f, red[0]);
+      else atomicMax(out + f, red[0]);
+    }
+    __syncthreads();
+  }
+}
+int slots_reduce(cube_handle* h, int nf, uint32_t or_mask, uint32_t add_mask, uint32_t* out) {
+  k_slots_reduce<<<1, 256, 0, h->stream>>>(h->mslots, nf, or_mask, add_mask, out);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
+}
 int fill_u32(cube_handle* h, void* p, int64_t n, uint32_t v) {
   const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
   k_fill_u32<<<(unsigned)std::max<int64_t>(nb, 1), 256, 0, h->stream>>>((uint32_t*)p, n, v);
@@ -115,6 +147,8 @@ int codec_build(cube_handle* h) {
   HIP_TRY(h, hipMalloc(&h->vdt_new, sizeof(double) * t.dec.size()));
   HIP_TRY(h, hipMalloc(&h->dscal, sizeof(double) * 4));
   HIP_TRY(h, hipMalloc(&h->drep, sizeof(StepReport)));
+  HIP_TRY(h, hipMalloc(&h->dred_l1, sizeof(double) * 256 * 8));  // first-level sums (scan.hip)
+  HIP_TRY(h, hipMalloc(&h->mslots, sizeof(uint32_t) * MSLOTS * MSLOT_FIELDS));
   HIP_TRY(h, hipHostMalloc(&h->rep, sizeof(StepReport), hipHostMallocDefault));
   HIP_TRY(h, hipMemset(h->drep, 0, sizeof(StepReport)));
   memset(h->rep, 0, sizeof(StepReport));
@@ -269,7 +303,7 @@ extern "C" int cube_finalize(cube_handle* h) {
   if (h->rep) (void)hipHostFree(h->rep);
   void* bufs[] = {h->xp, h->vp, h->pid, h->xp2, h->vp2, h->pid2, h->disp, h->order, h->rhoc, h->rhoc2, h->vfield, h->vfield2,
                   h->cstart, h->cstart2, h->fc, h->dred, h->uflags, h->scan_tmp, h->vdec, h->venc, h->vdt, h->dbig,
-                  h->dscratch_own ? h->dscratch : nullptr, h->pp_meta, h->pp_red, h->vdt_new, h->dscal, h->drep};
+                  h->dscratch_own ? h->dscratch : nullptr, h->pp_meta, h->pp_red, h->vdt_new, h->dscal, h->drep, h->dred_l1, h->mslots};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   mesh_free(h->fine);
@@ -553,6 +587,32 @@ extern "C" int cube_probe_pp_forces(cube_handle* h, double* f_pp) {
   return r;
 }
 
+__global__ void k_probe_venc(const double* thr, int32_t kmax, int izipv, const double* x, int64_t n, int32_t* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = venc_fast(thr, kmax, izipv, x[i]);
+}
+extern "C" int cube_probe_vencode(cube_handle* h, const double* x, int64_t n, int32_t* codes) {
+  CUBE_EXCLUSIVE;
+  if (!h || n < 0 || (n > 0 && (!x || !codes))) return CUBE_ERR_ARG;
+  if (n == 0) return CUBE_OK;
+  if (hipSetDevice(h->p.device) != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, "hipSetDevice failed");
+  double* dx = nullptr;
+  int32_t* dc = nullptr;
+  hipError_t e = hipMalloc(&dx, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(&dc, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+    k_probe_venc<<<(unsigned)nb, 256, 0, h->stream>>>(h->venc, h->vkmax, (int)h->p.izipv, dx, n, dc);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(codes, dc, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+  hipFree(dx);
+  hipFree(dc);
+  if (e != hipSuccess) return cube_fail(h, CUBE_ERR_HIP, hipGetErrorString(e));
+  return CUBE_OK;
+}
 extern "C" int cube_probe_density(cube_handle* h, float* rho) {
   CUBE_EXCLUSIVE;
   if (!h || !rho) return CUBE_ERR_ARG;

@@ -109,6 +109,11 @@ struct cube_handle {
   // reductions / flags
   double* dred = nullptr;     // double partials
   int64_t dred_n = 0;
+  double* dred_l1 = nullptr;  // first-level sums of large reductions
+  // per-workgroup maxima, ORs and counts spread over MSLOTS slots, so that
+  // millions of workgroups do not meet on one address; slots_reduce folds
+  // them into the flags
+  uint32_t* mslots = nullptr;
   uint32_t* uflags = nullptr; // small device scratch for maxima and flags
   int64_t* scan_tmp = nullptr;
   int64_t scan_tmp_n = 0;
@@ -174,6 +179,12 @@ int sigma_new_to_device(cube_handle* h);
 // fill n 32-bit words with v on the handle's stream (a kernel, so that a
 // captured step holds no memset node)
 int fill_u32(cube_handle* h, void* p, int64_t n, uint32_t v);
+#define MSLOTS 1024
+#define MSLOT_FIELDS 8
+// out[f] (f < nf) combined with the fold of field f over the MSLOTS slots
+// (slot s, field f at slots[s * MSLOT_FIELDS + f]): OR where bit f of
+// or_mask is set, sum where bit f of add_mask is, max otherwise
+int slots_reduce(cube_handle* h, int nf, uint32_t or_mask, uint32_t add_mask, uint32_t* out);
 // While one thread captures a step into a hipGraph, no other thread may run
 // a legacy-stream operation (synchronous copies and memsets) or a
 // device-synchronising one (hipFree): the runtime rejects them.  Captures
@@ -261,32 +272,19 @@ __device__ __forceinline__ double nint_d(double y) {
 __device__ __forceinline__ double vdecode(const double* dtab, int32_t v, int izipv) {
   return dtab[v + (izipv == 1 ? 128 : 32768)];
 }
-// velocity encode with host-libm-exact thresholds.  thr[k + kmax - 1] is the
-// smallest x with enc(x) >= k, for k in [-kmax+1, kmax].  An FP32 atan gives
-// the code to within one step; the thresholds settle it exactly.
+// velocity encode, equal to the host encoder's code for every argument.
+// thr[k + kmax - 1] is the smallest x with enc(x) >= k, for k in
+// [-kmax+1, kmax]; venc_fast (below) reads it only near ties.
+__device__ __forceinline__ int32_t venc_fast(const double* thr, int32_t kmax, int izipv, double x);
 __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int izipv, double c, double v) {
-  const float nv1 = izipv == 1 ? 255.0f : 65535.0f;
-  const double x = c * v;
-  const float y = nv1 * atanf((float)x) * (1.0f / PI_F);
-  int32_t n = (int32_t)rintf(y);
-  if (n > kmax) n = kmax;
-  if (n < -kmax) n = -kmax;
-  // both bounds of the guessed code are loaded together
-  for (int it = 0; it < 4; it++) {
-    const double lo = n > -kmax ? thr[n + kmax - 1] : -HUGE_VAL;
-    const double hi = n < kmax ? thr[n + kmax] : HUGE_VAL;
-    if (x < lo) { n--; continue; }
-    if (x >= hi) { n++; continue; }
-    break;
-  }
-  return n;
+  return venc_fast(thr, kmax, izipv, c * v);
 }
 
-// vencode in two stages, so that callers can issue the threshold loads of
-// several values together: the FP32 atan guess, then the correction from
-// the guess's two thresholds (lo = thr[n+kmax-1], hi = thr[n+kmax], or
-// -inf/+inf at the ends) and, rarely, further thresholds.  The result equals
-// vencode's: the correction only ever moves in one direction.
+// The exact table encoder, in two stages: the FP32 atan guess (within one
+// code of the answer), then the correction from the guess's two thresholds
+// (lo = thr[n+kmax-1], hi = thr[n+kmax], or -inf/+inf at the ends) and,
+// rarely, further thresholds; the correction only ever moves in one
+// direction.
 __device__ __forceinline__ int32_t venc_guess(int32_t kmax, int izipv, double x) {
   const float nv1 = izipv == 1 ? 255.0f : 65535.0f;
   const float y = nv1 * atanf((float)x) * (1.0f / PI_F);
@@ -310,6 +308,27 @@ __device__ __forceinline__ int32_t venc_finish(const double* thr, int32_t kmax, 
     for (int it = 1; it < 4 && n < kmax && x >= thr[n + kmax]; it++) n++;
   }
   return n;
+}
+
+// Encode without the threshold tables for all but near-ties: the host
+// encoder rounds y = nv1*atan(x)/pi (FP64 atan, pi in real(4)) to the
+// nearest integer.  The device's FP64 atan differs from the host's by a few
+// ulps, so y differs by far less than 1e-8 here (|y| < 2^15); whenever y is
+// more than 1e-8 from a half-integer both round to the same code, and only
+// the rest (about 2e-8 of the arguments) settle it with the exact host
+// thresholds.
+__device__ __forceinline__ int32_t venc_fast(const double* thr, int32_t kmax, int izipv, double x) {
+  const double nv1 = izipv == 1 ? 255.0 : 65535.0;
+  const double y = nv1 * atan(x) / (double)PI_F;
+  const double r = floor(y + 0.5);
+  if (fabs(y - r) < 0.5 - 1e-8) {
+    int32_t n = (int32_t)r;
+    if (n > kmax) n = kmax;
+    if (n < -kmax) n = -kmax;
+    return n;
+  }
+  const int32_t g = venc_guess(kmax, izipv, x);
+  return venc_finish(thr, kmax, x, g, venc_lo(thr, kmax, g), venc_hi(thr, kmax, g));
 }
 
 // ------------------------------------------------------------- launchers

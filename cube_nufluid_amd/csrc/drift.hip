@@ -58,7 +58,7 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
                                                        const int64_t* __restrict__ cstart, int64_t n3c,
                                                        const float* __restrict__ vfield, float dt_mid, double c_old,
                                                        const double* __restrict__ vdt, int dmax, uint16_t* disp,
-                                                       int32_t* cnt_new, uint32_t* flags) {
+                                                       int32_t* cnt_new, uint32_t* mslot) {
   __shared__ int64_t cs[DR_CH + 1];
   __shared__ int smax[3];
   __shared__ int sbad, sconf;
@@ -154,10 +154,12 @@ __global__ __launch_bounds__(DR_NT) void k_drift_pass1(Geo g, const void* __rest
   if (bad) atomicOr(&sbad, bad);
   if (conf) atomicAdd(&sconf, conf);
   __syncthreads();
-  if (threadIdx.x < 3) atomicMax(&flags[threadIdx.x], (uint32_t)smax[threadIdx.x]);
+  // slot fields 0..2 max displacement, 3 too far / crowded, 4 frame conflicts
+  uint32_t* sl = mslot + (blockIdx.x & (MSLOTS - 1)) * MSLOT_FIELDS;
+  if (threadIdx.x < 3 && smax[threadIdx.x]) atomicMax(&sl[threadIdx.x], (uint32_t)smax[threadIdx.x]);
   if (threadIdx.x == 0) {
-    if (sbad) atomicOr(&flags[3], (uint32_t)sbad);
-    if (sconf) atomicAdd(&flags[4], (uint32_t)sconf);
+    if (sbad) atomicOr(&sl[3], (uint32_t)sbad);
+    if (sconf) atomicAdd(&sl[4], (uint32_t)sconf);
   }
 }
 
@@ -543,22 +545,18 @@ __device__ __forceinline__ void drift_write_body(Geo g, const uint64_t* __restri
       for (int e = 0; e < 3; e++) x[j][e] = ok[j] ? vr_at(q, e) : 0.0;
     }
     int32_t n[DC_SL][3];
-    double tlo[DC_SL][3], thi[DC_SL][3];
 #pragma unroll
     for (int j = 0; j < DC_SL; j++)
 #pragma unroll
       for (int e = 0; e < 3; e++) {
         x[j][e] = c_old * (x[j][e] - (double)svf[cl[j] * 3 + e]);
-        n[j][e] = venc_guess(kmax, g.izipv, x[j][e]);
-        tlo[j][e] = venc_lo(venc, kmax, n[j][e]);
-        thi[j][e] = venc_hi(venc, kmax, n[j][e]);
+        n[j][e] = venc_fast(venc, kmax, g.izipv, x[j][e]);
       }
     double v[DC_SL][3];
 #pragma unroll
     for (int j = 0; j < DC_SL; j++)
 #pragma unroll
       for (int e = 0; e < 3; e++) {
-        n[j][e] = venc_finish(venc, kmax, x[j][e], n[j][e], tlo[j][e], thi[j][e]);
         if (ok[j]) store_v(vp_o, 3 * (q0 + j * DR_NT) + e, g.izipv, n[j][e]);
         v[j][e] = vdecode(vdt, n[j][e], g.izipv);
       }
@@ -654,12 +652,15 @@ static int drift_launch(cube_handle* h, float dt_old, float dt, int* nlayer_out)
   const int64_t nbw = (h->n3c + DW_CELLS - 1) / DW_CELLS;
   if (nb3 * 3 > h->dred_n - 16 || nbw * 3 > h->dred_n - 16) return cube_fail(h, CUBE_ERR_ARG, "reduction buffer too small");
   int r;
-  if ((r = fill_u32(h, flags, 8, 0u)) || (r = fill_u32(h, h->rhoc2, h->n3c, 0u))) return r;
+  if ((r = fill_u32(h, flags, 8, 0u)) || (r = fill_u32(h, h->rhoc2, h->n3c, 0u)) ||
+      (r = fill_u32(h, h->mslots, MSLOTS * MSLOT_FIELDS, 0u)))
+    return r;
   int pid1 = prof_begin(h, PS_DRIFT1);
   k_drift_pass1<<<(unsigned)nb1, DR_NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, dt_mid, c_old, vdt, dmax,
-                                                         h->disp, h->rhoc2, flags);
+                                                         h->disp, h->rhoc2, h->mslots);
   prof_end(h, pid1);
   HIP_TRY(h, hipGetLastError());
+  if ((r = slots_reduce(h, 5, 1u << 3, 1u << 4, flags))) return r;
   int pid2 = prof_begin(h, PS_DRIFT_SCAN);
   r = scan_counts(h, h->rhoc2, h->n3c, h->cstart2);
   prof_end(h, pid2);

@@ -101,8 +101,32 @@ __global__ void k_reduce_fields(const double* part, int64_t nparts, int nfields,
   }
   if (threadIdx.x == 0) out[f] = tmp[0];
 }
+// Large inputs in two levels: RL_BLOCKS workgroups per field each sum a
+// fixed contiguous chunk, then one workgroup per field sums the chunks; the
+// partition depends only on nparts, so the order is fixed
+#define RL_BLOCKS 256
+__global__ void k_reduce_chunks(const double* part, int64_t nparts, int nfields, double* l1) {
+  __shared__ double tmp[SCAN_NT];
+  const int f = blockIdx.y;
+  const int64_t chunk = (nparts + RL_BLOCKS - 1) / RL_BLOCKS;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(nparts, i0 + chunk);
+  double s = 0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += SCAN_NT) s += part[i * nfields + f];
+  tmp[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = SCAN_NT / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) tmp[threadIdx.x] += tmp[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) l1[(int64_t)blockIdx.x * nfields + f] = tmp[0];
+}
 int reduce_doubles_dev(cube_handle* h, const double* part, int64_t nparts, int nfields, double* dev_out) {
-  k_reduce_fields<<<nfields, SCAN_NT, 0, h->stream>>>(part, nparts, nfields, dev_out);
+  if (nparts > 64 * 1024 && nfields <= 8 && h->dred_l1) {
+    k_reduce_chunks<<<dim3(RL_BLOCKS, nfields), SCAN_NT, 0, h->stream>>>(part, nparts, nfields, h->dred_l1);
+    k_reduce_fields<<<nfields, SCAN_NT, 0, h->stream>>>(h->dred_l1, RL_BLOCKS, nfields, dev_out);
+  } else {
+    k_reduce_fields<<<nfields, SCAN_NT, 0, h->stream>>>(part, nparts, nfields, dev_out);
+  }
   HIP_TRY(h, hipGetLastError());
   return 0;
 }

@@ -431,3 +431,10 @@ class Cube:
         rho = np.empty((nf, nf, nf), dtype=np.float32)
         _lib.check(self.lib, self.h, self.lib.cube_probe_density(self.h, _lib.ptr(rho)))
         return rho
+
+    def probe_vencode(self, x):
+        """device velocity codes of the atan arguments x = c*v"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        codes = np.empty(len(x), dtype=np.int32)
+        _lib.check(self.lib, self.h, self.lib.cube_probe_vencode(self.h, _lib.ptr(x), len(x), _lib.ptr(codes)))
+        return codes

@@ -157,6 +157,12 @@ int cube_probe_pp_forces(cube_handle* h, double* f_pp);
    x fastest) -- the periodic field pm.f90:76-107 builds tile by tile. */
 int cube_probe_density(cube_handle* h, float* rho);
 
+/* Velocity codes of n arguments x = c*v of update_particle.f90:119's atan
+   (c = sqrt(pi/2)/(sigma*vrel_boost)), from the device encoder that the
+   drift, kick and PP kernels use, for the handle's izipv.  Leaves the state
+   unchanged. */
+int cube_probe_vencode(cube_handle* h, const double* x, int64_t n, int32_t* codes);
+
 /* Matter power spectrum of the current positions on the fine mesh -- the
    estimator of utilities/cicpower.f90:85-257 with powerspectrum.f90:27-137
    (CIC density, delta = rho/mean - 1, transform / nf^3, sinc^4 window

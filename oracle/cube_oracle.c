@@ -127,6 +127,11 @@ int32_t orc_vencode(double v, float sigma, int64_t izipv) {
   double y = (double)nv1 * atan(c * v) / (double)PI_F;
   return (int32_t)nint_d(y);
 }
+/* the same encode from the atan argument x = c*v, for n values */
+void orc_vencode_x(const double* x, int64_t n, int64_t izipv, int32_t* out) {
+  float nv1 = (float)(nvbin_of(izipv) - 1);
+  for (int64_t i = 0; i < n; i++) out[i] = (int32_t)nint_d((double)nv1 * atan(x[i]) / (double)PI_F);
+}
 
 /* ------------------------------------------------------------ geometry */
 static inline int64_t wrapi(int64_t a, int64_t n) { a %= n; return a < 0 ? a + n : a; }

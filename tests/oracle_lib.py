@@ -48,6 +48,7 @@ def load():
         "orc_set_perturb": (None, [ctypes.c_int]),
         "orc_vdecode": (_f64, [_i32, _f32, _i64]),
         "orc_vencode": (_i32, [_f64, _f32, _i64]),
+        "orc_vencode_x": (None, [_P, _i64, _i64, _P]),
         "orc_fft_r2c": (None, [_P, _i64, _P, _P, ctypes.c_int]),
         "orc_density": (ctypes.c_int, [_P, _P, ctypes.c_int]),
         "orc_set_tile_limit": (None, [_P, _i64]),
@@ -207,6 +208,14 @@ def vdecode(v, sigma, izipv):
 
 def vencode(v, sigma, izipv):
     return load().orc_vencode(float(v), float(sigma), int(izipv))
+
+
+def vencode_x(x, izipv):
+    """codes of the atan arguments x = c*v (update_particle.f90:119)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty(len(x), dtype=np.int32)
+    load().orc_vencode_x(p(x), len(x), int(izipv), p(out))
+    return out
 
 
 def fft_r2c(r):
