@@ -288,128 +288,141 @@ struct KickMeshes {
   const T* c;  // [3][nc^3]
 };
 
-template <typename T, int BX, int BY, int BZ, int NT>
+template <typename T, int BX, int NT>
 __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
                                                   int64_t n3c, const float* __restrict__ vfield, KickMeshes<T> F, int64_t Nf,
-                                                  float a_mid, float dt, double c_old, const double* __restrict__ cnew_p,
+                                                  float a_mid, float dt, const double* __restrict__ cnew_p,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
                                                   const double* __restrict__ venc, int32_t kmax,
                                                   uint32_t* mslot, double* probe_f, double* probe_c) {
-  // the workgroup's block: BX x BY x BZ coarse cells, NR = BY*BZ rows of BX
-  // cells, each row a run of consecutive ranks (BX divides nt)
-  constexpr int NR = BY * BZ;
-  constexpr int SX = 4 * BX + 1, SY = 4 * BY + 1, SZ = 4 * BZ + 1;       // fine slab [3][SZ][SY][SX]
-  constexpr int FROW = SZ * SY * SX, NFS = 3 * FROW;
-  constexpr int CX = BX + 2, CY = BY + 2, CZ = BZ + 2;                   // coarse slab [3][CZ][CY][CX]
-  constexpr int CROW = CZ * CY * CX, NCS = 3 * CROW;
+  constexpr int W = NT / 64;                               // wavefronts
+  constexpr int XF = 4 * BX, SX = XF + 1;                  // fine slab [3][5][5][SX]
+  constexpr int FROW = 25 * SX, NFS = 3 * FROW;
+  constexpr int NRF = 75;                                  // fine rows (d, z, y)
+  constexpr int NCH = (XF + 63) / 64;                      // 64-node chunks of a row's first XF nodes
+  constexpr int KRF = (NRF + W - 1) / W;                   // fine rows per wavefront
+  constexpr int CX = BX + 2, CROW = 9 * CX, NCS = 3 * CROW; // coarse slab [3][3][3][CX]
+  constexpr int NRC = 27, KRC = (NRC + W - 1) / W;
   static_assert(CX <= 64, "a wavefront per coarse row");
   __shared__ T sF[NFS];
   __shared__ T sC[NCS];
-  __shared__ int64_t cs[NR][BX + 1];
-  __shared__ int64_t rowrank[NR];
-  __shared__ int32_t rowpref[NR + 1];
-  __shared__ float red[4][NT / 64];
-  const int nbx = g.nc / BX, nby = g.nc / BY;
+  __shared__ int64_t cs[BX + 1];
+  __shared__ float red[4][W];
+  const int nbx = g.nc / BX;
   const int b = blockIdx.x;
-  const int32_t c0[3] = {(b % nbx) * BX, ((b / nbx) % nby) * BY, (b / (nbx * nby)) * BZ};
-  const int tid = threadIdx.x;
+  const int32_t c0[3] = {(b % nbx) * BX, (b / nbx) % g.nc, b / (nbx * g.nc)};
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const double c_new = *cnew_p;  // vcoef(sigma_vi_new), formed on the device by the drift
   const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
-  if (tid < NR) rowrank[tid] = rank_of(g, c0[0], c0[1] + tid % BY, c0[2] + tid / BY);
+  const int64_t r0 = rank_of(g, c0[0], c0[1], c0[2]);
+  if (tid <= BX) cs[tid] = cstart[r0 + tid];
   __syncthreads();
-  for (int i = tid; i < NR * (BX + 1); i += NT) cs[i / (BX + 1)][i % (BX + 1)] = cstart[rowrank[i / (BX + 1)] + i % (BX + 1)];
-  __syncthreads();
-  if (tid == 0) {
-    rowpref[0] = 0;
-    for (int j = 0; j < NR; j++) rowpref[j + 1] = rowpref[j] + (int32_t)(cs[j][BX] - cs[j][0]);
-  }
-  __syncthreads();
-  const int32_t ntot = rowpref[NR];
-  // particle k of the block: its row j and global index
-  auto locate = [&](int32_t k, int& j) -> int64_t {
-    j = 0;
-#pragma unroll
-    for (int q = 1; q < NR; q++) j += rowpref[q] <= k ? 1 : 0;
-    return cs[j][0] + (k - rowpref[j]);
-  };
+  const int64_t p0 = cs[0];
+  const int32_t ntot = (int32_t)(cs[BX] - p0);
   // this thread's first particle: its position and velocity codes are
   // requested before the slabs and its old velocities decoded while they
   // arrive, so those latencies overlap the staging
-  const int32_t ka = tid;
-  int ja = 0;
-  int64_t pa = ka < ntot ? locate(ka, ja) : 0;
+  int64_t pa = p0 + tid;
   uint32_t xa[3] = {0u, 0u, 0u};
   int32_t va[3] = {0, 0, 0};
   double vo_a[3] = {0.0, 0.0, 0.0};
-  if (ka < ntot) {
+  if (tid < ntot) {
 #pragma unroll
     for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pa + e, g.izipx);
 #pragma unroll
     for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
   }
-  // stage the fine slab (3*SZ*SY rows of SX nodes) and the coarse slab
-  // (3*CZ*CY rows of CX nodes), dealt to the threads in row-major order so
-  // consecutive lanes read consecutive nodes of a row; all of a thread's
-  // loads are issued before its LDS stores
+  // stage the slabs row by row: row r of the fine slab is component d =
+  // r / 25, z = (r / 5) % 5, y = r % 5, and a wavefront takes rows wv,
+  // wv + W, ...  (uniform, so the row addresses are scalar), its lanes
+  // consecutive nodes of the row; the far face x = XF of all 75 rows is
+  // loaded by the first 75 threads.  All loads are issued before the LDS
+  // stores.
   {
-    constexpr int KF = (NFS + NT - 1) / NT, KC = (NCS + NT - 1) / NT;
-    T v[KF], vc[KC];
+    T v[KRF][NCH], vfar = (T)0, vc[KRC];
+    const uint32_t xb = 4u * (uint32_t)c0[0];
 #pragma unroll
-    for (int k = 0; k < KF; k++) {
-      const int e = tid + k * NT;
-      const int row = e / SX, x = e - row * SX;  // row = (d*SZ + z)*SY + y
-      const int y = row % SY, z = (row / SY) % SZ, d = row / (SY * SZ);
+    for (int k = 0; k < KRF; k++) {
+      const int r = wv + k * W;
+      if (r < NRF) {
+        const int d = r / 25, z = (r / 5) % 5, y = r % 5;
+        const T* f = d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]);
+        const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
+        const uint32_t gy = (4u * (uint32_t)c0[1] + y) & mf, gz = (4u * (uint32_t)c0[2] + z) & mf;
+        // scalar row base, 32-bit byte offset per lane
+        const char* row = (const char*)(f + (int64_t)(gz * (uint32_t)Nf + gy) * P);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+          const int x = ch * 64 + lane;
+          if (x < XF) v[k][ch] = *(const T*)(row + (uint32_t)(((xb + x) & mf) * (uint32_t)sizeof(T)));
+        }
+      }
+    }
+    if (tid < NRF) {
+      const int d = tid / 25, z = (tid / 5) % 5, y = tid % 5;
       const T* f = d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]);
       const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
-      const uint32_t gy = (uint32_t)(4 * c0[1] + y) & mf, gz = (uint32_t)(4 * c0[2] + z) & mf;
-      v[k] = e < NFS ? f[(int64_t)(gz * (uint32_t)Nf + gy) * P + ((uint32_t)(4 * c0[0] + x) & mf)] : (T)0;
+      const uint32_t gy = (4u * (uint32_t)c0[1] + y) & mf, gz = (4u * (uint32_t)c0[2] + z) & mf;
+      vfar = f[(int64_t)(gz * (uint32_t)Nf + gy) * P + ((xb + XF) & mf)];
     }
 #pragma unroll
-    for (int k = 0; k < KC; k++) {
-      const int e = tid + k * NT;
-      const int row = e / CX, xl = e - row * CX;  // row = (d*CZ + z)*CY + y
-      const int y = row % CY, z = (row / CY) % CZ, d = row / (CY * CZ);
-      const uint32_t gx = (uint32_t)(c0[0] - 1 + xl) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
-                     gz = (uint32_t)(c0[2] - 1 + z) & mc;
-      vc[k] = e < NCS ? F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx] : (T)0;
+    for (int k = 0; k < KRC; k++) {
+      const int r = wv + k * W;  // coarse row: component r / 9, z = (r / 3) % 3, y = r % 3
+      if (r < NRC && lane < CX) {
+        const int d = r / 9, z = (r / 3) % 3, y = r % 3;
+        const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
+                       gz = (uint32_t)(c0[2] - 1 + z) & mc;
+        vc[k] = F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx];
+      }
     }
-    if (ka < ntot) {
+    if (tid < ntot) {
 #pragma unroll
       for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_old, va[e], g.izipv);
     }
 #pragma unroll
-    for (int k = 0; k < KF; k++) {
-      const int e = tid + k * NT;
-      if (e < NFS) sF[e] = v[k];
-    }
+    for (int k = 0; k < KRF; k++) {
+      const int r = wv + k * W;
+      if (r < NRF) {
 #pragma unroll
-    for (int k = 0; k < KC; k++) {
-      const int e = tid + k * NT;
-      if (e < NCS) sC[e] = vc[k];
+        for (int ch = 0; ch < NCH; ch++) {
+          const int x = ch * 64 + lane;
+          if (x < XF) sF[r * SX + x] = v[k][ch];
+        }
+      }
+    }
+    if (tid < NRF) sF[tid * SX + XF] = vfar;
+#pragma unroll
+    for (int k = 0; k < KRC; k++) {
+      const int r = wv + k * W;
+      if (r < NRC && lane < CX) sC[r * CX + lane] = vc[k];
     }
   }
   __syncthreads();
   // max |F|^2 over the fine nodes this workgroup owns (the slab without its
   // far faces; the workgroups' owned nodes tile the mesh), summed in float
-  // in component order: the dt_fine input of pm.f90
+  // in component order: the dt_fine input of pm.f90.  Chunk q of 64 nodes:
+  // row q / NCH of the 16 (z, y) rows, x from (q % NCH) * 64.
   float f2loc = 0.f;
-  for (int i = tid; i < 64 * BX * BY * BZ; i += NT) {
-    const int x = i % (4 * BX), y = (i / (4 * BX)) % (4 * BY), z = i / (16 * BX * BY);
-    const int o = (z * SY + y) * SX + x;
-    const float a = (float)sF[o], b = (float)sF[FROW + o], c = (float)sF[2 * FROW + o];
-    float s2 = a * a;
-    s2 = fmaf(b, b, s2);
-    s2 = fmaf(c, c, s2);
-    f2loc = fmaxf(f2loc, s2);
+#pragma unroll
+  for (int k = 0; k < (16 * NCH + W - 1) / W; k++) {
+    const int q = wv + k * W;
+    const int x = (q % NCH) * 64 + lane, y = (q / NCH) % 4, z = (q / NCH) / 4;
+    if (q < 16 * NCH && x < XF) {
+      const int o = (z * 5 + y) * SX + x;
+      const float a = (float)sF[o], bq = (float)sF[FROW + o], c = (float)sF[2 * FROW + o];
+      float s2 = a * a;
+      s2 = fmaf(bq, bq, s2);
+      s2 = fmaf(c, c, s2);
+      f2loc = fmaxf(f2loc, s2);
+    }
   }
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax[3] = {0.f, 0.f, 0.f};
-  int jcur = ja;
-  for (int32_t k = ka; k < ntot; k += NT) {
+  for (int32_t k = tid; k < ntot; k += NT) {
     // this particle's codes and old velocities; the next particle's codes
     // are requested now and decoded at the end of this one
     const int64_t p = pa;
-    const int j = jcur;
     uint32_t xu[3];
     double vo[3];
 #pragma unroll
@@ -419,7 +432,7 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
     }
     const int32_t kn = k + NT;
     if (kn < ntot) {
-      pa = locate(kn, jcur);
+      pa = p0 + kn;
 #pragma unroll
       for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pa + e, g.izipx);
 #pragma unroll
@@ -428,12 +441,12 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
     int lo = 0, hi = BX;
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
-      if (cs[j][mid] <= p) lo = mid; else hi = mid;
+      if (cs[mid] <= p) lo = mid; else hi = mid;
     }
     float vf[3];
 #pragma unroll
-    for (int e = 0; e < 3; e++) vf[e] = vfield[(rowrank[j] + lo) * 3 + e];
-    const int32_t c[3] = {c0[0] + lo, c0[1] + j % BY, c0[2] + j / BY};
+    for (int e = 0; e < 3; e++) vf[e] = vfield[(r0 + lo) * 3 + e];
+    const int32_t c[3] = {c0[0] + lo, c0[1], c0[2]};
     T dx1[3], dx2[3], cx1[3], cx2[3];
     uint32_t f1[3], f2[3];
     int lf[3], lc[3];
@@ -448,14 +461,14 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
       // a tile-local float coordinate may round up to 4*nt; the mask wraps it
       f1[e] = (uint32_t)(CUBE_NCELL * torg + a1 - 1) & mf;
       f2[e] = (f1[e] + 1) & mf;
-      lf[e] = CUBE_NCELL * (c[e] - c0[e]) + (a1 - 1 - CUBE_NCELL * (il - 1));  // 0..4*BX (x), 0..4*BY (y), 0..4*BZ (z)
+      lf[e] = CUBE_NCELL * (c[e] - c0[e]) + (a1 - 1 - CUBE_NCELL * (il - 1));  // 0..4*BX (x), 0..4 (y, z)
       const T tc = (T)(((double)(torg + il - 1) + xo) - 0.5);
       const int b1 = (int)floor(tc) + 1;  // 1-based global node: b1-1 is c-1 or c
       cx1[e] = (T)b1 - tc;
       cx2[e] = 1 - cx1[e];
-      lc[e] = b1 - c0[e];  // local coarse node of the lower tap: 0..BX (x), 0..BY (y), 0..BZ (z)
+      lc[e] = b1 - c0[e];  // local coarse node of the lower tap: 0..BX (x), 0..1 (y, z)
     }
-    const bool in_slab = lf[0] < 4 * BX && lf[1] < 4 * BY && lf[2] < 4 * BZ;
+    const bool in_slab = lf[0] < XF && lf[1] < 4 && lf[2] < 4;
     // the 8 taps: bit 0 picks x, bit 1 y, bit 2 z (order of pm.f90:146-153)
     const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
     int sf[8], sc[8];
@@ -463,8 +476,8 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-      sf[q] = ((lf[2] + oz) * SY + lf[1] + oy) * SX + lf[0] + ox;
-      sc[q] = ((lc[2] + oz) * CY + lc[1] + oy) * CX + lc[0] + ox;
+      sf[q] = ((lf[2] + oz) * 5 + lf[1] + oy) * SX + lf[0] + ox;
+      sc[q] = ((lc[2] + oz) * 3 + lc[1] + oy) * CX + lc[0] + ox;
       Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
       Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
@@ -493,7 +506,7 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
     // fine kick: each tap's F*k*w in the mesh precision added to the old
     // velocity in tap order, straight from the slab (no per-tap arrays
     // stay live through the codec stages)
-    double v[3], x[3];
+    double v[3];
     int32_t n[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -512,8 +525,7 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
         }
       }
       // re-quantise with the new sigma: encode, then decode
-      x[d] = c_new * v[d];
-      n[d] = venc_fast(venc, kmax, g.izipv, x[d]);
+      n[d] = venc_fast(venc, kmax, g.izipv, c_new * v[d]);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) v[d] = vdecode(vdt_new, n[d], g.izipv);
@@ -524,8 +536,7 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
       for (int q = 0; q < 8; q++) v[d] = v[d] + (double)(sC[d * CROW + sc[q]] * kf * Cw[q]);
       const float av = (float)fabs(v[d] + (double)vf[d]);
       vmax[d] = fmaxf(vmax[d], av);
-      x[d] = c_new * v[d];
-      n[d] = venc_fast(venc, kmax, g.izipv, x[d]);
+      n[d] = venc_fast(venc, kmax, g.izipv, c_new * v[d]);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, n[d]);
@@ -540,12 +551,12 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
     for (int d = 0; d < 4; d++) {
       float m = mv[d];
       for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-      if ((tid & 63) == 0) red[d][tid >> 6] = m;
+      if (lane == 0) red[d][wv] = m;
     }
     __syncthreads();
     if (tid < 4) {
       float m = red[tid][0];
-      for (int i = 1; i < NT / 64; i++) m = fmaxf(m, red[tid][i]);
+      for (int i = 1; i < W; i++) m = fmaxf(m, red[tid][i]);
       // slot fields: 0 max |F|^2, 1..3 vmax (non-negative floats order as
       // their bit patterns)
       atomicMax(mslot + (blockIdx.x & (MSLOTS - 1)) * MSLOT_FIELDS + (tid == 3 ? 0 : tid + 1), __float_as_uint(m));
@@ -553,10 +564,9 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
   }
 }
 
-template <typename T, int BX, int BY, int BZ, int NT>
-static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-                        const double* c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
-                        double* probe_f, double* probe_c) {
+template <typename T, int BX, int NT>
+static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt,
+                        const double* c_new, const double* vo, const double* vn, double* probe_f, double* probe_c) {
   Geo g = geo_of(h);
   KickMeshes<T> F;
   for (int d = 0; d < 3; d++) {
@@ -564,23 +574,20 @@ static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t p
     F.pf[d] = pf[d];
   }
   F.c = (const T*)h->fc;
-  (void)vmax_bits;
-  (void)f2max_bits;
-  k_kick<T, BX, BY, BZ, NT><<<(unsigned)(h->n3c / (BX * BY * BZ)), NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
-                                                                    a_mid, dt, c_old, c_new, vo, vn, h->venc, h->vkmax, h->mslots,
-                                                                    probe_f, probe_c);
+  k_kick<T, BX, NT><<<(unsigned)(h->n3c / BX), NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
+                                                                 a_mid, dt, c_new, vo, vn, h->venc, h->vkmax, h->mslots,
+                                                                 probe_f, probe_c);
 }
 
 template <typename T>
-static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt, double c_old,
-                     const double* c_new, const double* vo, const double* vn, uint32_t* vmax_bits, uint32_t* f2max_bits,
-                     double* probe_f, double* probe_c) {
+static int kick_pick(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt,
+                     const double* c_new, const double* vo, const double* vn, double* probe_f, double* probe_c) {
   if (h->nt % 32 == 0)
-    kick_launch<T, 32, 1, 1, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+    kick_launch<T, 32, 256>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c);
   else if (h->nt % 16 == 0)
-    kick_launch<T, 16, 1, 1, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+    kick_launch<T, 16, 256>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c);
   else if (h->nt % 4 == 0)
-    kick_launch<T, 4, 1, 1, 256>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+    kick_launch<T, 4, 256>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c);
   else
     return cube_fail(h, CUBE_ERR_ARG, "nt=%lld must be a multiple of 4", (long long)h->nt);
   return 0;
@@ -596,8 +603,8 @@ int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float
   if (r0) return r0;
   int pr = prof_begin(h, PS_KICK);
   int r = h->prec == 4
-              ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c)
-              : kick_pick<double>(h, Ff, pf, a_mid, dt, c_old, c_new, vo, vn, vmax_bits, f2max_bits, probe_f, probe_c);
+              ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c)
+              : kick_pick<double>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c);
   prof_end(h, pr);
   if (r) return r;
   HIP_TRY(h, hipGetLastError());
