@@ -261,7 +261,8 @@ extern "C" int cube_profile_enable(cube_handle* h, int on) {
 static const char* kSlotNames[PS_NSLOT] = {"drift_pass1", "drift_scan", "drift_scatter", "drift_sort_big", "drift_write",
                                            "fluid", "pp", "cic_coarse",
                                            "fft_coarse", "cic_fine", "fft_x_r2c", "fft_y_fwd", "fft_zconv",
-                                           "fft_y_inv", "fft_x_c2r", "kick", "fluid_density", "fluid_gravity"};
+                                           "fft_y_inv", "fft_x_c2r", "kick_fine", "fluid_density", "fluid_gravity",
+                                           "kick_coarse"};
 
 extern "C" int cube_profile_read(cube_handle* h, double* ms, int64_t* count, int nslot) {
   if (!h || nslot < 0) return CUBE_ERR_ARG;
@@ -439,7 +440,7 @@ static int pm_launch(cube_handle* h, float a_mid, float dt, double* probe_f, dou
   // fine mesh, kx-tiled spectrum: the density in R goes to a tiled
   // spectrum in W; components 0 and 1 are convolved into X and inverted into
   // S (F0) and R (F1), component 2 in place in W (the spectrum's last use)
-  // and inverted into X (F2).  All three stay resident for one fused kick.
+  // and inverted into X (F2).  All three stay resident for the fine kick.
   if ((r = cic_fine(h, nullptr, &dr->mass[2]))) return r;
   prd = nu ? prof_begin(h, PS_FLUID_DENS) : -1;
   if (nu && (r = fl_density_fine(h, a_mid, nullptr, &dr->mass[3]))) return r;
@@ -589,7 +590,7 @@ extern "C" int cube_probe_pp_forces(cube_handle* h, double* f_pp) {
 
 __global__ void k_probe_venc(const double* thr, int32_t kmax, int izipv, const double* x, int64_t n, int32_t* out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = venc_fast(thr, kmax, izipv, x[i]);
+    out[i] = venc_fast(thr, kAtanTab, kmax, izipv, x[i]);
 }
 extern "C" int cube_probe_vencode(cube_handle* h, const double* x, int64_t n, int32_t* codes) {
   CUBE_EXCLUSIVE;

@@ -47,7 +47,7 @@ struct Mesh {
 // Per-kernel timing with HIP events on the handle's stream (diagnostic).
 enum ProfSlot {
   PS_DRIFT1 = 0, PS_DRIFT_SCAN, PS_DRIFT_SCATTER, PS_DRIFT_SORT, PS_DRIFT_WRITE, PS_FLUID, PS_PP, PS_CIC_C, PS_FFT_C, PS_CIC_F, PS_FFT_XF, PS_FFT_YF,
-  PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_FLUID_DENS, PS_FLUID_GRAV, PS_NSLOT
+  PS_FFT_ZCONV, PS_FFT_YI, PS_FFT_XI, PS_KICK, PS_FLUID_DENS, PS_FLUID_GRAV, PS_KICK_C, PS_NSLOT
 };
 struct ProfRec { int slot; hipEvent_t a, b; };
 
@@ -272,12 +272,30 @@ __device__ __forceinline__ double nint_d(double y) {
 __device__ __forceinline__ double vdecode(const double* dtab, int32_t v, int izipv) {
   return dtab[v + (izipv == 1 ? 128 : 32768)];
 }
+// atan(j/32) for j = 0..32, correctly rounded (gen_atan_table.py prints
+// it): the breakpoints of the encoder's atan.  Kernels that encode many
+// values copy it to LDS (atan_tab_to_lds).
+static __constant__ double kAtanTab[33] = {
+    0.0, 0.031239833430268277, 0.06241880999595735, 0.09347678115858947,
+    0.12435499454676144, 0.15499674192394097, 0.18534794999569476, 0.21535769969773805,
+    0.24497866312686414, 0.2741674511196588, 0.3028848683749714, 0.3310960767041321,
+    0.35877067027057225, 0.38588266939807375, 0.4124104415973873, 0.43833655985795783,
+    0.4636476090008061, 0.48833395105640554, 0.5123894603107377, 0.5358112379604637,
+    0.5585993153435624, 0.5807563535676704, 0.6022873461349642, 0.6231993299340659,
+    0.6435011087932844, 0.6632029927060933, 0.6823165548747481, 0.7008544078844502,
+    0.7188299996216245, 0.7362574289814281, 0.7531512809621944, 0.7695264804056583,
+    0.7853981633974483};
+__device__ __forceinline__ void atan_tab_to_lds(double* s) {
+  if (threadIdx.x < 33) s[threadIdx.x] = kAtanTab[threadIdx.x];
+}
+
 // velocity encode, equal to the host encoder's code for every argument.
 // thr[k + kmax - 1] is the smallest x with enc(x) >= k, for k in
-// [-kmax+1, kmax]; venc_fast (below) reads it only near ties.
-__device__ __forceinline__ int32_t venc_fast(const double* thr, int32_t kmax, int izipv, double x);
+// [-kmax+1, kmax]; venc_fast (below) reads it only near ties.  at is
+// kAtanTab or a copy of it.
+__device__ __forceinline__ int32_t venc_fast(const double* thr, const double* at, int32_t kmax, int izipv, double x);
 __device__ __forceinline__ int32_t vencode(const double* thr, int32_t kmax, int izipv, double c, double v) {
-  return venc_fast(thr, kmax, izipv, c * v);
+  return venc_fast(thr, kAtanTab, kmax, izipv, c * v);
 }
 
 // The exact table encoder, in two stages: the FP32 atan guess (within one
@@ -310,32 +328,33 @@ __device__ __forceinline__ int32_t venc_finish(const double* thr, int32_t kmax, 
   return n;
 }
 
-// atan for the encoder below: atan(u) = u*P(u^2) on [0, 1] and
-// pi/2 - atan(1/|x|) above it, P the 18-term Chebyshev fit of
-// atan(sqrt(w))/sqrt(w) on [0, 1] (max error 1.2e-15; gen_atan_poly.py
-// prints it), 1/|x| from the hardware reciprocal and two Newton steps.
-// Its error in nv1*atan(x)/pi stays below 3e-11 (2e7 arguments checked
-// against the host libm).
-__device__ __forceinline__ double atan_enc(double x) {
-  constexpr double P[18] = {-4.572466752859746e-05, 0.0004611471226395625, -0.0021983905110338,
-                            0.006642618398837011,   -0.014458808736889693, 0.02458916421806469,
-                            -0.034874503444929254,  0.04377190010711523,   -0.051278290246137094,
-                            0.05845587712383866,    -0.06659119943643073,  0.07691170067941966,
-                            -0.09090787968013053,   0.11111102499422242,   -0.14285713910426773,
-                            0.19999999991319195,    -0.33333333333253284,  0.9999999999999988};
+// atan for the encoder below.  |x| > 1 is reduced to pi/2 - atan(1/|x|),
+// so u = min(|x|, 1/|x|) is in [0, 1]; with the nearest breakpoint
+// c = j/32, atan(u) = atan(c) + atan(h), h = (u - c)/(1 + u*c), |h| <= 1/64,
+// and atan(h) = h - h^3/3 + h^5/5 - h^7/7 leaves out less than 1e-17.
+// Reciprocals come from the hardware estimate and two Newton steps.  The
+// error in nv1*atan(x)/pi stays below 2e-11 (4e7 arguments checked against
+// the host libm).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double q = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, q, 1.0);
+  q = fma(q, e, q);
+  e = fma(-d, q, 1.0);
+  return fma(q, e, q);
+}
+__device__ __forceinline__ double atan_enc(double x, const double* at) {
   const double a = fabs(x);
   const bool big = a > 1.0;
-  const double ac = fmin(a, 1e300);  // keeps the Newton steps finite for |x| = inf
-  double q = __builtin_amdgcn_rcp(ac);
-  double e = fma(-ac, q, 1.0);
-  q = fma(q, e, q);
-  e = fma(-ac, q, 1.0);
-  q = fma(q, e, q);
-  const double u = big ? q : a, w = u * u;
-  double p = P[0];
-#pragma unroll
-  for (int i = 1; i < 18; i++) p = fma(p, w, P[i]);
-  double r = u * p;
+  const double u = big ? rcp_nr(fmin(a, 1e300)) : a;  // the bound keeps the Newton steps finite for |x| = inf
+  int j = (int)(u * 32.0 + 0.5);
+  j = j < 0 ? 0 : (j > 32 ? 32 : j);
+  const double c = (double)j * 0.03125;
+  const double h = (u - c) * rcp_nr(fma(u, c, 1.0));
+  const double h2 = h * h;
+  double t = fma(h2, -1.0 / 7.0, 1.0 / 5.0);
+  t = fma(t, h2, -1.0 / 3.0);
+  t = t * h2;
+  double r = at[j] + fma(h, t, h);
   r = big ? 1.5707963267948966 - r : r;
   return copysign(r, x);
 }
@@ -347,9 +366,9 @@ __device__ __forceinline__ double atan_enc(double x) {
 // whenever y is more than 1e-8 from a half-integer both round to the same
 // code, and only the rest (about 2e-8 of the arguments) settle it with the
 // exact host thresholds.
-__device__ __forceinline__ int32_t venc_fast(const double* thr, int32_t kmax, int izipv, double x) {
+__device__ __forceinline__ int32_t venc_fast(const double* thr, const double* at, int32_t kmax, int izipv, double x) {
   const double k = izipv == 1 ? 255.0 / (double)PI_F : 65535.0 / (double)PI_F;
-  const double y = atan_enc(x) * k;
+  const double y = atan_enc(x, at) * k;
   const double r = rint(y);
   if (fabs(y - r) < 0.5 - 1e-8) {
     int32_t n = (int32_t)r;

@@ -438,7 +438,7 @@ __device__ __forceinline__ void drift_write_body(Geo g, const uint64_t* __restri
                                                  const double* __restrict__ vdt, const double* __restrict__ venc,
                                                  int32_t kmax, void* xp_o, int32_t* pid_o, double* vr, void* vp_o,
                                                  float* vfield_o, int32_t* rhoc_o, double* part, const int64_t* scs,
-                                                 float* svf, double (*red)[DR_NT / 64], double* svr) {
+                                                 float* svf, double (*red)[DR_NT / 64], double* svr, const double* sat) {
   const int64_t qa = scs[0], q1 = scs[ncell];
   // vreal of slot q: LDS when the workgroup's slots fit, else the HBM scratch
   auto vr_at = [&](int64_t q, int e) -> double& { return STAGED ? svr[3 * (q - qa) + e] : vr[3 * q + e]; };
@@ -550,7 +550,7 @@ __device__ __forceinline__ void drift_write_body(Geo g, const uint64_t* __restri
 #pragma unroll
       for (int e = 0; e < 3; e++) {
         x[j][e] = c_old * (x[j][e] - (double)svf[cl[j] * 3 + e]);
-        n[j][e] = venc_fast(venc, kmax, g.izipv, x[j][e]);
+        n[j][e] = venc_fast(venc, sat, kmax, g.izipv, x[j][e]);
       }
     double v[DC_SL][3];
 #pragma unroll
@@ -594,18 +594,20 @@ __global__ __launch_bounds__(DR_NT) void k_drift_write(Geo g, const uint64_t* __
   __shared__ float svf[DW_CELLS * 3];
   __shared__ double red[3][DR_NT / 64];
   __shared__ double svr[3 * DW_STAGE];
+  __shared__ double sat[33];
   // XCD-contiguous block order: neighbouring cell rows gather from the same
   // source cells, so they run on the same XCD (the same L2)
   const int64_t r0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * DW_CELLS;
   const int ncell = (int)min((int64_t)DW_CELLS, n3c - r0);
   for (int i = threadIdx.x; i <= DW_CELLS; i += DR_NT) scs[i] = cs_new[r0 + min(i, ncell)];
+  atan_tab_to_lds(sat);
   __syncthreads();
   if (scs[ncell] - scs[0] <= DW_STAGE)
     drift_write_body<true>(g, keys, r0, ncell, cstart, xp, vp, pid, vfield, dt_mid, c_old, vdt, venc, kmax, xp_o, pid_o, vr, vp_o,
-                           vfield_o, rhoc_o, part, scs, svf, red, svr);
+                           vfield_o, rhoc_o, part, scs, svf, red, svr, sat);
   else
     drift_write_body<false>(g, keys, r0, ncell, cstart, xp, vp, pid, vfield, dt_mid, c_old, vdt, venc, kmax, xp_o, pid_o, vr,
-                            vp_o, vfield_o, rhoc_o, part, scs, svf, red, svr);
+                            vp_o, vfield_o, rhoc_o, part, scs, svf, red, svr, sat);
 }
 
 // sigma_vi_new = sqrt(sum v_res^2 / N) / sqrt(3) (update_particle.f90:213,

@@ -264,65 +264,93 @@ int cic_coarse(cube_handle* h, double* mass, double* mass_dev) {
 
 // ------------------------------------------------------------------ kick
 // The fine kick (decode with sigma_old, encode with sigma_new;
-// pm.f90:145-154) followed by the coarse kick (decode/encode with
-// sigma_new; pm.f90:353-363), all three components per particle.
+// pm.f90:145-154) and the coarse kick (decode/encode with sigma_new;
+// pm.f90:353-363), all three components per particle, as two passes over
+// the particles like the reference's: the fine pass stores each particle's
+// re-quantised code in vp, and the coarse pass decodes it from there.
 //
-// A workgroup takes BX consecutive coarse cells of one x-row of a tile.  It
-// first stages the force nodes those cells' particles can touch in LDS:
-// fine nodes 4*c0 + [0, 4*BX] x [0, 4] x [0, 4] of the three fine
-// components (rows loaded coalesced along x) and coarse nodes
-// c0 - 1 + [0, BX+1] x [0, 2] x [0, 2] of the three coarse ones.  The
-// particles then gather their 8+8 taps per component from LDS.  A float
+// A workgroup of either pass takes BX consecutive coarse cells of one
+// x-row of a tile.  The fine pass stages the fine force nodes its cells'
+// particles can touch in LDS, 4*c0 + [0, 4*BX] x [0, 4] x [0, 4] of the
+// three components; the coarse pass the coarse nodes c0 - 1 + [0, BX+1] x
+// [0, 2] x [0, 2].  Slab rows are dealt to the wavefronts (a row index is
+// uniform, so its address is scalar) and a row's nodes to the lanes.  The
+// particles then gather their 8 taps per component from LDS.  A float
 // in-cell coordinate can round up onto the next cell's far node; those
 // rare particles read their fine taps from HBM instead.
 //
 // Each tap adds (double)(F*k*w1*w2*w3) with k = a_mid*dt/6/pi formed once
 // in the mesh precision (the reference forms F*a_mid*dt/6/pi per tap; the
-// two agree to rounding).  With probe arrays set, the kick is replaced by
-// writing the interpolated forces.  Mesh sizes are powers of two, so node
-// indices wrap with masks.
-template <typename T>
-struct KickMeshes {
-  const T* f[3];
+// two agree to rounding).  With a probe array set, a pass writes its
+// interpolated forces instead of kicking.  Mesh sizes are powers of two, so
+// node indices wrap with masks.
+struct KickFine {
+  const void* f[3];
   int64_t pf[3];
-  const T* c;  // [3][nc^3]
 };
 
+// the cell of particle p among the workgroup's BX cells (offsets cs)
+template <int BX>
+__device__ __forceinline__ int cell_of(const int64_t* cs, int64_t p) {
+  int lo = 0, hi = BX;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cs[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// block reduction of NV non-negative float maxima into slot fields
+// f0..f0+NV-1 (non-negative floats order as their bit patterns)
+template <int NV, int W>
+__device__ __forceinline__ void slot_max(const float (&mv)[NV], float (*red)[W], uint32_t* mslot, int f0) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int d = 0; d < NV; d++) {
+    float m = mv[d];
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (lane == 0) red[d][wv] = m;
+  }
+  __syncthreads();
+  if (tid < NV) {
+    float m = red[tid][0];
+    for (int i = 1; i < W; i++) m = fmaxf(m, red[tid][i]);
+    atomicMax(mslot + (blockIdx.x & (MSLOTS - 1)) * MSLOT_FIELDS + f0 + tid, __float_as_uint(m));
+  }
+}
+
 template <typename T, int BX, int NT>
-__global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp, void* vp, const int64_t* __restrict__ cstart,
-                                                  int64_t n3c, const float* __restrict__ vfield, KickMeshes<T> F, int64_t Nf,
-                                                  float a_mid, float dt, const double* __restrict__ cnew_p,
-                                                  const double* __restrict__ vdt_old, const double* __restrict__ vdt_new,
-                                                  const double* __restrict__ venc, int32_t kmax,
-                                                  uint32_t* mslot, double* probe_f, double* probe_c) {
+__global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict__ xp, void* vp,
+                                                  const int64_t* __restrict__ cstart, KickFine F, int64_t Nf, float a_mid,
+                                                  float dt, const double* __restrict__ cnew_p,
+                                                  const double* __restrict__ vdt_old, const double* __restrict__ venc,
+                                                  int32_t kmax, uint32_t* mslot, double* probe_f) {
   constexpr int W = NT / 64;                               // wavefronts
-  constexpr int XF = 4 * BX, SX = XF + 1;                  // fine slab [3][5][5][SX]
+  constexpr int XF = 4 * BX, SX = XF + 1;                  // slab [3][5][5][SX]
   constexpr int FROW = 25 * SX, NFS = 3 * FROW;
-  constexpr int NRF = 75;                                  // fine rows (d, z, y)
+  constexpr int NRF = 75;                                  // rows (d, z, y)
   constexpr int NCH = (XF + 63) / 64;                      // 64-node chunks of a row's first XF nodes
-  constexpr int KRF = (NRF + W - 1) / W;                   // fine rows per wavefront
-  constexpr int CX = BX + 2, CROW = 9 * CX, NCS = 3 * CROW; // coarse slab [3][3][3][CX]
-  constexpr int NRC = 27, KRC = (NRC + W - 1) / W;
-  static_assert(CX <= 64, "a wavefront per coarse row");
+  constexpr int KRF = (NRF + W - 1) / W;                   // rows per wavefront
   __shared__ T sF[NFS];
-  __shared__ T sC[NCS];
   __shared__ int64_t cs[BX + 1];
-  __shared__ float red[4][W];
+  __shared__ float red[1][W];
+  __shared__ double sAt[33];
   const int nbx = g.nc / BX;
   const int b = blockIdx.x;
   const int32_t c0[3] = {(b % nbx) * BX, (b / nbx) % g.nc, b / (nbx * g.nc)};
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const double c_new = *cnew_p;  // vcoef(sigma_vi_new), formed on the device by the drift
-  const uint32_t Nc = (uint32_t)g.nc, mf = (uint32_t)Nf - 1, mc = Nc - 1;
+  const uint32_t mf = (uint32_t)Nf - 1;
   const int64_t r0 = rank_of(g, c0[0], c0[1], c0[2]);
   if (tid <= BX) cs[tid] = cstart[r0 + tid];
+  atan_tab_to_lds(sAt);
   __syncthreads();
   const int64_t p0 = cs[0];
   const int32_t ntot = (int32_t)(cs[BX] - p0);
   // this thread's first particle: its position and velocity codes are
-  // requested before the slabs and its old velocities decoded while they
-  // arrive, so those latencies overlap the staging
+  // requested before the slab and its old velocities decoded while the
+  // slab arrives
   int64_t pa = p0 + tid;
   uint32_t xa[3] = {0u, 0u, 0u};
   int32_t va[3] = {0, 0, 0};
@@ -333,21 +361,19 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
 #pragma unroll
     for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
   }
-  // stage the slabs row by row: row r of the fine slab is component d =
-  // r / 25, z = (r / 5) % 5, y = r % 5, and a wavefront takes rows wv,
-  // wv + W, ...  (uniform, so the row addresses are scalar), its lanes
-  // consecutive nodes of the row; the far face x = XF of all 75 rows is
+  // row r: component d = r / 25, z = (r / 5) % 5, y = r % 5; a wavefront
+  // takes rows wv, wv + W, ...; the far face x = XF of all 75 rows is
   // loaded by the first 75 threads.  All loads are issued before the LDS
   // stores.
   {
-    T v[KRF][NCH], vfar = (T)0, vc[KRC];
+    T v[KRF][NCH], vfar = (T)0;
     const uint32_t xb = 4u * (uint32_t)c0[0];
 #pragma unroll
     for (int k = 0; k < KRF; k++) {
       const int r = wv + k * W;
       if (r < NRF) {
         const int d = r / 25, z = (r / 5) % 5, y = r % 5;
-        const T* f = d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]);
+        const T* f = (const T*)(d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]));
         const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
         const uint32_t gy = (4u * (uint32_t)c0[1] + y) & mf, gz = (4u * (uint32_t)c0[2] + z) & mf;
         // scalar row base, 32-bit byte offset per lane
@@ -361,20 +387,10 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
     }
     if (tid < NRF) {
       const int d = tid / 25, z = (tid / 5) % 5, y = tid % 5;
-      const T* f = d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]);
+      const T* f = (const T*)(d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]));
       const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
       const uint32_t gy = (4u * (uint32_t)c0[1] + y) & mf, gz = (4u * (uint32_t)c0[2] + z) & mf;
       vfar = f[(int64_t)(gz * (uint32_t)Nf + gy) * P + ((xb + XF) & mf)];
-    }
-#pragma unroll
-    for (int k = 0; k < KRC; k++) {
-      const int r = wv + k * W;  // coarse row: component r / 9, z = (r / 3) % 3, y = r % 3
-      if (r < NRC && lane < CX) {
-        const int d = r / 9, z = (r / 3) % 3, y = r % 3;
-        const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
-                       gz = (uint32_t)(c0[2] - 1 + z) & mc;
-        vc[k] = F.c[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx];
-      }
     }
     if (tid < ntot) {
 #pragma unroll
@@ -392,11 +408,6 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
       }
     }
     if (tid < NRF) sF[tid * SX + XF] = vfar;
-#pragma unroll
-    for (int k = 0; k < KRC; k++) {
-      const int r = wv + k * W;
-      if (r < NRC && lane < CX) sC[r * CX + lane] = vc[k];
-    }
   }
   __syncthreads();
   // max |F|^2 over the fine nodes this workgroup owns (the slab without its
@@ -418,7 +429,6 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
     }
   }
   const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
-  float vmax[3] = {0.f, 0.f, 0.f};
   for (int32_t k = tid; k < ntot; k += NT) {
     // this particle's codes and old velocities; the next particle's codes
     // are requested now and decoded at the end of this one
@@ -438,18 +448,11 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
 #pragma unroll
       for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
     }
-    int lo = 0, hi = BX;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (cs[mid] <= p) lo = mid; else hi = mid;
-    }
-    float vf[3];
-#pragma unroll
-    for (int e = 0; e < 3; e++) vf[e] = vfield[(r0 + lo) * 3 + e];
+    const int lo = cell_of<BX>(cs, p);
     const int32_t c[3] = {c0[0] + lo, c0[1], c0[2]};
-    T dx1[3], dx2[3], cx1[3], cx2[3];
+    T dx1[3], dx2[3];
     uint32_t f1[3], f2[3];
-    int lf[3], lc[3];
+    int lf[3];
     for (int e = 0; e < 3; e++) {
       const int il = (g.lnt >= 0 ? (c[e] & (g.nt - 1)) : c[e] % g.nt) + 1;
       const int torg = c[e] - (il - 1);
@@ -462,121 +465,207 @@ __global__ __launch_bounds__(NT) void k_kick(Geo g, const void* __restrict__ xp,
       f1[e] = (uint32_t)(CUBE_NCELL * torg + a1 - 1) & mf;
       f2[e] = (f1[e] + 1) & mf;
       lf[e] = CUBE_NCELL * (c[e] - c0[e]) + (a1 - 1 - CUBE_NCELL * (il - 1));  // 0..4*BX (x), 0..4 (y, z)
-      const T tc = (T)(((double)(torg + il - 1) + xo) - 0.5);
-      const int b1 = (int)floor(tc) + 1;  // 1-based global node: b1-1 is c-1 or c
-      cx1[e] = (T)b1 - tc;
-      cx2[e] = 1 - cx1[e];
-      lc[e] = b1 - c0[e];  // local coarse node of the lower tap: 0..BX (x), 0..1 (y, z)
     }
     const bool in_slab = lf[0] < XF && lf[1] < 4 && lf[2] < 4;
     // the 8 taps: bit 0 picks x, bit 1 y, bit 2 z (order of pm.f90:146-153)
     const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
-    int sf[8], sc[8];
-    T Fw[8], Cw[8];
+    int sf[8];
+    T Fw[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
       sf[q] = ((lf[2] + oz) * 5 + lf[1] + oy) * SX + lf[0] + ox;
-      sc[q] = ((lc[2] + oz) * 3 + lc[1] + oy) * CX + lc[0] + ox;
       Fw[q] = (ox ? dx2[0] : dx1[0]) * (oy ? dx2[1] : dx1[1]) * (oz ? dx2[2] : dx1[2]);
-      Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
     }
-    if (probe_f) {
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        double sfv = 0, scv = 0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          T Fv;
-          if (in_slab) {
-            Fv = sF[d * FROW + sf[q]];
-          } else {
-            const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
-            const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
-            Fv = F.f[d][(int64_t)rowf * F.pf[d] + (ox ? f2[0] : f1[0])];
-          }
-          sfv += (double)(Fv * Fw[q]);
-          scv += (double)(sC[d * CROW + sc[q]] * Cw[q]);
-        }
-        probe_f[3 * p + d] = sfv;
-        probe_c[3 * p + d] = scv;
-      }
-      continue;
-    }
-    // fine kick: each tap's F*k*w in the mesh precision added to the old
-    // velocity in tap order, straight from the slab (no per-tap arrays
-    // stay live through the codec stages)
-    double v[3];
+    // each tap's F*k*w in the mesh precision added to the old velocity in
+    // tap order (a probe: F*w summed in double), then re-quantised with the
+    // new sigma
     int32_t n[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      v[d] = vo[d];
+      double v = probe_f ? 0.0 : vo[d];
+      const T kd = probe_f ? (T)1 : kf;
       if (in_slab) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) v[d] = v[d] + (double)(sF[d * FROW + sf[q]] * kf * Fw[q]);
+        for (int q = 0; q < 8; q++) v = v + (double)(sF[d * FROW + sf[q]] * kd * Fw[q]);
       } else {
-        const T* Ff = F.f[d];
+        const T* Ff = (const T*)F.f[d];
         const int64_t P = F.pf[d];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
           const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
           const uint32_t rowf = (oz ? f2[2] : f1[2]) * (uint32_t)Nf + (oy ? f2[1] : f1[1]);
-          v[d] = v[d] + (double)(Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])] * kf * Fw[q]);
+          v = v + (double)(Ff[(int64_t)rowf * P + (ox ? f2[0] : f1[0])] * kd * Fw[q]);
         }
       }
-      // re-quantise with the new sigma: encode, then decode
-      n[d] = venc_fast(venc, kmax, g.izipv, c_new * v[d]);
+      if (probe_f) probe_f[3 * p + d] = v;
+      else n[d] = venc_fast(venc, sAt, kmax, g.izipv, c_new * v);
     }
+    if (!probe_f) {
 #pragma unroll
-    for (int d = 0; d < 3; d++) v[d] = vdecode(vdt_new, n[d], g.izipv);
-    // coarse kick from the coarse slab, then encode
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[d] = v[d] + (double)(sC[d * CROW + sc[q]] * kf * Cw[q]);
-      const float av = (float)fabs(v[d] + (double)vf[d]);
-      vmax[d] = fmaxf(vmax[d], av);
-      n[d] = venc_fast(venc, kmax, g.izipv, c_new * v[d]);
+      for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, n[d]);
     }
-#pragma unroll
-    for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, n[d]);
     if (kn < ntot) {
 #pragma unroll
       for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_old, va[e], g.izipv);
     }
   }
-  {
-    const float mv[4] = {vmax[0], vmax[1], vmax[2], f2loc};
+  const float mv[1] = {f2loc};
+  slot_max<1, W>(mv, red, mslot, 0);  // slot field 0: max |F|^2
+}
+
+template <typename T, int BX, int NT>
+__global__ __launch_bounds__(NT) void k_kick_coarse(Geo g, const void* __restrict__ xp, void* vp,
+                                                    const int64_t* __restrict__ cstart, int64_t n3c,
+                                                    const float* __restrict__ vfield, const T* __restrict__ Fc,
+                                                    float a_mid, float dt, const double* __restrict__ cnew_p,
+                                                    const double* __restrict__ vdt_new, const double* __restrict__ venc,
+                                                    int32_t kmax, uint32_t* mslot, double* probe_c) {
+  constexpr int W = NT / 64;
+  constexpr int CX = BX + 2, CROW = 9 * CX, NCS = 3 * CROW;  // slab [3][3][3][CX]
+  constexpr int NRC = 27, KRC = (NRC + W - 1) / W;
+  static_assert(CX <= 64, "a wavefront per coarse row");
+  __shared__ T sC[NCS];
+  __shared__ int64_t cs[BX + 1];
+  __shared__ float red[3][W];
+  __shared__ double sAt[33];
+  const int nbx = g.nc / BX;
+  const int b = blockIdx.x;
+  const int32_t c0[3] = {(b % nbx) * BX, (b / nbx) % g.nc, b / (nbx * g.nc)};
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double c_new = *cnew_p;
+  const uint32_t Nc = (uint32_t)g.nc, mc = Nc - 1;
+  const int64_t r0 = rank_of(g, c0[0], c0[1], c0[2]);
+  if (tid <= BX) cs[tid] = cstart[r0 + tid];
+  atan_tab_to_lds(sAt);
+  __syncthreads();
+  const int64_t p0 = cs[0];
+  const int32_t ntot = (int32_t)(cs[BX] - p0);
+  int64_t pa = p0 + tid;
+  uint32_t xa[3] = {0u, 0u, 0u};
+  int32_t va[3] = {0, 0, 0};
+  double vo_a[3] = {0.0, 0.0, 0.0};
+  if (tid < ntot) {
 #pragma unroll
-    for (int d = 0; d < 4; d++) {
-      float m = mv[d];
-      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-      if (lane == 0) red[d][wv] = m;
+    for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pa + e, g.izipx);
+#pragma unroll
+    for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
+  }
+  // row r: component r / 9, z = (r / 3) % 3, y = r % 3
+  {
+    T vc[KRC];
+#pragma unroll
+    for (int k = 0; k < KRC; k++) {
+      const int r = wv + k * W;
+      if (r < NRC && lane < CX) {
+        const int d = r / 9, z = (r / 3) % 3, y = r % 3;
+        const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
+                       gz = (uint32_t)(c0[2] - 1 + z) & mc;
+        vc[k] = Fc[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx];
+      }
     }
-    __syncthreads();
-    if (tid < 4) {
-      float m = red[tid][0];
-      for (int i = 1; i < W; i++) m = fmaxf(m, red[tid][i]);
-      // slot fields: 0 max |F|^2, 1..3 vmax (non-negative floats order as
-      // their bit patterns)
-      atomicMax(mslot + (blockIdx.x & (MSLOTS - 1)) * MSLOT_FIELDS + (tid == 3 ? 0 : tid + 1), __float_as_uint(m));
+    if (tid < ntot) {
+#pragma unroll
+      for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_new, va[e], g.izipv);
+    }
+#pragma unroll
+    for (int k = 0; k < KRC; k++) {
+      const int r = wv + k * W;
+      if (r < NRC && lane < CX) sC[r * CX + lane] = vc[k];
     }
   }
+  __syncthreads();
+  const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
+  float vmax[3] = {0.f, 0.f, 0.f};
+  for (int32_t k = tid; k < ntot; k += NT) {
+    const int64_t p = pa;
+    uint32_t xu[3];
+    double vo[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      xu[e] = xa[e];
+      vo[e] = vo_a[e];
+    }
+    const int32_t kn = k + NT;
+    if (kn < ntot) {
+      pa = p0 + kn;
+#pragma unroll
+      for (int e = 0; e < 3; e++) xa[e] = load_xu(xp, 3 * pa + e, g.izipx);
+#pragma unroll
+      for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
+    }
+    const int lo = cell_of<BX>(cs, p);
+    float vf[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) vf[e] = vfield[(r0 + lo) * 3 + e];
+    const int32_t c[3] = {c0[0] + lo, c0[1], c0[2]};
+    T cx1[3], cx2[3];
+    int lc[3];
+    for (int e = 0; e < 3; e++) {
+      const double xo = xoff(xu[e], g.izipx);
+      const T tc = (T)(((double)c[e] + xo) - 0.5);
+      const int b1 = (int)floor(tc) + 1;  // 1-based global node: b1-1 is c-1 or c
+      cx1[e] = (T)b1 - tc;
+      cx2[e] = 1 - cx1[e];
+      lc[e] = b1 - c0[e];  // local node of the lower tap: 0..BX (x), 0..1 (y, z)
+    }
+    const int ord[8] = {0, 1, 2, 4, 6, 5, 3, 7};
+    int sc[8];
+    T Cw[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int o = ord[q], ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+      sc[q] = ((lc[2] + oz) * 3 + lc[1] + oy) * CX + lc[0] + ox;
+      Cw[q] = (ox ? cx2[0] : cx1[0]) * (oy ? cx2[1] : cx1[1]) * (oz ? cx2[2] : cx1[2]);
+    }
+    // the velocity decoded from the fine pass's code, the coarse taps in
+    // tap order, vmax of the kicked velocity plus the cell's vfield, and the
+    // final code (a probe: F*w summed in double)
+    int32_t n[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double v = probe_c ? 0.0 : vo[d];
+      const T kd = probe_c ? (T)1 : kf;
+#pragma unroll
+      for (int q = 0; q < 8; q++) v = v + (double)(sC[d * CROW + sc[q]] * kd * Cw[q]);
+      if (probe_c) {
+        probe_c[3 * p + d] = v;
+      } else {
+        vmax[d] = fmaxf(vmax[d], (float)fabs(v + (double)vf[d]));
+        n[d] = venc_fast(venc, sAt, kmax, g.izipv, c_new * v);
+      }
+    }
+    if (!probe_c) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) store_v(vp, 3 * p + d, g.izipv, n[d]);
+    }
+    if (kn < ntot) {
+#pragma unroll
+      for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_new, va[e], g.izipv);
+    }
+  }
+  slot_max<3, W>(vmax, red, mslot, 1);  // slot fields 1..3: vmax
 }
 
 template <typename T, int BX, int NT>
 static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float a_mid, float dt,
                         const double* c_new, const double* vo, const double* vn, double* probe_f, double* probe_c) {
   Geo g = geo_of(h);
-  KickMeshes<T> F;
+  KickFine F;
   for (int d = 0; d < 3; d++) {
-    F.f[d] = (const T*)Ff[d];
+    F.f[d] = Ff[d];
     F.pf[d] = pf[d];
   }
-  F.c = (const T*)h->fc;
-  k_kick<T, BX, NT><<<(unsigned)(h->n3c / BX), NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, F, h->fine.N,
-                                                                 a_mid, dt, c_new, vo, vn, h->venc, h->vkmax, h->mslots,
-                                                                 probe_f, probe_c);
+  const unsigned nb = (unsigned)(h->n3c / BX);
+  int pr = prof_begin(h, PS_KICK);
+  k_kick_fine<T, BX, NT><<<nb, NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, F, h->fine.N, a_mid, dt, c_new, vo,
+                                                   h->venc, h->vkmax, h->mslots, probe_f);
+  prof_end(h, pr);
+  pr = prof_begin(h, PS_KICK_C);
+  k_kick_coarse<T, BX, NT><<<nb, NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, (const T*)h->fc,
+                                                     a_mid, dt, c_new, vn, h->venc, h->vkmax, h->mslots, probe_c);
+  prof_end(h, pr);
 }
 
 template <typename T>
@@ -599,13 +688,12 @@ int kick_all(cube_handle* h, const void* const Ff[3], const int64_t pf[3], float
   const double* c_new = h->dscal;  // and its table h->vdt_new, both written by the drift
   const double* vn = h->vdt_new;
   if (vmax_bits != f2max_bits + 1) return cube_fail(h, CUBE_ERR_ARG, "kick: vmax must follow f2max");
+  if (!probe_f != !probe_c) return cube_fail(h, CUBE_ERR_ARG, "kick: probe both force arrays or neither");
   int r0 = fill_u32(h, h->mslots, MSLOTS * MSLOT_FIELDS, 0u);
   if (r0) return r0;
-  int pr = prof_begin(h, PS_KICK);
   int r = h->prec == 4
               ? kick_pick<float>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c)
               : kick_pick<double>(h, Ff, pf, a_mid, dt, c_new, vo, vn, probe_f, probe_c);
-  prof_end(h, pr);
   if (r) return r;
   HIP_TRY(h, hipGetLastError());
   // max |F|^2 always; vmax only for a kick (a probe leaves it unset)
