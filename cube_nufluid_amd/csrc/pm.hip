@@ -300,29 +300,41 @@ __device__ __forceinline__ int cell_of(const int64_t* cs, int64_t p) {
   return lo;
 }
 
+// maximum over the wavefront of non-negative floats as their bit patterns
+// (which order as the floats do), by DPP: pairs and quads, half rows, rows,
+// then rows 0-1 into row 1 and rows 0-3 into row 3; lane 63 holds it
+__device__ __forceinline__ uint32_t wave_max_bits(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false));  // row_half_mirror
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false));  // row_mirror
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // block reduction of NV non-negative float maxima into slot fields
-// f0..f0+NV-1 (non-negative floats order as their bit patterns)
+// f0..f0+NV-1
 template <int NV, int W>
-__device__ __forceinline__ void slot_max(const float (&mv)[NV], float (*red)[W], uint32_t* mslot, int f0) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+__device__ __forceinline__ void slot_max(const float (&mv)[NV], uint32_t (*red)[W], uint32_t* mslot, int f0) {
+  const int tid = threadIdx.x, wv = tid >> 6;
 #pragma unroll
   for (int d = 0; d < NV; d++) {
-    float m = mv[d];
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if (lane == 0) red[d][wv] = m;
+    const uint32_t m = wave_max_bits(__float_as_uint(mv[d]));
+    if ((tid & 63) == 0) red[d][wv] = m;
   }
   __syncthreads();
   if (tid < NV) {
-    float m = red[tid][0];
-    for (int i = 1; i < W; i++) m = fmaxf(m, red[tid][i]);
-    atomicMax(mslot + (blockIdx.x & (MSLOTS - 1)) * MSLOT_FIELDS + f0 + tid, __float_as_uint(m));
+    uint32_t m = red[tid][0];
+    for (int i = 1; i < W; i++) m = max(m, red[tid][i]);
+    atomicMax(mslot + (blockIdx.x & (MSLOTS - 1)) * MSLOT_FIELDS + f0 + tid, m);
   }
 }
 
 template <typename T, int BX, int NT>
 __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict__ xp, void* vp,
-                                                  const int64_t* __restrict__ cstart, KickFine F, int64_t Nf, float a_mid,
-                                                  float dt, const double* __restrict__ cnew_p,
+                                                  const int64_t* __restrict__ cstart, KickFine F, int64_t Nf, T kf,
+                                                  const double* __restrict__ cnew_p,
                                                   const double* __restrict__ vdt_old, const double* __restrict__ venc,
                                                   int32_t kmax, uint32_t* mslot, double* probe_f) {
   constexpr int W = NT / 64;                               // wavefronts
@@ -330,10 +342,9 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
   constexpr int FROW = 25 * SX, NFS = 3 * FROW;
   constexpr int NRF = 75;                                  // rows (d, z, y)
   constexpr int NCH = (XF + 63) / 64;                      // 64-node chunks of a row's first XF nodes
-  constexpr int KRF = (NRF + W - 1) / W;                   // rows per wavefront
   __shared__ T sF[NFS];
   __shared__ int64_t cs[BX + 1];
-  __shared__ float red[1][W];
+  __shared__ uint32_t red[1][W];
   __shared__ double sAt[33];
   const int nbx = g.nc / BX;
   const int b = blockIdx.x;
@@ -361,27 +372,34 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
 #pragma unroll
     for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
   }
-  // row r: component d = r / 25, z = (r / 5) % 5, y = r % 5; a wavefront
-  // takes rows wv, wv + W, ...; the far face x = XF of all 75 rows is
-  // loaded by the first 75 threads.  All loads are issued before the LDS
-  // stores.
+  // plane q = 5*d + z of the slab (component d, z row) holds the 5 rows
+  // y = 0..4; a wavefront takes planes wv, wv + W, ... (uniform, so plane
+  // and row addresses are scalar) and its lanes consecutive nodes of a row,
+  // at byte offsets that are the same for every row.  The far face x = XF
+  // of all 75 rows is loaded by the first 75 threads.  All loads are issued
+  // before the LDS stores.
   {
-    T v[KRF][NCH], vfar = (T)0;
+    constexpr int NPL = 15, KPL = (NPL + W - 1) / W;
+    T v[KPL][5][NCH], vfar = (T)0;
     const uint32_t xb = 4u * (uint32_t)c0[0];
+    uint32_t xo[NCH];
 #pragma unroll
-    for (int k = 0; k < KRF; k++) {
-      const int r = wv + k * W;
-      if (r < NRF) {
-        const int d = r / 25, z = (r / 5) % 5, y = r % 5;
+    for (int ch = 0; ch < NCH; ch++) xo[ch] = ((xb + (uint32_t)(ch * 64 + lane)) & mf) * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int k = 0; k < KPL; k++) {
+      const int q = wv + k * W;
+      if (q < NPL) {
+        const int d = q / 5, z = q % 5;
         const T* f = (const T*)(d == 0 ? F.f[0] : (d == 1 ? F.f[1] : F.f[2]));
         const int64_t P = d == 0 ? F.pf[0] : (d == 1 ? F.pf[1] : F.pf[2]);
-        const uint32_t gy = (4u * (uint32_t)c0[1] + y) & mf, gz = (4u * (uint32_t)c0[2] + z) & mf;
-        // scalar row base, 32-bit byte offset per lane
-        const char* row = (const char*)(f + (int64_t)(gz * (uint32_t)Nf + gy) * P);
+        const uint32_t gz = (4u * (uint32_t)c0[2] + z) & mf;
+        const T* plane = f + (int64_t)gz * Nf * P;
 #pragma unroll
-        for (int ch = 0; ch < NCH; ch++) {
-          const int x = ch * 64 + lane;
-          if (x < XF) v[k][ch] = *(const T*)(row + (uint32_t)(((xb + x) & mf) * (uint32_t)sizeof(T)));
+        for (int y = 0; y < 5; y++) {
+          const char* row = (const char*)(plane + (int64_t)((4u * (uint32_t)c0[1] + y) & mf) * P);
+#pragma unroll
+          for (int ch = 0; ch < NCH; ch++)
+            if (ch * 64 + lane < XF) v[k][y][ch] = *(const T*)(row + xo[ch]);
         }
       }
     }
@@ -397,14 +415,16 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
       for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_old, va[e], g.izipv);
     }
 #pragma unroll
-    for (int k = 0; k < KRF; k++) {
-      const int r = wv + k * W;
-      if (r < NRF) {
+    for (int k = 0; k < KPL; k++) {
+      const int q = wv + k * W;
+      if (q < NPL) {
 #pragma unroll
-        for (int ch = 0; ch < NCH; ch++) {
-          const int x = ch * 64 + lane;
-          if (x < XF) sF[r * SX + x] = v[k][ch];
-        }
+        for (int y = 0; y < 5; y++)
+#pragma unroll
+          for (int ch = 0; ch < NCH; ch++) {
+            const int x = ch * 64 + lane;
+            if (x < XF) sF[(q * 5 + y) * SX + x] = v[k][y][ch];
+          }
       }
     }
     if (tid < NRF) sF[tid * SX + XF] = vfar;
@@ -412,15 +432,15 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
   __syncthreads();
   // max |F|^2 over the fine nodes this workgroup owns (the slab without its
   // far faces; the workgroups' owned nodes tile the mesh), summed in float
-  // in component order: the dt_fine input of pm.f90.  Chunk q of 64 nodes:
-  // row q / NCH of the 16 (z, y) rows, x from (q % NCH) * 64.
+  // in component order: the dt_fine input of pm.f90.  Thread tid takes x =
+  // tid % XF of the 16 (z, y) rows tid / XF, tid / XF + NT / XF, ...
+  static_assert(NT % XF == 0, "whole rows per pass");
   float f2loc = 0.f;
+  {
+    const int x = tid % XF;
 #pragma unroll
-  for (int k = 0; k < (16 * NCH + W - 1) / W; k++) {
-    const int q = wv + k * W;
-    const int x = (q % NCH) * 64 + lane, y = (q / NCH) % 4, z = (q / NCH) / 4;
-    if (q < 16 * NCH && x < XF) {
-      const int o = (z * 5 + y) * SX + x;
+    for (int row = tid / XF; row < 16; row += NT / XF) {
+      const int o = ((row >> 2) * 5 + (row & 3)) * SX + x;
       const float a = (float)sF[o], bq = (float)sF[FROW + o], c = (float)sF[2 * FROW + o];
       float s2 = a * a;
       s2 = fmaf(bq, bq, s2);
@@ -428,7 +448,6 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
       f2loc = fmaxf(f2loc, s2);
     }
   }
-  const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   for (int32_t k = tid; k < ntot; k += NT) {
     // this particle's codes and old velocities; the next particle's codes
     // are requested now and decoded at the end of this one
@@ -517,17 +536,16 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
 template <typename T, int BX, int NT>
 __global__ __launch_bounds__(NT) void k_kick_coarse(Geo g, const void* __restrict__ xp, void* vp,
                                                     const int64_t* __restrict__ cstart, int64_t n3c,
-                                                    const float* __restrict__ vfield, const T* __restrict__ Fc,
-                                                    float a_mid, float dt, const double* __restrict__ cnew_p,
+                                                    const float* __restrict__ vfield, const T* __restrict__ Fc, T kf,
+                                                    const double* __restrict__ cnew_p,
                                                     const double* __restrict__ vdt_new, const double* __restrict__ venc,
                                                     int32_t kmax, uint32_t* mslot, double* probe_c) {
   constexpr int W = NT / 64;
   constexpr int CX = BX + 2, CROW = 9 * CX, NCS = 3 * CROW;  // slab [3][3][3][CX]
-  constexpr int NRC = 27, KRC = (NRC + W - 1) / W;
   static_assert(CX <= 64, "a wavefront per coarse row");
   __shared__ T sC[NCS];
   __shared__ int64_t cs[BX + 1];
-  __shared__ float red[3][W];
+  __shared__ uint32_t red[3][W];
   __shared__ double sAt[33];
   const int nbx = g.nc / BX;
   const int b = blockIdx.x;
@@ -552,31 +570,43 @@ __global__ __launch_bounds__(NT) void k_kick_coarse(Geo g, const void* __restric
 #pragma unroll
     for (int e = 0; e < 3; e++) va[e] = load_v(vp, 3 * pa + e, g.izipv);
   }
-  // row r: component r / 9, z = (r / 3) % 3, y = r % 3
+  // plane q = 3*d + z (component d, z row) holds the 3 rows y = 0..2; a
+  // wavefront takes planes wv, wv + W, ..., its first CX lanes the nodes
   {
-    T vc[KRC];
+    constexpr int NPL = 9, KPL = (NPL + W - 1) / W;
+    T vc[KPL][3];
+    const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc;
 #pragma unroll
-    for (int k = 0; k < KRC; k++) {
-      const int r = wv + k * W;
-      if (r < NRC && lane < CX) {
-        const int d = r / 9, z = (r / 3) % 3, y = r % 3;
-        const uint32_t gx = (uint32_t)(c0[0] - 1 + lane) & mc, gy = (uint32_t)(c0[1] - 1 + y) & mc,
-                       gz = (uint32_t)(c0[2] - 1 + z) & mc;
-        vc[k] = Fc[(size_t)d * n3c + (gz * Nc + gy) * Nc + gx];
+    for (int k = 0; k < KPL; k++) {
+      const int q = wv + k * W;
+      if (q < NPL) {
+        const int d = q / 3, z = q % 3;
+        const uint32_t gz = (uint32_t)(c0[2] - 1 + z) & mc;
+        const T* plane = Fc + (size_t)d * n3c + (size_t)gz * Nc * Nc;
+#pragma unroll
+        for (int y = 0; y < 3; y++) {
+          const T* row = plane + (size_t)(((uint32_t)(c0[1] - 1 + y) & mc) * Nc);
+          if (lane < CX) vc[k][y] = row[gx];
+        }
       }
     }
     if (tid < ntot) {
 #pragma unroll
       for (int e = 0; e < 3; e++) vo_a[e] = vdecode(vdt_new, va[e], g.izipv);
     }
+    // the slab holds F*k (F for a probe): each tap is then (F*k)*w
+    const T kd = probe_c ? (T)1 : kf;
 #pragma unroll
-    for (int k = 0; k < KRC; k++) {
-      const int r = wv + k * W;
-      if (r < NRC && lane < CX) sC[r * CX + lane] = vc[k];
+    for (int k = 0; k < KPL; k++) {
+      const int q = wv + k * W;
+      if (q < NPL) {
+#pragma unroll
+        for (int y = 0; y < 3; y++)
+          if (lane < CX) sC[(q * 3 + y) * CX + lane] = vc[k][y] * kd;
+      }
     }
   }
   __syncthreads();
-  const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   float vmax[3] = {0.f, 0.f, 0.f};
   for (int32_t k = tid; k < ntot; k += NT) {
     const int64_t p = pa;
@@ -626,9 +656,8 @@ __global__ __launch_bounds__(NT) void k_kick_coarse(Geo g, const void* __restric
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       double v = probe_c ? 0.0 : vo[d];
-      const T kd = probe_c ? (T)1 : kf;
 #pragma unroll
-      for (int q = 0; q < 8; q++) v = v + (double)(sC[d * CROW + sc[q]] * kd * Cw[q]);
+      for (int q = 0; q < 8; q++) v = v + (double)(sC[d * CROW + sc[q]] * Cw[q]);
       if (probe_c) {
         probe_c[3 * p + d] = v;
       } else {
@@ -658,13 +687,15 @@ static void kick_launch(cube_handle* h, const void* const Ff[3], const int64_t p
     F.pf[d] = pf[d];
   }
   const unsigned nb = (unsigned)(h->n3c / BX);
+  // k = a_mid*dt/6/pi in the mesh precision, in this operation order
+  const T kf = (T)a_mid * (T)dt / (T)6 / (T)PI_F;
   int pr = prof_begin(h, PS_KICK);
-  k_kick_fine<T, BX, NT><<<nb, NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, F, h->fine.N, a_mid, dt, c_new, vo,
-                                                   h->venc, h->vkmax, h->mslots, probe_f);
+  k_kick_fine<T, BX, NT><<<nb, NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, F, h->fine.N, kf, c_new, vo, h->venc,
+                                                   h->vkmax, h->mslots, probe_f);
   prof_end(h, pr);
   pr = prof_begin(h, PS_KICK_C);
-  k_kick_coarse<T, BX, NT><<<nb, NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, (const T*)h->fc,
-                                                     a_mid, dt, c_new, vn, h->venc, h->vkmax, h->mslots, probe_c);
+  k_kick_coarse<T, BX, NT><<<nb, NT, 0, h->stream>>>(g, h->xp, h->vp, h->cstart, h->n3c, h->vfield, (const T*)h->fc, kf,
+                                                     c_new, vn, h->venc, h->vkmax, h->mslots, probe_c);
   prof_end(h, pr);
 }
 
