@@ -59,9 +59,11 @@ def algorithmic_bytes(slot, cfg, npart):
         "fft_zconv": 8.5 * mf,
         "fft_y_inv": 8 * mf,
         "fft_x_c2r": 8 * mf,
-        # xp, vp read + write; three fine and three coarse force meshes,
-        # vfield and cell offsets
-        "kick": n * (bx + 2 * bv) + 12 * mf + 32 * mc,
+        # fine pass: xp, vp read + write, three fine force meshes, cell
+        # offsets; coarse pass: the same particle arrays, three coarse
+        # force meshes, vfield, cell offsets
+        "kick_fine": n * (bx + 2 * bv) + 12 * mf + 8 * mc,
+        "kick_coarse": n * (bx + 2 * bv) + 32 * mc,
         "fft_coarse": None,
     }
     return table.get(slot)
@@ -598,6 +600,15 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(names[dom], args.config),
                 "algorithmic_bytes_per_launch": abytes, "mean_launch_ms": round(avg_ms, 4)}
+    kick_info = None
+    if all(k in names and cnt[names.index(k)] for k in ("kick_fine", "kick_coarse")):
+        # the two kick passes together against the bytes one pass over the
+        # particles would need (xp, vp read + write once, all six force
+        # meshes, vfield, cell offsets)
+        kms = [ms[names.index(k)] / cnt[names.index(k)] for k in ("kick_fine", "kick_coarse")]
+        kb = npart * (3 * cfg.izipx + 6 * cfg.izipv) + 12 * float(cfg.nf) ** 3 + 32 * float(cfg.nc) ** 3
+        kick_info = {"fine_ms": round(kms[0], 3), "coarse_ms": round(kms[1], 3), "ms": round(sum(kms), 3),
+                     "algorithmic_bytes": kb, "achieved_gbs": round(kb / (sum(kms) * 1e-3) / 1e9, 1)}
     pp_info = None
     if cfg.pp_on and "pp" in names and cnt[names.index("pp")]:
         s_pp = names.index("pp")
@@ -627,6 +638,7 @@ def main():
                    "box_mpc_h": cfg.box, "z_start": z0, "state": state, "izipx": cfg.izipx, "izipv": cfg.izipv,
                    "pp": bool(cfg.pp_on), "pp_range": cfg.pp_range, "parallelism": "single GPU"},
         "roofline": roofline,
+        "kick": kick_info,
         "pp": pp_info,
         "per_kernel_ms": per_kernel,
     }

@@ -347,7 +347,9 @@ __global__ __launch_bounds__(NT) void k_kick_fine(Geo g, const void* __restrict_
   __shared__ uint32_t red[1][W];
   __shared__ double sAt[33];
   const int nbx = g.nc / BX;
-  const int b = blockIdx.x;
+  // XCD-contiguous block order: x-adjacent segments share the line that
+  // holds a slab row's far node, y-adjacent ones four of their five rows
+  const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int32_t c0[3] = {(b % nbx) * BX, (b / nbx) % g.nc, b / (nbx * g.nc)};
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
