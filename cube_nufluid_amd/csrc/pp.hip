@@ -348,30 +348,31 @@ struct PPacc<double> {
   __device__ __forceinline__ double get(int e) const { return a[e]; }
 };
 
-// pair term of ext_pp_force.f90:99-108 for r > rsoft: m pc(r/16) dx/r^3
+// pair term of ext_pp_force.f90:99-108 for r > rsoft: m pc(r/16) dx/r^3,
+// pc = 1 - 7/4 rc^3 + 3/4 rc^5 as 1 + rc^3 (3/4 rc^2 - 7/4).  FP32 with
+// explicit multiply-adds (the pair force has a tolerance bar, not a bit
+// bar); r2 from multiply-adds is within 2 ulps of the exact sum of the
+// squared offsets, inside the margin of the r > rsoft test
+__device__ __forceinline__ float pf_r2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+__device__ __forceinline__ float pf_scale(float r2, float ri, float mass) {
+  const float rc = r2 * ri * (1.0f / 16.0f);
+  const float rc2 = rc * rc;
+  const float pc = fmaf(rc2 * rc, fmaf(0.75f, rc2, -1.75f), 1.0f);
+  return mass * pc * (ri * ri * ri);
+}
 __device__ __forceinline__ bool pf_pair(float dx, float dy, float dz, float mass, float rsoft, float rsoft2, float& sc) {
-  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float r2 = pf_r2(dx, dy, dz);
   if (!pf_outside_rsoft(r2, dx, dy, dz, rsoft, rsoft2)) return false;
-  const float ri = rsqrtf(r2);
-  const float r = r2 * ri;
-  const float rc = r * (1.0f / 16.0f);
-  const float rc3 = rc * rc * rc;
-  const float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
-  sc = mass * pc * (ri * ri * ri);
+  sc = pf_scale(r2, rsqrtf(r2), mass);
   return true;
 }
 // the same pair term with the r > rsoft test as a select; an included pair
 // has r2 > rsoft2 > 0 to rounding, which keeps the hardware reciprocal
 // square root away from denormal inputs
 __device__ __forceinline__ float pf_pair_sel(float dx, float dy, float dz, float mass, float rsoft, float rsoft2) {
-  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float r2 = pf_r2(dx, dy, dz);
   const bool ok = pf_outside_rsoft(r2, dx, dy, dz, rsoft, rsoft2);
-  const float ri = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
-  const float r = r2 * ri;
-  const float rc = r * (1.0f / 16.0f);
-  const float rc3 = rc * rc * rc;
-  const float pc = 1.0f - 1.75f * rc3 + 0.75f * rc3 * rc * rc;
-  const float sc = mass * pc * (ri * ri * ri);
+  const float sc = pf_scale(r2, __builtin_amdgcn_rsqf(ok ? r2 : 1.0f), mass);
   return ok ? sc : 0.0f;
 }
 __device__ __forceinline__ bool pf_pair(double dx, double dy, double dz, double mass, double rsoft, double rsoft2, double& sc) {
@@ -459,9 +460,9 @@ __device__ __forceinline__ void pf_force(const EntT* ent, const RunT start, int 
           // branch-free: an excluded pair adds sc = 0 (a signed zero, which
           // leaves the running sum unchanged)
           sc = pf_pair_sel(dx, dy, dz, mass, rsoft, rsoft2);
-          sx += sc * dx;
-          sy += sc * dy;
-          sz += sc * dz;
+          sx = fma(sc, dx, sx);
+          sy = fma(sc, dy, sy);
+          sz = fma(sc, dz, sz);
         } else {
           if (!pf_pair(dx, dy, dz, mass, rsoft, rsoft2, sc)) continue;
           acc.add(0, sc * dx);
