@@ -10,6 +10,7 @@
 #include "cube_internal.hpp"
 
 #define CIC_NT 256
+#define CIC_U 4  // particles per thread and batch in the CIC kernels
 
 // --------------------------------------------------------------- helpers
 // Load cstart for the cells of a SPAN^3 region around block origin c0 (lo
@@ -98,36 +99,51 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_fine(Geo g, const void* __restri
   for (int i = threadIdx.x; i < F * F * F; i += CIC_NT) s[i] = 0ull;
   load_region<SPAN>(g, cstart, c0x, c0y, c0z, 1, beg, off);
   const int32_t total = off[NCELL];
-  for (int32_t t = threadIdx.x; t < total; t += CIC_NT) {
-    int ci = seg_find(off, NCELL, t);
-    int64_t p = beg[ci] + (t - off[ci]);
-    int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
-    int l0[3] = {l0x, l0y, l0z};
-    T dx1[3], dx2[3];
-    int n1[3];
-    for (int e = 0; e < 3; e++) {
-      int il = l0[e] + rel[e] + 1;  // tile-local 1-based cell index (0 = left ghost)
-      uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
-      T tx;
-      if constexpr (CENTRE)
-        tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx) - 0.5);
-      else
-        tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
-      int a1 = (int)floor(tx) + 1;
-      dx1[e] = (T)a1 - tx;
-      dx2[e] = 1 - dx1[e];
-      n1[e] = (a1 - 1) - CUBE_NCELL * l0[e];  // block-local node of tap 1
-    }
-    const T w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
-                    dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
-                    dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
-                    dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
-    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+  // CIC_U particles per thread and batch: their cell searches and position
+  // loads are issued together, then their taps added
+  for (int32_t t0 = threadIdx.x; t0 < total; t0 += CIC_NT * CIC_U) {
+    int cib[CIC_U];
+    uint32_t ub[CIC_U][3];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
-      if ((unsigned)x < (unsigned)F && (unsigned)y < (unsigned)F && (unsigned)z < (unsigned)F)
-        atomicAdd(&s[(z * F + y) * F + x], (unsigned long long)llrint((double)w[q] * CIC_FIX));
+    for (int j = 0; j < CIC_U; j++) {
+      const int32_t t = t0 + j * CIC_NT;
+      cib[j] = t < total ? seg_find(off, NCELL, t) : 0;
+      const int64_t p = beg[cib[j]] + (t - off[cib[j]]);
+#pragma unroll
+      for (int e = 0; e < 3; e++) ub[j][e] = t < total ? load_xu(xp, 3 * p + e, g.izipx) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < CIC_U; j++) {
+      if (t0 + j * CIC_NT >= total) break;
+      const int ci = cib[j];
+      int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
+      int l0[3] = {l0x, l0y, l0z};
+      T dx1[3], dx2[3];
+      int n1[3];
+      for (int e = 0; e < 3; e++) {
+        int il = l0[e] + rel[e] + 1;  // tile-local 1-based cell index (0 = left ghost)
+        uint32_t u = ub[j][e];
+        T tx;
+        if constexpr (CENTRE)
+          tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx) - 0.5);
+        else
+          tx = (T)((double)(CUBE_NCELL * (il - 1)) + (double)CUBE_NCELL * xoff(u, g.izipx));
+        int a1 = (int)floor(tx) + 1;
+        dx1[e] = (T)a1 - tx;
+        dx2[e] = 1 - dx1[e];
+        n1[e] = (a1 - 1) - CUBE_NCELL * l0[e];  // block-local node of tap 1
+      }
+      const T w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
+                      dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
+                      dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
+                      dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
+      const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
+        if ((unsigned)x < (unsigned)F && (unsigned)y < (unsigned)F && (unsigned)z < (unsigned)F)
+          atomicAdd(&s[(z * F + y) * F + x], (unsigned long long)llrint((double)w[q] * CIC_FIX));
+      }
     }
   }
   __syncthreads();
@@ -171,32 +187,45 @@ __global__ __launch_bounds__(CIC_NT) void k_cic_coarse(Geo g, const void* __rest
   for (int i = threadIdx.x; i < BC * BC * BC; i += CIC_NT) s[i] = 0ull;
   load_region<SPAN>(g, cstart, c0x, c0y, c0z, 1, beg, off);
   const int32_t total = off[NCELL];
-  for (int32_t t = threadIdx.x; t < total; t += CIC_NT) {
-    int ci = seg_find(off, NCELL, t);
-    int64_t p = beg[ci] + (t - off[ci]);
-    int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
-    int l0[3] = {l0x, l0y, l0z};
-    T dx1[3], dx2[3];
-    int n1[3];
-    for (int e = 0; e < 3; e++) {
-      int il = l0[e] + rel[e] + 1;
-      uint32_t u = load_xu(xp, 3 * p + e, g.izipx);
-      T tx = (T)(((double)(il - 1) + xoff(u, g.izipx)) - 0.5);
-      int a1 = (int)floor(tx) + 1;  // 1-based tile-local node
-      dx1[e] = (T)a1 - tx;
-      dx2[e] = 1 - dx1[e];
-      n1[e] = (a1 - 1) - l0[e];
-    }
-    const T w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
-                    dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
-                    dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
-                    dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
-    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+  for (int32_t t0 = threadIdx.x; t0 < total; t0 += CIC_NT * CIC_U) {
+    int cib[CIC_U];
+    uint32_t ub[CIC_U][3];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
-      if ((unsigned)x < (unsigned)BC && (unsigned)y < (unsigned)BC && (unsigned)z < (unsigned)BC)
-        atomicAdd(&s[(z * BC + y) * BC + x], (unsigned long long)llrint((double)w[q] * CIC_FIX));
+    for (int j = 0; j < CIC_U; j++) {
+      const int32_t t = t0 + j * CIC_NT;
+      cib[j] = t < total ? seg_find(off, NCELL, t) : 0;
+      const int64_t p = beg[cib[j]] + (t - off[cib[j]]);
+#pragma unroll
+      for (int e = 0; e < 3; e++) ub[j][e] = t < total ? load_xu(xp, 3 * p + e, g.izipx) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < CIC_U; j++) {
+      if (t0 + j * CIC_NT >= total) break;
+      const int ci = cib[j];
+      int rel[3] = {ci % SPAN - 1, (ci / SPAN) % SPAN - 1, ci / (SPAN * SPAN) - 1};
+      int l0[3] = {l0x, l0y, l0z};
+      T dx1[3], dx2[3];
+      int n1[3];
+      for (int e = 0; e < 3; e++) {
+        int il = l0[e] + rel[e] + 1;
+        uint32_t u = ub[j][e];
+        T tx = (T)(((double)(il - 1) + xoff(u, g.izipx)) - 0.5);
+        int a1 = (int)floor(tx) + 1;  // 1-based tile-local node
+        dx1[e] = (T)a1 - tx;
+        dx2[e] = 1 - dx1[e];
+        n1[e] = (a1 - 1) - l0[e];
+      }
+      const T w[8] = {dx1[0] * dx1[1] * dx1[2] * mass, dx2[0] * dx1[1] * dx1[2] * mass,
+                      dx1[0] * dx2[1] * dx1[2] * mass, dx1[0] * dx1[1] * dx2[2] * mass,
+                      dx1[0] * dx2[1] * dx2[2] * mass, dx2[0] * dx1[1] * dx2[2] * mass,
+                      dx2[0] * dx2[1] * dx1[2] * mass, dx2[0] * dx2[1] * dx2[2] * mass};
+      const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        int x = n1[0] + o[q][0], y = n1[1] + o[q][1], z = n1[2] + o[q][2];
+        if ((unsigned)x < (unsigned)BC && (unsigned)y < (unsigned)BC && (unsigned)z < (unsigned)BC)
+          atomicAdd(&s[(z * BC + y) * BC + x], (unsigned long long)llrint((double)w[q] * CIC_FIX));
+      }
     }
   }
   __syncthreads();
