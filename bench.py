@@ -56,7 +56,9 @@ def algorithmic_bytes(slot, cfg, npart):
         "cic_fine": n * bx + 4 * mf + 8 * mc,
         "fft_x_r2c": 8 * mf,
         "fft_y_fwd": 8 * mf,
-        "fft_zconv": 8.5 * mf,
+        # one launch for all three components on FP32 meshes: the spectrum
+        # read once, three convolved spectra written, three kernel octants
+        "fft_zconv": 17.5 * mf,
         "fft_y_inv": 8 * mf,
         "fft_x_c2r": 8 * mf,
         # fine pass: xp, vp read + write, three fine force meshes, cell

@@ -438,22 +438,36 @@ static int pm_launch(cube_handle* h, float a_mid, float dt, double* probe_f, dou
       return r;
   }
   // fine mesh, kx-tiled spectrum: the density in R goes to a tiled
-  // spectrum in W; components 0 and 1 are convolved into X and inverted into
-  // S (F0) and R (F1), component 2 in place in W (the spectrum's last use)
-  // and inverted into X (F2).  All three stay resident for the fine kick.
+  // spectrum in W.  FP32 meshes of 32^3 and up: one z pass convolves all
+  // three components into X, R and W (the spectrum's last use), and each is
+  // inverted into the real mesh the previous inversion left free: X -> S
+  // (F0), R -> X (F1), W -> R (F2).  Otherwise components 0 and 1 are
+  // convolved into X and inverted into S (F0) and R (F1), component 2 in
+  // place in W and inverted into X (F2).  All three stay resident for the
+  // fine kick.
   if ((r = cic_fine(h, nullptr, &dr->mass[2]))) return r;
   prd = nu ? prof_begin(h, PS_FLUID_DENS) : -1;
   if (nu && (r = fl_density_fine(h, a_mid, nullptr, &dr->mass[3]))) return r;
   if (nu) prof_end(h, prd);
   if ((r = mesh_t_fwd(h, h->fine, h->fine.R, h->fine.W))) return r;
   const int64_t Nf = h->fine.N;
-  const void* Ff[3] = {h->fine.S, h->fine.R, h->fine.X};
   const int64_t pf[3] = {Nf, Nf + 2, Nf + 2};
-  for (int d = 0; d < 3; d++) {
-    void* spec = d == 2 ? h->fine.W : h->fine.X;
-    if ((r = mesh_t_zconv(h, h->fine, d, h->fine.W, spec))) return r;
-    if ((r = mesh_t_inv(h, h->fine, spec, (void*)Ff[d], pf[d]))) return r;
+  void* Fw[3];
+  if (mesh_t_zconv3_fits(h->fine)) {
+    void* const spec[3] = {h->fine.X, h->fine.R, h->fine.W};
+    Fw[0] = h->fine.S, Fw[1] = h->fine.X, Fw[2] = h->fine.R;
+    if ((r = mesh_t_zconv3(h, h->fine, h->fine.W, spec))) return r;
+    for (int d = 0; d < 3; d++)
+      if ((r = mesh_t_inv(h, h->fine, spec[d], Fw[d], pf[d]))) return r;
+  } else {
+    Fw[0] = h->fine.S, Fw[1] = h->fine.R, Fw[2] = h->fine.X;
+    for (int d = 0; d < 3; d++) {
+      void* spec = d == 2 ? h->fine.W : h->fine.X;
+      if ((r = mesh_t_zconv(h, h->fine, d, h->fine.W, spec))) return r;
+      if ((r = mesh_t_inv(h, h->fine, spec, Fw[d], pf[d]))) return r;
+    }
   }
+  const void* Ff[3] = {Fw[0], Fw[1], Fw[2]};
   // the kick decodes with sigma_vi and encodes with sigma_vi_new, whose
   // coefficient the drift left in h->dscal; it also forms max |F|^2
   // (dt_fine) from the force slabs it stages
@@ -461,7 +475,6 @@ static int pm_launch(cube_handle* h, float a_mid, float dt, double* probe_f, dou
   const bool nu_kick = nu && !probe_f;  // probes leave every state unchanged
   const int prg = nu_kick ? prof_begin(h, PS_FLUID_GRAV) : -1;
   if (nu_kick && (r = fl_gravity_fine(h, Ff, pf, a_mid, dt))) return r;
-  void* const Fw[3] = {h->fine.S, h->fine.R, h->fine.X};
   if (nu_kick && (r = fl_gravity_finish(h, Fw, pf, a_mid, dt))) return r;
   if (nu_kick) prof_end(h, prg);
   HIP_TRY(h, hipMemcpyAsync(dr->pmflags, h->uflags + 32, sizeof(dr->pmflags), hipMemcpyDeviceToDevice, h->stream));

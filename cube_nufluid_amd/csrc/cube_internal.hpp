@@ -406,6 +406,10 @@ int mesh_fft3_r2c(cube_handle* h, Mesh& m);                  // full forward on 
 // dst (may alias); y inverse in place then C2R into out (pitch opitch)
 int mesh_t_fwd(cube_handle* h, Mesh& m, const void* src, void* dst);
 int mesh_t_zconv(cube_handle* h, Mesh& m, int d, const void* src, void* dst);
+// all three components of the fine convolution from one read of src
+// (FP32 meshes of 32^3 and up: mesh_t_zconv3_fits)
+bool mesh_t_zconv3_fits(const Mesh& m);
+int mesh_t_zconv3(cube_handle* h, Mesh& m, const void* src, void* const dst[3]);
 int mesh_t_inv(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch);
 int fine_kernel_build(cube_handle* h, const float* table);
 int coarse_kernel_build(cube_handle* h, const float* table);

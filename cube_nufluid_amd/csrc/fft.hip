@@ -618,6 +618,69 @@ __global__ __launch_bounds__(pick_ntz<T>(LOGN)) void k_tzconv(const cpx<T>* src,
   }
 }
 
+// all three components from one read of the spectrum: the strip's forward z
+// transform is kept in LDS while each component is multiplied by i*K_d,
+// transformed back and stored to dst[d] (dst[2] may be src).  Strips are
+// half as wide as k_tzconv's, so both copies fit; every column's
+// arithmetic is k_tzconv's, so the outputs are the same bits.
+template <typename T>
+constexpr int pick_wz3(int logn) { return pick_wz<T>(logn) / 2 > 0 ? pick_wz<T>(logn) / 2 : 1; }
+// meshes whose strips give every thread whole elements (all but 16^3)
+template <typename T>
+constexpr bool fits_z3(int logn) {
+  return ((1 << logn) * pick_wz3<T>(logn)) % pick_ntz<T>(logn) == 0 && (1 << logn) * pick_wz3<T>(logn) >= pick_ntz<T>(logn);
+}
+template <typename T, int LOGN>
+__global__ __launch_bounds__(pick_ntz<T>(LOGN)) void k_tzconv3(const cpx<T>* src, cpx<T>* dst0, cpx<T>* dst1, cpx<T>* dst2,
+                                                     const T* __restrict__ K, const cpx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, H = N / 2 + 1, W = pick_wz3<T>(LOGN), NP = TB / W, NTH = pick_ntz<T>(LOGN);
+  constexpr int WK = pick_wz<T>(LOGN);  // the kernel table's strip width
+  constexpr int PER = N * W / NTH;
+  static_assert(N * W % NTH == 0, "whole loads per thread");
+  static_assert(WK % W == 0, "strips inside the kernel table's strips");
+  __shared__ cpx<T> sa[N * W];
+  __shared__ cpx<T> sb[N * W];
+  int bx, by;
+  grid_block_xy(bx, by);
+  const int kh = bx / NP, part = bx % NP, ky = by, kx0 = kh * TB + part * W;
+  const int64_t stride = (int64_t)N * TB;
+  const int64_t base = ((int64_t)kh * N * N) * TB + (int64_t)ky * TB + part * W;
+  const int kyp = ky <= N / 2 ? ky : N - ky;
+  constexpr int G = (int)(t_tiles(N) * TB / WK);
+  col_load<T, N, W, NTH>(sa, src + base, stride, W);
+  lds_fft<T, N, W, false, W, NTH>(sa, tw, 1);
+  cpx<T>* const dsts[3] = {dst0, dst1, dst2};
+#pragma unroll 1
+  for (int d = 0; d < 3; d++) {
+    const T sy = (d == 1 && ky > N / 2) ? (T)-1 : (T)1;
+    // K[d][kyp][kx / WK][kzp][kx % WK]
+    const T* Ks = K + (((int64_t)d * H + kyp) * G + kx0 / WK) * H * WK + kx0 % WK;
+    T kv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = threadIdx.x + q * NTH, w = idx % W, n = idx / W;
+      const int kzp = n <= N / 2 ? n : N - n;
+      kv[q] = Ks[kzp * WK + w];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int idx = threadIdx.x + q * NTH, n = idx / W;
+      const T sz = (d == 2 && n > N / 2) ? (T)-1 : (T)1;
+      const T k = kv[q] * sy * sz;
+      const cpx<T> a = sa[idx];
+      sb[idx] = cpx<T>{-a.y * k, a.x * k};
+    }
+    __syncthreads();
+    lds_fft<T, N, W, true, W, NTH>(sb, tw, 1);
+    cpx<T>* const dst = dsts[d];
+    for (int idx = threadIdx.x; idx < N * W; idx += NTH) {
+      const int n = idx / W, w = idx % W;
+      dst[base + (int64_t)n * stride + w] = sb[idx];
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------ dispatchers
 #define LOGN_SWITCH(logn, ...)             \
   switch (logn) {                           \
@@ -766,6 +829,42 @@ static int t_zconv_t(cube_handle* h, Mesh& m, int d, const void* src, void* dst)
 }
 int mesh_t_zconv(cube_handle* h, Mesh& m, int d, const void* src, void* dst) {
   return m.prec == 4 ? t_zconv_t<float>(h, m, d, src, dst) : t_zconv_t<double>(h, m, d, src, dst);
+}
+bool mesh_t_zconv3_fits(const Mesh& m) {
+  if (m.prec != 4) return false;
+  const int logn = ilog2(m.N);
+  bool ok = false;
+  switch (logn) {
+    case 4: ok = fits_z3<float>(4); break;
+    case 5: ok = fits_z3<float>(5); break;
+    case 6: ok = fits_z3<float>(6); break;
+    case 7: ok = fits_z3<float>(7); break;
+    case 8: ok = fits_z3<float>(8); break;
+    case 9: ok = fits_z3<float>(9); break;
+    case 10: ok = fits_z3<float>(10); break;
+    case 11: ok = fits_z3<float>(11); break;
+    default: ok = false;
+  }
+  return ok;
+}
+int mesh_t_zconv3(cube_handle* h, Mesh& m, const void* src, void* const dst[3]) {
+  if (m.prec != 4) return cube_fail(h, CUBE_ERR_ARG, "three-component z convolution is built for FP32 meshes");
+  int logn = ilog2(m.N);
+  int64_t N = m.N;
+  int p1 = prof_begin(h, PS_FFT_ZCONV);
+  LOGN_SWITCH(logn, {
+    if constexpr (fits_z3<float>(L)) {
+      constexpr int WZ = pick_wz3<float>(L);
+      k_tzconv3<float, L><<<dim3((unsigned)(t_tiles(N) * (TB / WZ)), (unsigned)N), pick_ntz<float>(L), 0, h->stream>>>(
+          (const cpx<float>*)src, (cpx<float>*)dst[0], (cpx<float>*)dst[1], (cpx<float>*)dst[2], (const float*)m.K,
+          (const cpx<float>*)m.tw);
+    } else {
+      return cube_fail(h, CUBE_ERR_ARG, "mesh 2^%d too small for the three-component z convolution", L);
+    }
+  });
+  prof_end(h, p1);
+  HIP_TRY(h, hipGetLastError());
+  return 0;
 }
 
 template <typename T>
