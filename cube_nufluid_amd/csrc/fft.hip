@@ -433,8 +433,12 @@ __global__ __launch_bounds__(NT_FFT) void k_fft_zconv(const cpx<T>* src, cpx<T>*
 // is written and read as 16*W contiguous elements per kx tile.  The R2C and
 // C2R passes run out of place (real rows <-> tiled spectrum).
 #define TB 16
+// tiled y passes: twice the plain strip width (whole 64-byte halves of each
+// group at 2048^3), with twice the threads when that doubles the LDS tile
 template <typename T>
-constexpr int pick_wy(int logn) { return pick_w<T>(logn) < TB ? pick_w<T>(logn) : TB; }
+constexpr int pick_wy(int logn) { return 2 * pick_w<T>(logn) < TB ? 2 * pick_w<T>(logn) : TB; }
+template <typename T>
+constexpr int pick_nty(int logn) { return pick_wy<T>(logn) > pick_w<T>(logn) ? 2 * NT_FFT : NT_FFT; }
 // tiled x passes: rows per workgroup, half the LDS tile of the plain x pass
 template <typename T>
 constexpr int pick_wtx(int logn) { return pick_w<T>(logn - 1) > 1 ? pick_w<T>(logn - 1) / 2 : 1; }
@@ -555,16 +559,16 @@ __global__ __launch_bounds__(NT_FFT) void k_tx_c2r(const cpx<T>* __restrict__ sr
 // tiled y pass, in place: tile (kh, part, kz) = W columns kx = kh*16 +
 // part*W + w along ky (stride 16)
 template <typename T, int LOGN, bool INV>
-__global__ __launch_bounds__(NT_FFT) void k_ty(cpx<T>* data, const cpx<T>* __restrict__ tw) {
-  constexpr int N = 1 << LOGN, W = pick_wy<T>(LOGN), NP = TB / W;
+__global__ __launch_bounds__(pick_nty<T>(LOGN)) void k_ty(cpx<T>* data, const cpx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, W = pick_wy<T>(LOGN), NP = TB / W, NTH = pick_nty<T>(LOGN);
   __shared__ cpx<T> s[N * W];
   int bx, by;
   grid_block_xy(bx, by);
   const int kh = bx / NP, part = bx % NP, kz = by;
   const int64_t base = (((int64_t)kh * N + kz) * N) * TB + part * W;
-  col_load<T, N, W>(s, data + base, TB, W);
-  lds_fft<T, N, W, INV>(s, tw, 1);
-  for (int idx = threadIdx.x; idx < N * W; idx += NT_FFT) {
+  col_load<T, N, W, NTH>(s, data + base, TB, W);
+  lds_fft<T, N, W, INV, W, NTH>(s, tw, 1);
+  for (int idx = threadIdx.x; idx < N * W; idx += NTH) {
     const int n = idx / W, w = idx % W;
     data[base + (int64_t)n * TB + w] = s[idx];
   }
@@ -803,7 +807,7 @@ static int t_fwd_t(cube_handle* h, Mesh& m, const void* src, void* dst) {
     prof_end(h, p1);
     constexpr int WY = pick_wy<T>(L);
     int p2 = prof_begin(h, PS_FFT_YF);
-    k_ty<T, L, false><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), NT_FFT, 0, h->stream>>>((cpx<T>*)dst, tw);
+    k_ty<T, L, false><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), pick_nty<T>(L), 0, h->stream>>>((cpx<T>*)dst, tw);
     prof_end(h, p2);
   });
   HIP_TRY(h, hipGetLastError());
@@ -875,7 +879,7 @@ static int t_inv_t(cube_handle* h, Mesh& m, void* buf, void* out, int64_t opitch
   LOGN_SWITCH(logn, {
     constexpr int WY = pick_wy<T>(L);
     int pa = prof_begin(h, PS_FFT_YI);
-    k_ty<T, L, true><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), NT_FFT, 0, h->stream>>>((cpx<T>*)buf, tw);
+    k_ty<T, L, true><<<dim3((unsigned)(t_tiles(N) * (TB / WY)), (unsigned)N), pick_nty<T>(L), 0, h->stream>>>((cpx<T>*)buf, tw);
     prof_end(h, pa);
     constexpr int WX = pick_wtx<T>(L);
     int pb = prof_begin(h, PS_FFT_XI);
