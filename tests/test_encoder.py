@@ -24,7 +24,8 @@ PI_F = float(np.float32(np.pi))
 
 
 def atan_table():
-    src = open(HDR).read()
+    with open(HDR) as f:
+        src = f.read()
     m = re.search(r"kAtanTab\[33\]\s*=\s*\{([^}]*)\}", src)
     assert m, "kAtanTab not found"
     return np.array([float(v) for v in m.group(1).replace("\n", " ").split(",") if v.strip()])
@@ -33,7 +34,7 @@ def atan_table():
 def atan_enc(x, tab):
     a = np.abs(x)
     big = a > 1.0
-    with np.errstate(divide="ignore"):
+    with np.errstate(divide="ignore", over="ignore"):
         u = np.where(big, 1.0 / np.minimum(a, 1e300), a)
     j = np.clip(np.floor(u * 32.0 + 0.5), 0, 32).astype(np.int64)
     c = j * 0.03125
