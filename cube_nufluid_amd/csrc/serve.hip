@@ -33,7 +33,7 @@ extern "C" int cube_clock_init(cube_clock* c, const float* z_checkpoint, int n) 
   memset(c, 0, sizeof *c);
   for (int i = 0; i < n; i++) c->z_checkpoint[i] = z_checkpoint[i];
   c->n_checkpoint = n;
-  // initialize.f90:40-53, particle_initialization.f90:37-38, main.f90:42
+  // initialize.f90:40-53,73, particle_initialization.f90:37-38, main.f90:42
   c->a = 1.0f / (1.0f + z_checkpoint[0]);
   c->tau = -3.0f / sqrtf(1.0f / (1.0f + z_checkpoint[0]));
   c->dt_neu = 1.0f;

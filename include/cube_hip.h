@@ -290,7 +290,8 @@ typedef struct cube_clock {
   int64_t istep;
   float z_checkpoint[CUBE_MAX_CHECKPOINTS];
 } cube_clock;
-/* initialize.f90:40-53: a from z_checkpoint[0], cur_checkpoint = 2. */
+/* initialize.f90:40-53,73: a and tau from z_checkpoint[0], cur_checkpoint = 2;
+   1 <= n <= CUBE_MAX_CHECKPOINTS, else CUBE_ERR_ARG. */
 int cube_clock_init(cube_clock* c, const float* z_checkpoint, int n);
 /* One pass of the main-loop body: timestep, drift (update_x), PP if
    enabled, PM, and at a checkpoint step the closing half drift
