@@ -14,6 +14,7 @@
 // inputs (CIC and PP sums are order-independent), so a box gives the same
 // bits alone and in the mix.
 #include "cube_internal.hpp"
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -257,6 +258,7 @@ struct cube_ensemble {
   std::deque<Request> queue;
   std::vector<cube_ens_result> done;
   std::vector<std::thread> workers;
+  std::vector<std::thread::id> ended;  // workers whose box is done, not yet joined
   std::thread scheduler;
   int64_t used = 0, next_id = 1;
   int live = 0;
@@ -301,6 +303,7 @@ static void run_box(cube_ensemble* e, Request rq, double t_admit) {
     res.a_final = c.a;
   }
   if (rc && h) snprintf(res.message, sizeof res.message, "%s", h->err.c_str());
+  else if (rc) snprintf(res.message, sizeof res.message, "box set-up failed (code %d)", rc);
   res.status = rc;
   if (h) cube_finalize(h);
   res.t_done = e->now();
@@ -308,12 +311,32 @@ static void run_box(cube_ensemble* e, Request rq, double t_admit) {
   e->used -= rq.bytes;
   e->live -= 1;
   e->done.push_back(res);
+  e->ended.push_back(std::this_thread::get_id());
   e->cv.notify_all();
 }
 
 static void schedule(cube_ensemble* e) {
   std::unique_lock<std::mutex> lk(e->mu);
   for (;;) {
+    // join the workers whose boxes are done, with the lock released, so a
+    // long-lived engine holds threads only for its live boxes; a worker
+    // records itself last thing under the lock, so the join waits only for
+    // its return
+    if (!e->ended.empty()) {
+      std::vector<std::thread> finished;
+      for (std::thread::id id : e->ended) {
+        auto it = std::find_if(e->workers.begin(), e->workers.end(), [id](const std::thread& t) { return t.get_id() == id; });
+        if (it != e->workers.end()) {
+          finished.push_back(std::move(*it));
+          e->workers.erase(it);
+        }
+      }
+      e->ended.clear();
+      lk.unlock();
+      for (std::thread& t : finished) t.join();
+      lk.lock();
+      continue;  // the queue may have changed meanwhile
+    }
     // admit, in arrival order, every queued request that fits now
     bool admitted = false;
     for (auto it = e->queue.begin(); it != e->queue.end();) {

@@ -49,7 +49,6 @@ class Ensemble:
             raise RuntimeError("cube_ens_create failed (%d)" % rc)
         self.e = e
         self.kf, self.kc = read_kernel_tables()
-        self._keep = {}
 
     def submit(self, cfg, seed, z_i, z_f):
         p = _lib.CubeParams()
@@ -100,28 +99,30 @@ def run_ensemble(requests, max_live=64, mem_budget=0, log=None, step_mode=0):
     phase synchronises, 1 one synchronisation per step, 2 steps replayed as
     hipGraphs)."""
     ens = Ensemble(max_live=max_live, mem_budget=mem_budget, step_mode=step_mode)
-    t0 = time.perf_counter()
-    ids = {}
-    results = {}
-    pending = list(requests)
-    while pending or len(results) < len(requests):
-        now = time.perf_counter() - t0
-        while pending and pending[0]["t"] <= now:
-            rq = pending.pop(0)
-            ids[ens.submit(ensemble_config(rq["npd"]), rq["seed"], rq["z_i"], rq["z_f"])] = rq
-        for r in ens.poll():
-            rq = ids[r.id]
-            if r.status:
-                ens.close()
-                raise RuntimeError("request %d (%d^3) failed: %s" % (r.id, rq["npd"], r.message.decode()))
-            results[r.id] = dict(npd=rq["npd"], seed=rq["seed"], steps=r.steps, n=r.nparticles,
-                                 latency=r.t_done - r.t_submit, wait=r.t_admit - r.t_submit, t_submit=r.t_submit,
-                                 t_done=r.t_done, a_final=r.a_final, hash=int(r.state_hash))
-            if log:
-                log("  request %d: %d^3 particles, %d steps, latency %.3f s" % (r.id, rq["npd"], r.steps,
-                                                                              results[r.id]["latency"]))
-        time.sleep(0.002)
-    ens.close()
+    try:
+        t0 = time.perf_counter()
+        ids = {}
+        results = {}
+        pending = list(requests)
+        while pending or len(results) < len(requests):
+            now = time.perf_counter() - t0
+            while pending and pending[0]["t"] <= now:
+                rq = pending.pop(0)
+                ids[ens.submit(ensemble_config(rq["npd"]), rq["seed"], rq["z_i"], rq["z_f"])] = rq
+            for r in ens.poll():
+                rq = ids[r.id]
+                if r.status:
+                    raise RuntimeError("request %d (%d^3) failed: %s" % (r.id, rq["npd"], r.message.decode()))
+                results[r.id] = dict(npd=rq["npd"], seed=rq["seed"], steps=r.steps, n=r.nparticles,
+                                     latency=r.t_done - r.t_submit, wait=r.t_admit - r.t_submit, t_submit=r.t_submit,
+                                     t_done=r.t_done, a_final=r.a_final, hash=int(r.state_hash))
+                if log:
+                    log("  request %d: %d^3 particles, %d steps, latency %.3f s" % (r.id, rq["npd"], r.steps,
+                                                                                  results[r.id]["latency"]))
+            time.sleep(0.002)
+    finally:
+        # runs whatever was submitted to completion, then frees the engine
+        ens.close()
     res = list(results.values())
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
     lat = np.array([r["latency"] for r in res])
