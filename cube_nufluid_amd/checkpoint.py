@@ -124,15 +124,22 @@ def write_neu_checkpoint(fn, m_ev, gamma, isothermal, weights, fluids, dxs, ncb=
 
 def read_neu_checkpoint(fn, ncs, dxs, ncb=6):
     """Inverse of write_neu_checkpoint: (m_ev, gamma, isothermal, weights,
-    fluids as [5][nc][nc][nc] physical cells)."""
+    fluids as [5][nc][nc][nc] physical cells).  A file shorter than the
+    fluids it should hold is rejected."""
     with open(fn, "rb") as f:
-        m_ev, gamma = np.fromfile(f, dtype=np.float32, count=2)
-        iso = bool(np.fromfile(f, dtype=np.int32, count=1)[0])
-        w = np.fromfile(f, dtype=np.float32, count=3)
+        def load(dtype, count, what):
+            a = np.fromfile(f, dtype=dtype, count=count)
+            if a.size != count:
+                raise ValueError("truncated neutrino checkpoint: %s holds %d of %d values of %s" % (fn, a.size, count, what))
+            return a
+
+        m_ev, gamma = load(np.float32, 2, "the header")
+        iso = bool(load(np.int32, 1, "the header")[0])
+        w = load(np.float32, 3, "the header")
         fluids = []
-        for nc, dx in zip(ncs, dxs):
+        for i, (nc, dx) in enumerate(zip(ncs, dxs)):
             b = neu_ncb(dx, ncb)
             ne = nc + 2 * b
-            ext = np.fromfile(f, dtype=np.float32, count=5 * ne ** 3).reshape(ne, ne, ne, 5)
+            ext = load(np.float32, 5 * ne ** 3, "fluid %d" % (i + 1)).reshape(ne, ne, ne, 5)
             fluids.append(np.ascontiguousarray(np.moveaxis(ext, -1, 0)[:, b:b + nc, b:b + nc, b:b + nc]))
     return float(m_ev), float(gamma), iso, w, fluids

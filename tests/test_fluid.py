@@ -100,6 +100,12 @@ def test_neu_checkpoint_round_trip(tmp_path):
     assert (m, iso) == (np.float32(0.1), True) and abs(g - 5 / 3) < 1e-6
     for a, b2 in zip(fl, back):
         np.testing.assert_array_equal(a, b2)
+    # a short file stops with a message, wherever it ends
+    raw = (tmp_path / "0.000_neu1_1.bin").read_bytes()
+    for cut in (5, 4 * 6 - 1, 4 * 6 + 100, size - 4):
+        (tmp_path / "short.bin").write_bytes(raw[:cut])
+        with pytest.raises(ValueError, match="truncated neutrino checkpoint"):
+            read_neu_checkpoint(str(tmp_path / "short.bin"), (16, 8, 4), dxs)
 
 
 def test_oracle_pm_gravity_moves_momentum_not_density():
