@@ -47,7 +47,7 @@ class Cube:
     def from_ic(cls, cfg, seed, z, device=0, z_checkpoints=None, np_max=None, z_halofind=None):
         """Generate seeded Zel'dovich initial conditions directly in HBM
         (cube_ic_zeldovich) and return the resident box."""
-        from .ic import power_table, velocity_factor
+        from .ic import ic_vsim2phys, power_table, velocity_factor
         self = cls.__new__(cls)
         self.lib = _lib.load()
         self.cfg = cfg
@@ -74,7 +74,7 @@ class Cube:
         sim = _lib.SimHeader()
         _lib.check(self.lib, h, self.lib.cube_ic_zeldovich(h, ctypes.byref(ip), ctypes.byref(sim)))
         sim.h0, sim.omega_m, sim.omega_l, sim.s8 = cfg.h0, cfg.omega_m, cfg.omega_l, cfg.sigma8
-        sim.vsim2phys = (150.0 / a) * cfg.box * np.sqrt(cfg.omega_m) / cfg.nf
+        sim.vsim2phys = ic_vsim2phys(cfg, sim.a)
         sim.z_i_nu = cfg.z_i_nu
         self.sim = sim
         self._init_time(np.float32(a), z_checkpoints or [z, 0.0], z_halofind)
@@ -304,6 +304,17 @@ class Cube:
         self.dt = dt
         self.dt_neu = f32(1000.0)
 
+    def vsim2phys(self, with_h0):
+        """sim%vsim2phys at the current a, in real(4) left to right: the
+        PM step's (pm.f90:427, with h0) or the checkpoint's
+        (checkpoint.f90:30, without)."""
+        c = self.cfg
+        v = f32(f32(f32(1.5) / f32(self.a)) * f32(c.box))
+        if with_h0:
+            v = f32(v * f32(c.h0))
+        v = f32(f32(v * f32(100.0)) * np.sqrt(f32(c.omega_m)))
+        return f32(v / f32(c.nf))
+
     def update_x(self):
         out = _lib.DriftOut()
         _lib.check(self.lib, self.h, self.lib.cube_drift(self.h, float(self.dt_old), float(self.dt), ctypes.byref(out)))
@@ -322,7 +333,7 @@ class Cube:
         _lib.check(self.lib, self.h, self.lib.cube_pm(self.h, float(self.a_mid), float(self.dt), ctypes.byref(out)))
         s = self.sim
         s.dt_fine, s.dt_coarse, s.dt_vmax, s.vz_max = out.dt_fine, out.dt_coarse, out.dt_vmax, out.vz_max
-        s.vsim2phys = (1.5 / self.a) * self.cfg.box * self.cfg.h0 * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+        s.vsim2phys = self.vsim2phys(with_h0=True)
         self.last_pm = out
         return out
 
@@ -347,7 +358,7 @@ class Cube:
                 if opath is not None:
                     st = self.download()
                     st.sim.cur_checkpoint = self.cur_checkpoint
-                    st.sim.vsim2phys = (1.5 / self.a) * self.cfg.box * 100.0 * np.sqrt(self.cfg.omega_m) / self.cfg.nf
+                    st.sim.vsim2phys = self.vsim2phys(with_h0=False)
                     zc = self.z_checkpoints[self.cur_checkpoint - 1]
                     if nu:  # main.f90:75-81, before the particle files
                         self.nu_checkpoint(output_name(opath, zc, "neu1"))
@@ -370,9 +381,7 @@ class Cube:
         zh = self.z_halofind
         z = float(zh[self.cur_halofind - 1]) if self.cur_halofind <= len(zh) else 0.0
         self.sim.a = self.a
-        # checkpoint.f90:30 in real(4)
-        self.sim.vsim2phys = f32(f32(f32(f32(f32(1.5) / self.a) * f32(self.cfg.box)) * f32(100.0))
-                                 * np.sqrt(f32(self.cfg.omega_m))) / f32(self.cfg.nf)
+        self.sim.vsim2phys = self.vsim2phys(with_h0=False)
         t0 = time.perf_counter()
         cat = self.halofind(self.halo_params, nhid=self.cfg.nf ** 3 if opath is not None else 0)
         self.timings.setdefault("halofind", []).append(time.perf_counter() - t0)

@@ -118,6 +118,14 @@ def rank_to_cell(r, nt, nnt):
     return (t % nnt * nt + l % nt, (t // nnt) % nnt * nt + (l // nt) % nt, t // (nnt * nnt) * nt + l // (nt * nt))
 
 
+def ic_vsim2phys(cfg, a):
+    """sim%vsim2phys of the initial conditions (initial_conditions.f90:198)
+    in real(4), left to right."""
+    f32 = np.float32
+    v = f32(f32(f32(150.0) / f32(a)) * f32(cfg.box))
+    return f32(f32(v * np.sqrt(f32(cfg.omega_m))) / f32(cfg.nf))
+
+
 def encode_particles(cfg, pos, vel, pid, a, sigma_vi=None, z_i=None):
     """Pack continuous positions (coarse-cell units, any real) and velocities
     (simulation units) into a BoxState in checkpoint order."""
@@ -157,7 +165,7 @@ def encode_particles(cfg, pos, vel, pid, a, sigma_vi=None, z_i=None):
     sim.box = cfg.box
     sim.h0 = cfg.h0
     sim.omega_m, sim.omega_l, sim.s8 = cfg.omega_m, cfg.omega_l, cfg.sigma8
-    sim.vsim2phys = (150.0 / a) * cfg.box * np.sqrt(cfg.omega_m) / cfg.nf
+    sim.vsim2phys = ic_vsim2phys(cfg, a)
     sim.sigma_vi = sigma_vi
     sim.z_i = (1.0 / a - 1.0) if z_i is None else z_i
     sim.z_i_nu = 10.0

@@ -1,0 +1,71 @@
+"""vsim2phys, the velocity unit the hosts write into sim headers, has the
+reference's real(4) bits: the initial conditions' (initial_conditions.f90:
+198), the PM step's (pm.f90:427, with h0) and the checkpoint's
+(checkpoint.f90:30, without h0).  Each expression is compiled here as a
+Fortran program (amdflang at -O0, default real = real(4)) and evaluated
+for a range of a and box parameters; the Python hosts must give the same
+bits.  Evaluating in double and rounding once, or dropping the h0 factor,
+fails."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from cube_nufluid_amd.driver import Cube
+from cube_nufluid_amd.ic import ic_vsim2phys
+from cube_nufluid_amd.params import BoxConfig
+
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+
+PROGRAM = """program v
+  implicit none
+  real :: a, box, h0, omega_m
+  integer :: nf, i, n
+  read(*,*) box, h0, omega_m, nf, n
+  do i = 1, n
+    read(*,*) a
+    print '(3I12)', transfer((150./a)*box*sqrt(omega_m)/nf, 0), &
+      transfer((1.5/a)*box*h0*100.*sqrt(omega_m)/nf, 0), &
+      transfer((1.5/a)*box*100.*sqrt(omega_m)/nf, 0)
+  end do
+end program
+"""
+
+
+@pytest.fixture(scope="module")
+def fortran(tmp_path_factory):
+    if not os.path.exists(FC):
+        pytest.skip("no Fortran compiler")
+    d = tmp_path_factory.mktemp("vsim")
+    (d / "v.f90").write_text(PROGRAM)
+    subprocess.run([FC, "-O0", str(d / "v.f90"), "-o", str(d / "v")], check=True, capture_output=True)
+    return str(d / "v")
+
+
+def fortran_bits(exe, box, h0, om, nf, avals):
+    inp = "%.9g %.9g %.9g %d %d\n" % (box, h0, om, nf, len(avals)) + "".join("%.9g\n" % a for a in avals)
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout.split()
+    return np.array(out, dtype=np.int64).reshape(-1, 3).astype(np.uint32)
+
+
+@pytest.mark.parametrize("nc,box,h0,om", [(16, 100.0, 0.68, 0.31), (512, 1000.0, 0.6774, 0.3089), (128, 400.0, 0.7, 0.27)])
+def test_vsim2phys_matches_the_reference_expressions(fortran, nc, box, h0, om):
+    cfg = BoxConfig(nc=nc, nnt=1, ncb=6, box=box)
+    cfg.h0, cfg.omega_m = h0, om
+    rng = np.random.default_rng(nc)
+    avals = np.concatenate([np.float32(1) / (np.float32(1) + np.array([49, 10, 3, 1, 0.5, 0], np.float32)),
+                            rng.uniform(0.015, 1.0, 200).astype(np.float32)])
+    # the inputs as real(4) values, written with enough digits to read back exactly
+    want = fortran_bits(fortran, np.float32(box), np.float32(h0), np.float32(om), cfg.nf, avals)
+    host = Cube.__new__(Cube)
+    host.cfg = cfg
+    got = np.empty_like(want)
+    for i, a in enumerate(avals):
+        host.a = np.float32(a)
+        got[i] = [np.float32(ic_vsim2phys(cfg, a)).view(np.uint32), host.vsim2phys(True).view(np.uint32),
+                  host.vsim2phys(False).view(np.uint32)]
+    for col, name in enumerate(("initial conditions", "PM step", "checkpoint")):
+        bad = np.flatnonzero(got[:, col] != want[:, col])
+        assert bad.size == 0, (name, avals[bad[:3]], got[bad[:3], col], want[bad[:3], col])
