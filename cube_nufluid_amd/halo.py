@@ -138,8 +138,10 @@ def find_halos(lib, handle, hp, max_halos=1 << 20, max_pid_words=None, nhid=0):
 
 
 def halo_prefix(opath, z, image=1):
-    """opath//'image1/'//z_str//'_' with z written as f7.3 (basic_functions.fh:68-74)."""
-    return os.path.join(opath, "image%d" % image, "%.3f_" % z)
+    """opath//'image1/'//z_str//'_' with z written as f7.3 (basic_functions.fh:68-74),
+    the same z string as the checkpoint files."""
+    from .checkpoint import z2str
+    return os.path.join(opath, "image%d" % image, z2str(z) + "_")
 
 
 def write_halo_files(prefix, cat, image=1):

@@ -1,4 +1,8 @@
-"""vsim2phys, the velocity unit the hosts write into sim headers, has the
+"""Scalars and strings the hosts write into CUBE's files, against the same
+expressions compiled by a Fortran compiler (amdflang at -O0, default real =
+real(4)).
+
+vsim2phys, the velocity unit the hosts write into sim headers, has the
 reference's real(4) bits: the initial conditions' (initial_conditions.f90:
 198), the PM step's (pm.f90:427, with h0) and the checkpoint's
 (checkpoint.f90:30, without h0).  Each expression is compiled here as a
@@ -69,3 +73,41 @@ def test_vsim2phys_matches_the_reference_expressions(fortran, nc, box, h0, om):
     for col, name in enumerate(("initial conditions", "PM step", "checkpoint")):
         bad = np.flatnonzero(got[:, col] != want[:, col])
         assert bad.size == 0, (name, avals[bad[:3]], got[bad[:3], col], want[bad[:3], col])
+
+
+Z_PROGRAM = """program f
+  implicit none
+  real :: z
+  integer :: i, n
+  character(len=7) :: s
+  read(*,*) n
+  do i = 1, n
+    read(*,*) z
+    write(s, '(f7.3)') z
+    print '(A)', trim(adjustl(s))
+  end do
+end program
+"""
+
+
+def test_file_name_redshift_matches_fortran_f73(tmp_path):
+    """Checkpoint and halo files carry z as write(z_str,'(f7.3)') of the
+    real(4) redshift, trimmed (basic_functions.fh:36-42, 52-87).  The
+    redshift lists are read as (f8.4), so four-decimal values whose last
+    digit is 5, and binary ties such as 0.0625, are checked too."""
+    from cube_nufluid_amd.checkpoint import output_name, z2str
+    from cube_nufluid_amd.halo import halo_prefix
+    if not os.path.exists(FC):
+        pytest.skip("no Fortran compiler")
+    (tmp_path / "f.f90").write_text(Z_PROGRAM)
+    subprocess.run([FC, "-O0", str(tmp_path / "f.f90"), "-o", str(tmp_path / "f")], check=True, capture_output=True)
+    zs = np.concatenate([np.arange(0, 64, 1 / 16), np.round(np.arange(0, 50, 0.0005), 4),
+                         [99.9995, 127.5, 999.0625]]).astype(np.float32)
+    inp = "%d\n" % len(zs) + "".join("%.9g\n" % z for z in zs)
+    want = subprocess.run([str(tmp_path / "f")], input=inp, capture_output=True, text=True, check=True).stdout.split()
+    got = [z2str(z) for z in zs]
+    bad = [(float(z), w, g) for z, w, g in zip(zs, want, got) if w != g]
+    assert not bad, bad[:5]
+    for z in (0.0025, 1.0005, 49.0):
+        assert os.path.basename(halo_prefix("o", z)) + "halo_1.bin" == \
+            os.path.basename(output_name("o", z, "halo"))
