@@ -169,6 +169,7 @@ class Cube:
             self.h, ctypes.byref(s), _lib.ptr(xp), _lib.ptr(vp), _lib.ptr(pid), _lib.ptr(rhoc), _lib.ptr(vf)))
         for f in ("a", "t", "tau"):
             setattr(s, f, float(getattr(self, f)))
+        s.timestep = self.sim.timestep  # counted by this host's timestep (timestep.f90:10)
         return BoxState(s, xp, vp, pid, rhoc, vf)
 
     def close(self):

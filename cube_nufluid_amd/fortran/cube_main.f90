@@ -205,8 +205,11 @@ contains
   subroutine checkpoint(z)
     real, intent(in) :: z
     integer :: v, irc
+    integer(c_int64_t) :: nstep
+    nstep = sim%timestep   ! counted here, in timestep (timestep.f90:10)
     irc = cube_download_state(h, sim, c_loc(xpb), c_loc(vpb), pid, rhoc, vfield)
     call cube_ok(h, irc, 'cube_download_state')
+    sim%timestep = nstep
     sim%a = a; sim%t = t; sim%tau = tau
     sim%cur_checkpoint = cur_checkpoint
     sim%vsim2phys = (1.5 / a) * box * 100. * sqrt(omega_m) / (nc * 4)

@@ -45,6 +45,11 @@ def test_fortran_host_matches_python_host(tmp_path):
         for f in ("xp", "vp", "pid", "rhoc", "vfield"):
             assert np.array_equal(getattr(a, f), getattr(b, f)), (z, f)
         assert a.sim.nplocal == b.sim.nplocal and a.sim.a == pytest.approx(b.sim.a, rel=1e-6)
+        # sim%timestep counts the steps taken so far (timestep.f90:10)
+        assert a.sim.timestep == b.sim.timestep > 0, (z, a.sim.timestep, b.sim.timestep)
+    h8 = read_checkpoint(str(pdir) + "/", zs[1], cfg.nc).sim.timestep
+    h6 = read_checkpoint(str(pdir) + "/", zs[2], cfg.nc).sim.timestep
+    assert 0 < h8 < h6
 
 
 def test_fortran_host_halofind_matches_python_host(tmp_path):
