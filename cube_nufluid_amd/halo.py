@@ -162,8 +162,13 @@ def write_halo_files(prefix, cat, image=1):
 
 
 def read_halo_file(path):
+    """(nhalo_tot, nhalo, den_odc, records) of a halo file; one whose
+    records do not match its count is rejected."""
     with open(path, "rb") as f:
         hdr = np.fromfile(f, dtype=np.int32, count=2)
-        den = np.fromfile(f, dtype=np.float32, count=1)[0]
-        recs = np.fromfile(f, dtype=np.float32).reshape(-1, HALO_RECORD_FLOATS)
-    return int(hdr[0]), int(hdr[1]), float(den), recs
+        den = np.fromfile(f, dtype=np.float32, count=1)
+        recs = np.fromfile(f, dtype=np.float32)
+    if hdr.size != 2 or den.size != 1 or recs.size != int(hdr[1]) * HALO_RECORD_FLOATS:
+        raise ValueError("halo file %s: %d values after the header, %s halos of %d declared"
+                         % (path, recs.size, hdr[1] if hdr.size == 2 else "?", HALO_RECORD_FLOATS))
+    return int(hdr[0]), int(hdr[1]), float(den[0]), recs.reshape(-1, HALO_RECORD_FLOATS)

@@ -207,6 +207,12 @@ def test_halo_files_round_trip(tmp_path):
     w = np.fromfile(prefix + "halo_pid_1.bin", dtype=np.int32)
     np.testing.assert_array_equal(w, words)
     assert (tmp_path / "image1" / "0.000_hid_1.bin").stat().st_size == 2 * cfg.nf ** 3
+    # a file cut short, or holding more records than it declares, is rejected
+    raw = open(prefix + "halo_1.bin", "rb").read()
+    for bad in (raw[:-4], raw[:-4 * HALO_FIELDS["s_mean"].stop], raw + raw[12:12 + 168], raw[:6]):
+        open(prefix + "bad.bin", "wb").write(bad)
+        with pytest.raises(ValueError, match="halo file"):
+            read_halo_file(prefix + "bad.bin")
 
 
 # ------------------------------------------------------------------ GPU
