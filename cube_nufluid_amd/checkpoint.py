@@ -12,6 +12,8 @@ Files are stream binaries named '{z:f7.3}_{content}_{image}.bin'
 """
 import dataclasses
 import os
+import re
+from decimal import Decimal
 
 import numpy as np
 
@@ -87,15 +89,45 @@ def read_checkpoint(opath, z, nc, image=1, izipx=None, izipv=None):
     return BoxState(sim, xp, vp, pid, rhoc, vfield)
 
 
-def read_z_list(path):
-    """initialize.f90:55-61: one redshift per line, format (f8.4); an empty
-    list is an error ('z_checkpoint.txt empty')."""
+_F84 = re.compile(r"([+-]?)(\d*)(?:\.(\d*))?(?:[eEdDqQ]([+-]?\d+)|([+-]\d+))?")
+
+
+def read_f84(record):
+    """The value a Fortran read with edit descriptor (f8.4) takes from one
+    record, as a float32 value: the first 8 columns (a shorter record is
+    padded with blanks), blanks ignored, an all-blank field reads 0, and a
+    mantissa written without a decimal point has 4 implied decimal places
+    ("10" reads 0.001).  Anything else in the field is an error."""
+    s = record[:8].replace(" ", "")
+    if not s:
+        return 0.0
+    m = _F84.fullmatch(s)
+    if m is None or not (m.group(2) or m.group(3)):
+        raise ValueError("bad real input for (f8.4): %r" % record[:8])
+    sign, ip, fp, e1, e2 = m.groups()
+    digits, scale = (ip, -4) if fp is None else (ip + fp, -len(fp))
+    v = Decimal(sign + (digits or "0")).scaleb(scale + int(e1 or e2 or 0))
+    f = float(v)
+    c = np.float32(f)
+    # decimal -> double -> float32 rounds twice, which can only go wrong
+    # when the double lands exactly halfway between two float32 values
+    lo, hi = (np.nextafter(c, np.float32(-np.inf)), c) if float(c) > f else (c, np.nextafter(c, np.float32(np.inf)))
+    if (float(lo) + float(hi)) / 2 == f and Decimal(f) != v:
+        c = hi if v > Decimal(f) else lo
+    return float(c)
+
+
+def read_z_list(path, nmax=999):
+    """initialize.f90:55-61: one redshift per record, read with (f8.4)
+    (read_f84) until the end of the file, at most nmax_redshift-1 = 999 of
+    them.  A blank record reads 0, as in the reference; a file with no
+    records is an error ('z_checkpoint.txt empty', initialize.f90:61)."""
     zs = []
     with open(path) as f:
         for line in f:
-            line = line.strip()
-            if line:
-                zs.append(float(line))
+            if len(zs) == nmax:
+                break
+            zs.append(read_f84(line.rstrip("\n")))
     if not zs:
         raise ValueError("%s empty" % os.path.basename(path))
     return zs

@@ -137,8 +137,12 @@ class Cube:
         self.final_step = False
         self.cur_checkpoint += 1  # main.f90:42
         # HALOFIND (initialize.f90:62-71, main.f90:43): the list as read from
-        # z_halofind.txt; the main loop starts at entry cur_checkpoint+1, and
-        # entries past the list read as 0 (the unset array)
+        # z_halofind.txt (or that file's path); the main loop starts at entry
+        # cur_checkpoint+1, and entries past the list read as 0 (the unset
+        # array)
+        if isinstance(z_halofind, (str, os.PathLike)):
+            from .checkpoint import read_z_list
+            z_halofind = read_z_list(z_halofind)
         self.z_halofind = list(z_halofind) if z_halofind else None
         self.cur_halofind = self.cur_checkpoint + 1
         self.halofind_step = False

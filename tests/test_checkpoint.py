@@ -73,12 +73,37 @@ def test_truncated_and_inconsistent_files_stop(tmp_path):
 
 
 def test_z_list_parse_empty_and_missing(tmp_path):
+    """A blank record reads 0 under (f8.4), so only a file without records
+    is empty; text the format cannot read stops the run."""
     p = tmp_path / "z_checkpoint.txt"
     p.write_text("  49.0000\n  10.0000\n\n   0.0000\n")
-    assert read_z_list(str(p)) == [49.0, 10.0, 0.0]
+    assert read_z_list(str(p)) == [49.0, 10.0, 0.0, 0.0]
+    b = tmp_path / "blank.txt"
+    b.write_text("\n\n")
+    assert read_z_list(str(b)) == [0.0, 0.0]
     e = tmp_path / "empty.txt"
-    e.write_text("\n\n")
+    e.write_text("")
     with pytest.raises(ValueError, match="empty"):
         read_z_list(str(e))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("49.0\n0.5 extra\n")
+    with pytest.raises(ValueError, match="f8.4"):
+        read_z_list(str(bad))
     with pytest.raises(FileNotFoundError):
         read_z_list(str(tmp_path / "missing.txt"))
+
+
+def test_host_reads_both_redshift_lists_from_files(tmp_path):
+    """Cube takes z_checkpoint.txt and z_halofind.txt as paths, read as the
+    reference reads them (initialize.f90:55-71)."""
+    from cube_nufluid_amd.driver import Cube
+    zc, zh = tmp_path / "z_checkpoint.txt", tmp_path / "z_halofind.txt"
+    zc.write_text("  49.0000\n   0.0000\n")
+    zh.write_text("   1.0000\n   0.5000\n")
+    host = Cube.__new__(Cube)
+    host._init_time(np.float32(0.02), str(zc), zh)
+    assert host.z_checkpoints == [49.0, 0.0] and host.z_halofind == [1.0, 0.5]
+    assert host.cur_checkpoint == 2 and host.cur_halofind == 3
+    zh.write_text("")
+    with pytest.raises(ValueError, match="z_halofind.txt empty"):
+        Cube.__new__(Cube)._init_time(np.float32(0.02), str(zc), str(zh))

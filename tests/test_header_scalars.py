@@ -111,3 +111,52 @@ def test_file_name_redshift_matches_fortran_f73(tmp_path):
     for z in (0.0025, 1.0005, 49.0):
         assert os.path.basename(halo_prefix("o", z)) + "halo_1.bin" == \
             os.path.basename(output_name("o", z, "halo"))
+
+
+ZLIST_PROGRAM = """program r
+  implicit none
+  real :: z(1000)
+  integer :: i, n, u
+  open(newunit=u, file='z.txt', status='old')
+  do i = 1, 999
+    read(u, end=71, fmt='(f8.4)') z(i)
+  enddo
+  71 n = i - 1
+  close(u)
+  print '(I6)', n
+  do i = 1, n
+    print '(I12)', transfer(z(i), 0)
+  end do
+end program
+"""
+
+
+@pytest.mark.parametrize("nlines", [400, 1005])
+def test_redshift_list_reads_as_fortran_f84(tmp_path, nlines):
+    """z_checkpoint.txt as initialize.f90:55-61 reads it, (f8.4) up to 999
+    records: only the first 8 columns count, blanks are ignored, a blank
+    record is 0, and a number without a decimal point has 4 implied
+    decimals.  A plain float() of each line gets many of these records wrong."""
+    from cube_nufluid_amd.checkpoint import read_z_list
+    if not os.path.exists(FC):
+        pytest.skip("no Fortran compiler")
+    rng = np.random.default_rng(nlines)
+    lines = ["", "0", "10", "  49.0000", "49.000000", "100.00000", "   12345", "1e1", "1.5+2", "1.5d-1", "-0.5",
+             "5.", " 2.25  "]
+    while len(lines) < nlines:
+        s = ("%.*f" % (int(rng.integers(0, 6)), rng.uniform(0, 200)) if rng.random() < 0.8
+             else str(int(rng.integers(0, 99999))))
+        line = " " * int(rng.integers(0, 3)) + s
+        if len(line) >= 8 and rng.random() < 0.2:
+            line += "  past column 8"
+        lines.append(line)
+    (tmp_path / "z.txt").write_text("".join(ln + "\n" for ln in lines))
+    (tmp_path / "r.f90").write_text(ZLIST_PROGRAM)
+    subprocess.run([FC, "-O0", "r.f90", "-o", "r"], cwd=tmp_path, check=True, capture_output=True)
+    out = subprocess.run(["./r"], cwd=tmp_path, capture_output=True, text=True, check=True).stdout.split()
+    n = int(out[0])
+    want = np.array(out[1:], dtype=np.int64).astype(np.uint32)
+    got = np.array(read_z_list(str(tmp_path / "z.txt")), dtype=np.float32).view(np.uint32)
+    assert n == len(got) == min(nlines, 999)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, [(lines[i], got[i], want[i]) for i in bad[:5]]
