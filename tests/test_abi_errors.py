@@ -5,7 +5,8 @@ rejects NULL with CUBE_ERR_ARG before it touches the device, as the
 reference's routines stop on bad input rather than run on.  The exceptions
 follow C's free(): cube_finalize(NULL) and cube_ens_destroy(NULL) succeed
 and do nothing.  The host-only routines (clock set-up, slot names, the last
-error) are checked on real arguments too."""
+error) are checked on real arguments too, and so is the ensemble engine's
+life cycle when no box is admitted."""
 import ctypes
 import os
 import re
@@ -110,3 +111,39 @@ def test_clock_init(lib):
     zfull = np.linspace(20, 0, cap).astype(np.float32)
     assert lib.cube_clock_init(ctypes.byref(c), zfull.ctypes.data_as(ctypes.c_void_p), cap) == CUBE_OK
     assert c.n_checkpoint == cap and np.float32(c.z_checkpoint[cap - 1]) == zfull[-1]
+
+
+def test_ensemble_lifecycle_without_boxes(lib):
+    """An engine given its own memory budget starts and stops on the host:
+    a request whose footprint exceeds the budget is refused with
+    CUBE_ERR_CAPACITY before it is queued, poll rejects a negative count,
+    and destroy returns with nothing left to run."""
+    from cube_nufluid_amd.params import read_kernel_tables
+    ep = _lib.EnsParams()
+    ep.device, ep.max_live, ep.mem_budget, ep.step_mode = 0, 2, 1 << 20, 0
+    e = ctypes.c_void_p()
+    assert lib.cube_ens_create(ctypes.byref(ep), ctypes.byref(e)) == CUBE_OK and e.value
+    try:
+        p = _lib.CubeParams()
+        p.nc, p.nnt, p.ncb, p.izipx, p.izipv, p.np_max = 32, 2, 6, 2, 2, 64 ** 3
+        p.omega_m, p.omega_l, p.wde, p.precision = 0.31, 0.69, -1.0, 4
+        assert lib.cube_footprint(ctypes.byref(p), 2) > ep.mem_budget
+        kf, kc = read_kernel_tables()
+        lnk = np.log(np.array([1e-3, 1e2]))
+        lnp = np.zeros(2)
+        ip = _lib.ICParams()
+        ip.seed, ip.np_nc, ip.nk, ip.a, ip.box = 1, 2, 2, 0.02, 100.0
+        ip.lnk, ip.lnp = _lib.ptr(lnk), _lib.ptr(lnp)
+        z = np.array([49.0, 0.0], np.float32)
+        rid = ctypes.c_int64(-7)
+        assert lib.cube_ens_submit(e, ctypes.byref(p), _lib.ptr(kf), _lib.ptr(kc), ctypes.byref(ip), _lib.ptr(z), 2,
+                                   ctypes.byref(rid)) == 2  # CUBE_ERR_CAPACITY
+        assert rid.value == -7
+        n = ctypes.c_int32(5)
+        assert lib.cube_ens_poll(e, None, -1, ctypes.byref(n)) == CUBE_ERR_ARG
+        assert lib.cube_ens_poll(e, None, 0, ctypes.byref(n)) == CUBE_OK and n.value == 0
+        q, l = ctypes.c_int32(-1), ctypes.c_int32(-1)
+        assert lib.cube_ens_pending(e, ctypes.byref(q), ctypes.byref(l)) == CUBE_OK and (q.value, l.value) == (0, 0)
+        assert lib.cube_ens_clock(e) >= 0.0
+    finally:
+        assert lib.cube_ens_destroy(e) == CUBE_OK
